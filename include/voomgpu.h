@@ -1,0 +1,205 @@
+/*
+ * voomgpu.h -- C ABI of the MI355X monodomain time step (libvoomgpu.so).
+ *
+ * One handle = one cardiac body (mixed P1-tet + Purkinje-bar mesh, one cell
+ * model family) and its Crank-Nicolson/Strang solver, resident in the HBM of
+ * the current HIP device.  All calls on one handle must come from one host
+ * thread; handles are independent.  Work is ordered on the handle's own HIP
+ * stream (vg_get_stream).  Every function returns VG_OK or a negative error
+ * code and never exits the process; vg_last_error(h) gives the message.
+ *
+ * Reference interfaces each entry point replaces (paths relative to
+ * voomEP/src/):
+ *   vg_create            CardiacBody::New (Body/CardiacBody.cc:975-1101) +
+ *                        CardiacBody::Compute (:78-187) + EPSolver::New
+ *                        (Solvers/EPSolver.cc:339-361) +
+ *                        EP_CrankNicholson::Initialize (EP_CrankNicholson.cc:5-76)
+ *                        + EPSolver::computeCapacitance (EPSolver.cc:214-219)
+ *   vg_set_stimulus_*    CardiacBody::setStimulusCurrent x4 (CardiacBody.cc:243-324)
+ *   vg_step              EP_CrankNicholson::Solve (EP_CrankNicholson.cc:78-156)
+ *   vg_get/set_voltage   EPSolver::InitializeVoltage / _Voltage view
+ *                        (EPSolver.h:187-307), EPSolver::WriteOutput payload
+ *                        (EPSolver.cc:116-136)
+ *   vg_get/set_states    CardiacBody::getStateVariables / setStateVariables
+ *                        (CardiacBody.cc:438-463)
+ *   vg_ecg_*             computeGradOneByR .. computeECG
+ *                        (Applications/CardiacEP/ComputeECG.h:30-189)
+ *   vg_assemble_csr      CardiacBody::getDiffusionMatrix / getCapacitanceMatrix /
+ *                        computeLumpedCapacitance (CardiacBody.cc:32-67,348-391)
+ *   vg_cells_*           IonicModel::Compute_Ion (IonicModel/IonicModel.h:33-34)
+ *                        on independent cells
+ */
+#ifndef VOOMGPU_H
+#define VOOMGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VG_OK 0
+#define VG_ERR_ARG (-1)
+#define VG_ERR_HIP (-2)
+#define VG_ERR_NOT_CONVERGED (-3)
+#define VG_ERR_NONFINITE (-4)
+#define VG_ERR_ALLOC (-5)
+#define VG_ERR_UNSUPPORTED (-6)
+
+/* Cell models, numbered like CardiacBody::New's CellModelType switch
+ * (CardiacBody.cc:988-1093). */
+#define VG_MODEL_LR 0
+#define VG_MODEL_TT 2
+
+/* Arithmetic of the hot path. */
+#define VG_PREC_FP64 0  /* every array and operation in FP64 */
+#define VG_PREC_MIXED 1 /* FP32 states, matrices and vectors; FP64 CG dots,
+                           residual norms and ECG sums */
+
+/* Ionic integrators. */
+#define VG_INTEG_REFERENCE 0   /* the reference's scheme: LR Rush-Larsen + FE;
+                                  tT backward-Euler gates + Newton (J^T solve) */
+#define VG_INTEG_RUSH_LARSEN 1 /* tT: Rush-Larsen gates + forward-Euler
+                                  concentrations, one rate evaluation */
+
+/* Reference behaviours kept behind flags (vg_model.quirks). */
+#define VG_QUIRK_V0_ZERO 1u       /* start LR/tT at V = 0 like the driver
+                                     (EPSimulator.cc:463, IonicModel.h:42) */
+#define VG_QUIRK_PMJ_STICKY 2u    /* never reset the PMJ override
+                                     (Mesh/MixedMesh3D.cc:14,31-34) */
+#define VG_QUIRK_BAR_1X_VOLUME 4u /* integrate bars with pi r^2 L instead of
+                                     the reference's 2 pi r^2 L
+                                     (MixedMesh3D.cc:97-107) */
+
+/* Per-node conductance scalings of the ten Tusscher model (vg_model). */
+#define VG_TT_SCALE_GNA 0
+#define VG_TT_SCALE_GKR 1
+#define VG_TT_SCALE_GKS 2
+#define VG_TT_SCALE_GCAL 3
+#define VG_TT_SCALE_GTO 4
+#define VG_TT_NSCALE 5
+
+typedef struct vg_handle vg_handle;
+typedef struct vg_cells vg_cells;
+
+/* Mesh, 0-based.  conn has 4 slots per element; bars (elem_nn 2) use the
+ * first two.  fibers: 9 numbers (f, s, n) per element or NULL (isotropic).
+ * tissue: per-node type (11/12 = LV/RV Purkinje) or NULL.  PMJ overrides are
+ * keyed by node like the reference's .angle file: an element whose first node
+ * is a PMJ node takes cosines and length from pmj_data (4 per PMJ). */
+typedef struct {
+  int64_t n_nodes;
+  const double *xyz;
+  int64_t n_elems;
+  const int8_t *elem_nn;
+  const int32_t *conn;
+  const double *fibers;
+  const int8_t *tissue;
+  int64_t n_pmj;
+  const int32_t *pmj_node;
+  const double *pmj_data;
+  double radius; /* bar radius (cm), TWODRADIUS in the reference decks */
+} vg_mesh;
+
+typedef struct {
+  int cell_model;     /* VG_MODEL_* */
+  int precision;      /* VG_PREC_* */
+  int integrator;     /* VG_INTEG_* */
+  double D[3];        /* tissue diffusivity along f, s, n (cm^2/ms) */
+  double Dp[3];       /* Purkinje diffusivity */
+  double DpLeft[3];   /* LV Purkinje diffusivity */
+  double Cm;          /* uF/cm^2 */
+  double dt;          /* diffusion step (ms) */
+  double dt_ionic;    /* ionic sub-step (ms) */
+  double cg_tol;      /* ||r||/||r0|| */
+  int cg_maxit;
+  unsigned quirks;    /* VG_QUIRK_* */
+  int n_param_sets;   /* tT: number of scaling sets (0 = all ones) */
+  const double *param_scales;    /* [n_param_sets * VG_TT_NSCALE] */
+  const uint16_t *param_index;   /* [n_nodes] set of each node, or NULL */
+  int sell_sigma;     /* rows sorted by length in windows of this many rows
+                         (0 or 1 = keep the mesh order) */
+} vg_model;
+
+typedef struct {
+  int64_t steps;
+  int64_t cg_iterations;      /* summed over all half-steps */
+  int64_t cg_solves;
+  int64_t cg_not_converged;   /* solves that hit cg_maxit */
+  int64_t newton_iterations;  /* tT reference integrator only (device sum) */
+  int64_t n_nodes;
+  int64_t nnz;
+  int64_t sell_padded_nnz;
+  double last_cg_relres;      /* ||r||/||r0|| of the last solve */
+  double sim_time;            /* ms */
+} vg_stats;
+
+const char *vg_version(void);
+
+/* On failure *out may still receive a handle that carries the message for
+ * vg_last_error; release it with vg_destroy either way. */
+int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out);
+void vg_destroy(vg_handle *h);
+const char *vg_last_error(const vg_handle *h);
+int vg_get_stream(vg_handle *h, void **hip_stream);
+
+int vg_set_stimulus_nodes(vg_handle *h, const int32_t *nodes, int64_t n, double istim);
+int vg_set_stimulus_sphere(vg_handle *h, const double center[3], double radius,
+                           double istim);
+int vg_set_stimulus_axis(vg_handle *h, int axis, double limit, double istim);
+int vg_set_stimulus_plane(vg_handle *h, const double origin[3], const double normal[3],
+                          double istim);
+int vg_clear_stimulus(vg_handle *h);
+
+int vg_step(vg_handle *h, int nsteps);
+int vg_synchronize(vg_handle *h);
+
+int vg_get_voltage(vg_handle *h, double *V);
+int vg_set_voltage(vg_handle *h, const double *V);
+int vg_get_states(vg_handle *h, int *nvar, double *states);
+int vg_set_states(vg_handle *h, int nvar, const double *states);
+int vg_get_lumped_capacitance(vg_handle *h, double *CL);
+
+int vg_activation_enable(vg_handle *h, double threshold_mV);
+int vg_get_activation(vg_handle *h, double *t_act);
+
+/* The lead field is built from the same mesh the handle was created with
+ * (ComputeECG builds its own Mesh the same way, ComputeECG.cc:76-79). */
+int vg_ecg_setup(vg_handle *h, const vg_mesh *mesh, const double *electrodes,
+                 int n_electrodes, const double D_ecg[3], double radius);
+int vg_ecg(vg_handle *h, double *out);
+int vg_ecg_record(vg_handle *h, int steps_per_frame, int max_frames);
+int vg_ecg_frames(vg_handle *h, double *out, int *n_frames);
+
+int vg_profile_enable(vg_handle *h, int enable);
+int vg_profile_get(vg_handle *h, int kernel_class, double *total_ms, int64_t *launches);
+#define VG_KC_RHS 0
+#define VG_KC_CG_SPMV 1
+#define VG_KC_CG_UPDATE 2
+#define VG_KC_IONIC 3
+#define VG_KC_APPLY 4
+#define VG_KC_COUNT 5
+
+int vg_stats_get(vg_handle *h, vg_stats *out);
+
+/* Host-only assembly (no device needed): CSR of K (with Cm), M and the
+ * lumped capacitance.  Call with NULL arrays to get n and nnz first. */
+int vg_assemble_csr(const vg_mesh *mesh, const vg_model *model, int64_t *n,
+                    int64_t *nnz, int64_t *row_ptr, int32_t *col, double *K,
+                    double *M, double *CL);
+
+/* Independent cells (IonicModel adapter): one Compute_Ion per cell per call,
+ * run on the device. */
+int vg_cells_create(int cell_model, int precision, int integrator, int64_t n_cells,
+                    vg_cells **out);
+void vg_cells_destroy(vg_cells *c);
+int vg_cells_set_scales(vg_cells *c, int n_param_sets, const double *param_scales,
+                        const uint16_t *param_index);
+int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V,
+                         const double *istim, double *dVdt);
+int vg_cells_get_states(vg_cells *c, int *nvar, double *states);
+int vg_cells_set_states(vg_cells *c, int nvar, const double *states);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

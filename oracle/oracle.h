@@ -1,0 +1,139 @@
+/*
+ * oracle.h -- CPU FP64 restatement of voomEP's monodomain time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (ucla_cmg_amd/, include/)
+ * may include, link or call this code.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, and only as the checker or as the
+ * timed CPU baseline.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * voomEP/src/).  The reference itself cannot be compiled in this image: its
+ * ionic translation units include voom.h, which pulls in Blitz++ and tvmet
+ * (absent), and the solver needs Trilinos Epetra/AztecOO (absent).
+ *
+ * Parity pins (what checks each function beyond its own formula):
+ *   or_assemble        closed-form P1 K_e = V G D G^T, M_e = V/20 (1+d_ab),
+ *                      C_L = V/4, K*1 = 0, symmetry, sum(M) = Cm*Vol, bar
+ *                      closed forms (tests/test_oracle_fe.py)
+ *   or_pcg             dense numpy/scipy solve (tests/test_oracle_cg.py)
+ *   or_cn_step         CN eigen-mode decay factor (tests/test_oracle_cn.py)
+ *   or_tt_rates        charge conservation across dV/dt and the six
+ *                      concentration equations; near-stationarity at the
+ *                      reference's initial state; constants and initial
+ *                      state against tests/golden/ionic_tables.json
+ *   or_tt_compute_ion  backward-Euler gate closed form; Newton fixed point
+ *                      solves x - x^n - dt f(x) = 0; J^T linear solve against
+ *                      numpy.linalg.solve
+ *   or_lr_compute_ion  Rush-Larsen gate closed form at fixed V; Ca_i forward
+ *                      Euler closed form; constants against the golden table.
+ *                      The individual alpha/beta rate expressions of LR are
+ *                      parity unpinned beyond rest/upstroke markers.
+ *   or_ecg_direct      brute-force numpy quadrature sum, uniform V -> 0,
+ *                      dipole far-field scaling
+ */
+#ifndef VOOM_ORACLE_H
+#define VOOM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mixed mesh: elements with 4 nodes (P1 tets) or 2 nodes (Purkinje bars),
+ * 0-based connectivity, 4 slots per element (bars use the first 2). */
+typedef struct {
+  int64_t n_nodes;
+  const double *xyz;        /* [3*n_nodes] */
+  int64_t n_elems;
+  const int8_t *elem_nn;    /* [n_elems] 4 or 2 */
+  const int32_t *conn;      /* [4*n_elems] */
+  const double *fibers;     /* [9*n_elems] rows f,s,n per element, or NULL (isotropic) */
+  const int8_t *tissue;     /* [n_nodes] tissue type (11 = LV Purkinje) or NULL */
+  int64_t n_pmj;
+  const int32_t *pmj_node;  /* [n_pmj] 0-based node id keyed like MixedMesh3D _orient */
+  const double *pmj_data;   /* [4*n_pmj] cosines(3), length */
+  int pmj_sticky;           /* 1 = reproduce the never-reset PMJ flag */
+  double radius;            /* bar radius (cm) */
+  double D[3], Dp[3], DpLeft[3];
+  double Cm;
+} or_mesh;
+
+/* CSR with K (stiffness, Cm included as in CardiacBody::Compute) and M
+ * (consistent mass without Cm) sharing one sparsity pattern. */
+typedef struct {
+  int64_t n;
+  int64_t nnz;
+  int64_t *row_ptr;   /* [n+1] */
+  int32_t *col;       /* [nnz] */
+  double *K;          /* [nnz] */
+  double *M;          /* [nnz] */
+  double *CL;         /* [n] lumped capacitance */
+} or_csr;
+
+int or_assemble(const or_mesh *m, or_csr *out);
+void or_csr_free(or_csr *c);
+
+/* Element-level access for closed-form tests. */
+void or_tet_element(const double x[4][3], const double Dmat[3][3], double Cm,
+                    double Ke[4][4], double Me[4][4], double CLe[4]);
+
+/* Jacobi PCG, x0 = x on entry, stop at ||r||2 <= tol*||r0||2 (AztecOO AZ_r0). */
+int or_pcg(int64_t n, const int64_t *rp, const int32_t *col, const double *A,
+           const double *b, double *x, double tol, int maxit, double *relres);
+
+/* ---- ionic models ------------------------------------------------------ */
+enum { OR_LR = 0, OR_TT = 2 };
+
+#define OR_LR_NSTATE 8
+#define OR_TT_NSTATE 19
+#define OR_TT_NCONST 53
+#define OR_TT_NALG 70
+
+void or_lr_init(double *s);
+double or_lr_xi(void);
+double or_lr_compute_ion(double *s, double Cm, double dt, double V, double istim);
+
+void or_tt_constants(double *c);
+void or_tt_init(double *s);
+void or_tt_rates(const double *c, double *state, double *alg, double *rates);
+double or_tt_compute_ion(const double *c, double *s, double Cm, double dt,
+                         double V, double istim, int *newton_iters);
+/* One Newton linear solve exactly as the reference forms it (J^T). */
+void or_tt_newton_system(const double *c, double *s, const double *stateN,
+                         const double *oldRates, double dt, double alg12,
+                         double Jac[36], double B[6]);
+int or_solve6_lapack_layout(const double Jac[36], double B[6]);
+
+/* ---- whole-step driver ------------------------------------------------- */
+typedef struct {
+  int model;            /* OR_LR or OR_TT */
+  int64_t n;
+  const or_csr *csr;
+  double dt, dt_ion, Cm, cg_tol;
+  int cg_maxit;
+  double *V;            /* [n] */
+  double *states;       /* [n*nstate] AoS like getInternalParameters */
+  double *istim;        /* [n] uA/cm^3 */
+  const double *tt_const; /* [n_param*53] or NULL -> reference constants */
+  const uint16_t *param_idx; /* [n] or NULL */
+  /* work */
+  double *A, *B, *rhs, *r, *z, *p, *q, *diag;
+  int64_t cg_iters_total;
+  int64_t newton_iters_total;
+  int nthreads;
+} or_sim;
+
+int or_sim_init(or_sim *s);
+void or_sim_free(or_sim *s);
+/* One EP_CrankNicholson::Solve. */
+int or_cn_step(or_sim *s);
+
+/* ---- pseudo-ECG (ComputeECG.h:30-189) ---------------------------------- */
+void or_ecg_direct(const or_mesh *m, const double Decg[3], double radius,
+                   const double *V, int ne, const double *electrodes,
+                   double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,988 @@
+// vg_api.hip -- the C ABI of libvoomgpu: handle life cycle, the Strang/CN
+// time step (EP_CrankNicholson::Solve, Solvers/EP_CrankNicholson.cc:78-156),
+// stimulus, state and ECG access.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/voomgpu.h"
+#include "vg_host.hpp"
+#include "vg_kernels.hpp"
+
+using namespace vg;
+
+namespace {
+
+// Rest state of the Luo-Rudy model used as the default initial state: the
+// reference's initial gates (LR.cc:19-25) run unstimulated for 10 s from
+// -84 mV with dt = 0.01 ms.  Order m, h, j, d, f, X, Ca_i; then V.
+constexpr double LR_REST[7] = {0.0016534680667748946, 0.9834762339347045, 0.9896274219694648,
+                               0.0029656171926269563, 0.999981405275416, 0.005617388831370626,
+                               0.00016111399466747764};
+constexpr double LR_REST_V = -84.59413360428096;
+
+const char *hip_str(hipError_t e) { return hipGetErrorString(e); }
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t alloc(size_t b) {
+    bytes = b;
+    return hipMalloc(&p, b ? b : 1);
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class X> X *as() const { return static_cast<X *>(p); }
+};
+
+struct ProfileClass {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+int nstate_of(int model) { return model == VG_MODEL_LR ? LR_NSTATE : TT_NSTATE; }
+
+// Ionic sub-step sequence of EP_CrankNicholson::Solve (:114-143).
+bool substeps(double dT, double dt_ion, SubSteps &s) {
+  s.n = 0;
+  double total = 0.0;
+  double dt = (dt_ion < dT) ? dt_ion : dT;
+  while (total < dT) {
+    if (s.n >= MAX_SUB) return false;
+    s.dt[s.n++] = dt;
+    total += dt;
+    dt = (total + dt < dT) ? dt : dT - total;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct vg_handle {
+  std::string err;
+  int prec = VG_PREC_FP64, model = VG_MODEL_TT, integ = VG_INTEG_REFERENCE;
+  unsigned quirks = 0;
+  double dt = 0.01, dt_ion = 0.01, Cm = 1.0, tol = 1e-7;
+  int maxit = 1000;
+  int64_t n = 0, n_alloc = 0, n_slices = 0, nnz = 0, padded = 0;
+  size_t tsz = 8;  // sizeof(T)
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::vector<int32_t> new2old, old2new;
+  std::vector<double> xyz;        // original order
+  std::vector<char> purkinje;     // original order
+  std::vector<double> CL;         // original order
+  std::vector<double> istim_host; // original order
+  bool stim_dirty = false, stim_active = false;
+  SubSteps sub{};
+  // device
+  DevBuf off, width, col, Ahat, Krhs, sinv, diag;
+  DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
+  DevBuf cg, part, counter, newton;
+  DevBuf H, ecg_out, ecg_frames;
+  CgState *h_cg = nullptr;  // pinned
+  int nsets = 1;
+  int cg_guess = 8;
+  // activation
+  bool at_enabled = false;
+  double at_thr = -40.0;
+  // ecg
+  int ne = 0;
+  int ecg_every = 0, ecg_max = 0, ecg_count = 0;
+  // stats
+  int64_t steps = 0, cg_iters = 0, cg_solves = 0, cg_notconv = 0;
+  double last_relres = 0.0, t = 0.0;
+  bool profiling = false;
+  ProfileClass prof[VG_KC_COUNT];
+  std::vector<hipEvent_t> ev_pool;
+
+  int fail(int code, const std::string &m) {
+    err = m;
+    return code;
+  }
+  int hipfail(hipError_t e, const char *what) {
+    err = std::string(what) + ": " + hip_str(e);
+    return VG_ERR_HIP;
+  }
+  ~vg_handle() {
+    for (auto &c : prof)
+      for (auto &pr : c.pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+      }
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (h_cg) (void)hipHostFree(h_cg);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct vg_cells {
+  int prec, model, integ;
+  int64_t n, n_alloc;
+  DevBuf states, scales, pidx, V, stim, dvdt;
+  int nsets = 1;
+};
+
+#define HIPCHK(h, call)                                   \
+  do {                                                    \
+    hipError_t _e = (call);                               \
+    if (_e != hipSuccess) return (h)->hipfail(_e, #call); \
+  } while (0)
+
+namespace {
+
+// Launch bracketing for vg_profile_*: a pair of events per launch.
+struct ProfScope {
+  vg_handle *h;
+  int kc;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(vg_handle *hh, int k) : h(hh), kc(k) {
+    if (!h->profiling) return;
+    if (h->ev_pool.size() < 2) {
+      for (int i = 0; i < 64; i++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) == hipSuccess) h->ev_pool.push_back(e);
+      }
+    }
+    if (h->ev_pool.size() < 2) return;
+    a = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    b = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    (void)hipEventRecord(a, h->stream);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    (void)hipEventRecord(b, h->stream);
+    h->prof[kc].pending.emplace_back(a, b);
+  }
+};
+
+void drain_profile(vg_handle *h) {
+  for (auto &c : h->prof) {
+    for (auto &pr : c.pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        c.total_ms += ms;
+        c.launches += 1;
+      }
+      h->ev_pool.push_back(pr.first);
+      h->ev_pool.push_back(pr.second);
+    }
+    c.pending.clear();
+  }
+}
+
+Reduce red_of(vg_handle *h) { return Reduce{h->part.as<double>(), h->counter.as<unsigned>()}; }
+
+unsigned blocks_nodes(const vg_handle *h) { return (unsigned)(h->n_alloc / BLK); }
+unsigned blocks_slices(const vg_handle *h) { return (unsigned)((h->n_slices + WAVES - 1) / WAVES); }
+
+template <class T>
+int launch_rhs(vg_handle *h, const T *V) {
+  ProfScope ps(h, VG_KC_RHS);
+  hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
+                     V, h->r.as<T>(), h->diag.as<T>(), h->n_slices, h->cg.as<CgState>(),
+                     red_of(h), h->tol * h->tol, h->maxit);
+  return VG_OK;
+}
+
+template <class T>
+void launch_cg_pair(vg_handle *h) {
+  {
+    ProfScope ps(h, VG_KC_CG_SPMV);
+    hipLaunchKernelGGL(k_cg_spmv_dots<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(),
+                       h->Ahat.as<T>(), h->r.as<T>(), h->w.as<T>(), h->n_slices,
+                       h->cg.as<CgState>(), red_of(h));
+  }
+  {
+    ProfScope ps(h, VG_KC_CG_UPDATE);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
+                       h->x.as<T>(), h->r.as<T>(), h->p.as<T>(), h->s.as<T>(), h->w.as<T>(),
+                       h->diag.as<T>(), h->n_alloc, h->cg.as<CgState>(), red_of(h));
+  }
+}
+
+// Runs CG to convergence: launches the expected number of iterations, then
+// reads the device state back and continues if needed.
+template <class T>
+int cg_solve(vg_handle *h) {
+  int launched = 0;
+  int batch = h->cg_guess;
+  for (;;) {
+    for (int k = 0; k < batch; k++) launch_cg_pair<T>(h);
+    launched += batch;
+    HIPCHK(h, hipMemcpyAsync(h->h_cg, h->cg.p, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->h_cg->done) break;
+    if (launched > h->maxit + 2) break;
+    batch = 4;
+  }
+  const CgState &c = *h->h_cg;
+  h->cg_iters += c.iter;
+  h->cg_solves += 1;
+  h->last_relres = c.rho0 > 0 ? std::sqrt(c.rho / c.rho0) : 0.0;
+  if (c.notconv || !c.done) h->cg_notconv += 1;
+  h->cg_guess = std::max(1, c.iter + 1);
+  if (c.notconv && !std::isfinite(c.rho)) return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
+  return VG_OK;
+}
+
+template <class T, int MODEL, int INTEG>
+void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
+  IonicArgs a{h->n, h->n_alloc, h->nsets, h->Cm, 500};
+  hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
+                     Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
+                     h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
+                     h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
+                     h->scales.as<T>(), h->sub, a, h->newton.as<unsigned long long>(),
+                     h->cg.as<CgState>());
+}
+
+template <class T>
+void launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
+  ProfScope ps(h, VG_KC_IONIC);
+  if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+  else if (h->integ == VG_INTEG_REFERENCE)
+    launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+  else launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN>(h, Vin, x, Vout);
+}
+
+template <class T>
+void launch_apply(vg_handle *h, double t0, bool ecg_frame) {
+  ProfScope ps(h, VG_KC_APPLY);
+  const T thr = (T)h->at_thr;
+  double *ecg_dst = nullptr;
+  if (ecg_frame) ecg_dst = h->ecg_frames.as<double>() + (size_t)h->ecg_count * h->ne;
+  const T *Hm = h->H.p ? h->H.as<T>() : nullptr;
+#define VG_APPLY(AT, ECG)                                                                        \
+  hipLaunchKernelGGL((k_apply<T, AT, ECG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,      \
+                     h->Vb.as<T>(), h->x.as<T>(), h->sinv.as<T>(), h->Va.as<T>(), h->n,          \
+                     h->at.as<T>(), thr, t0, h->dt, Hm, h->ne, h->n_alloc, red_of(h), ecg_dst,   \
+                     h->cg.as<CgState>())
+  if (h->at_enabled) {
+    if (ecg_frame) VG_APPLY(true, true);
+    else VG_APPLY(true, false);
+  } else {
+    if (ecg_frame) VG_APPLY(false, true);
+    else VG_APPLY(false, false);
+  }
+#undef VG_APPLY
+}
+
+int upload_stim(vg_handle *h) {
+  if (!h->stim_dirty) return VG_OK;
+  h->stim_active = false;
+  for (double v : h->istim_host)
+    if (v != 0.0) {
+      h->stim_active = true;
+      break;
+    }
+  const int64_t n = h->n;
+  if (h->tsz == 8) {
+    std::vector<double> buf(h->n_alloc, 0.0);
+    for (int64_t r = 0; r < n; r++) buf[r] = h->istim_host[h->new2old[r]];
+    HIPCHK(h, hipMemcpyAsync(h->istim.p, buf.data(), buf.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> buf(h->n_alloc, 0.0f);
+    for (int64_t r = 0; r < n; r++) buf[r] = (float)h->istim_host[h->new2old[r]];
+    HIPCHK(h, hipMemcpyAsync(h->istim.p, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->stim_dirty = false;
+  return VG_OK;
+}
+
+template <class T>
+int step_t(vg_handle *h) {
+  int rc = upload_stim(h);
+  if (rc) return rc;
+  const double t0 = h->t;
+  // Step 1: half-step diffusion, C_m M + dT/4 K against C_m M - dT/4 K.
+  launch_rhs<T>(h, h->Va.as<T>());
+  if ((rc = cg_solve<T>(h))) return rc;
+  // Step 2: ionic update over dT in the reference's sub-steps, fused with
+  // V = V_a + D^-1/2 x.
+  launch_ionic<T>(h, h->Va.as<T>(), h->x.as<T>(), h->Vb.as<T>());
+  // Step 3: second half-step.
+  launch_rhs<T>(h, h->Vb.as<T>());
+  if ((rc = cg_solve<T>(h))) return rc;
+  h->steps += 1;
+  h->t = h->steps * h->dt;
+  const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
+                     h->ecg_count < h->ecg_max;
+  launch_apply<T>(h, t0, frame);
+  if (frame) h->ecg_count += 1;
+  return VG_OK;
+}
+
+template <class T>
+int upload_vec(vg_handle *h, void *dst, const double *src_orig, double fill) {
+  std::vector<T> buf(h->n_alloc, (T)fill);
+  for (int64_t r = 0; r < h->n; r++) buf[r] = (T)src_orig[h->new2old[r]];
+  HIPCHK(h, hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return VG_OK;
+}
+
+template <class T>
+int download_vec(vg_handle *h, const void *src, double *dst_orig) {
+  std::vector<T> buf(h->n_alloc);
+  HIPCHK(h, hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int64_t r = 0; r < h->n; r++) dst_orig[h->new2old[r]] = (double)buf[r];
+  return VG_OK;
+}
+
+int upload(vg_handle *h, void *dst, const double *src_orig, double fill = 0.0) {
+  return h->tsz == 8 ? upload_vec<double>(h, dst, src_orig, fill)
+                     : upload_vec<float>(h, dst, src_orig, fill);
+}
+int download(vg_handle *h, const void *src, double *dst_orig) {
+  return h->tsz == 8 ? download_vec<double>(h, src, dst_orig) : download_vec<float>(h, src, dst_orig);
+}
+
+template <class T>
+int upload_raw(vg_handle *h, DevBuf &b, const std::vector<double> &v) {
+  std::vector<T> buf(v.begin(), v.end());
+  HIPCHK(h, b.alloc(buf.size() * sizeof(T)));
+  HIPCHK(h, hipMemcpy(b.p, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VG_OK;
+}
+
+int set_states_impl(vg_handle *h, int nvar, const double *st) {
+  const int ns = nstate_of(h->model);
+  if (nvar != ns) return h->fail(VG_ERR_ARG, "state count mismatch");
+  // Device keeps every state except V (LR slot 7, tT slot 0).
+  const int vslot = h->model == VG_MODEL_LR ? 7 : 0;
+  const int64_t n = h->n, na = h->n_alloc;
+  std::vector<double> soa((size_t)(ns - 1) * na, 0.0);
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t o = h->new2old[r];
+    int k2 = 0;
+    for (int k = 0; k < ns; k++) {
+      if (k == vslot) continue;
+      soa[(size_t)k2 * na + r] = st[o * ns + k];
+      k2++;
+    }
+  }
+  // Padding lanes get the first node's states so every lane computes finite values.
+  for (int k = 0; k < ns - 1; k++)
+    for (int64_t r = n; r < na; r++) soa[(size_t)k * na + r] = soa[(size_t)k * na];
+  if (h->tsz == 8) {
+    HIPCHK(h, hipMemcpy(h->states.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(soa.begin(), soa.end());
+    HIPCHK(h, hipMemcpy(h->states.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return VG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vg_version(void) { return "voomgpu 0.1 (gfx950)"; }
+
+const char *vg_last_error(const vg_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int vg_assemble_csr(const vg_mesh *mesh, const vg_model *model, int64_t *n, int64_t *nnz,
+                    int64_t *row_ptr, int32_t *col, double *K, double *M, double *CL) {
+  if (!mesh || !model || !n || !nnz) return VG_ERR_ARG;
+  HostCsr c;
+  if (!assemble_csr(*mesh, *model, c).empty()) return VG_ERR_ARG;
+  *n = c.n;
+  *nnz = (int64_t)c.col.size();
+  if (row_ptr) std::copy(c.row_ptr.begin(), c.row_ptr.end(), row_ptr);
+  if (col) std::copy(c.col.begin(), c.col.end(), col);
+  if (K) std::copy(c.K.begin(), c.K.end(), K);
+  if (M) std::copy(c.M.begin(), c.M.end(), M);
+  if (CL) std::copy(c.CL.begin(), c.CL.end(), CL);
+  return VG_OK;
+}
+
+int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
+  if (!out) return VG_ERR_ARG;
+  *out = nullptr;
+  std::unique_ptr<vg_handle> h(new vg_handle());
+  if (!mesh || !model) return VG_ERR_ARG;
+  if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_TT) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_UNSUPPORTED, "cell model not supported");
+  }
+  if (model->precision != VG_PREC_FP64 && model->precision != VG_PREC_MIXED) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "unknown precision");
+  }
+  if (!(model->dt > 0) || !(model->dt_ionic > 0) || !(model->Cm > 0) || !(model->cg_tol > 0) ||
+      model->cg_maxit < 1) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "dt, dt_ionic, Cm, cg_tol and cg_maxit must be positive");
+  }
+  h->prec = model->precision;
+  h->model = model->cell_model;
+  h->integ = model->integrator;
+  h->quirks = model->quirks;
+  h->dt = model->dt;
+  h->dt_ion = model->dt_ionic;
+  h->Cm = model->Cm;
+  h->tol = model->cg_tol;
+  h->maxit = model->cg_maxit;
+  h->tsz = h->prec == VG_PREC_FP64 ? 8 : 4;
+  vg_handle *hp = h.get();
+  if (!substeps(h->dt, h->dt_ion, h->sub)) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "too many ionic sub-steps per step");
+  }
+  HostCsr csr;
+  std::string e = assemble_csr(*mesh, *model, csr);
+  if (!e.empty()) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, e);
+  }
+  HostSell sell;
+  e = build_sell(csr, h->dt, h->Cm, model->sell_sigma, sell);
+  if (!e.empty()) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, e);
+  }
+  h->n = csr.n;
+  h->nnz = (int64_t)csr.col.size();
+  h->n_slices = sell.n_slices;
+  h->padded = sell.padded;
+  h->n_alloc = ((h->n + BLK - 1) / BLK) * BLK;
+  h->new2old = sell.new2old;
+  h->old2new = sell.old2new;
+  h->CL = csr.CL;
+  h->xyz.assign(mesh->xyz, mesh->xyz + 3 * mesh->n_nodes);
+  h->purkinje.assign(h->n, 0);
+  if (mesh->tissue)
+    for (int64_t i = 0; i < h->n; i++) h->purkinje[i] = (mesh->tissue[i] == 11 || mesh->tissue[i] == 12);
+  h->istim_host.assign(h->n, 0.0);
+
+  int rc;
+  {
+    hipError_t he = hipGetDevice(&h->device);
+    if (he != hipSuccess) {
+      *out = h.release();
+      return (*out)->hipfail(he, "hipGetDevice");
+    }
+    he = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+      *out = h.release();
+      return (*out)->hipfail(he, "hipStreamCreate");
+    }
+  }
+  *out = h.release();
+  vg_handle *H_ = *out;
+  (void)hp;
+  const size_t T = H_->tsz;
+  const int64_t na = H_->n_alloc;
+  // Operator.
+  HIPCHK(H_, H_->off.alloc(sell.offset.size() * 8));
+  HIPCHK(H_, hipMemcpy(H_->off.p, sell.offset.data(), sell.offset.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->width.alloc(sell.width.size() * 4));
+  HIPCHK(H_, hipMemcpy(H_->width.p, sell.width.data(), sell.width.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->col.alloc(sell.col.size() * 4));
+  HIPCHK(H_, hipMemcpy(H_->col.p, sell.col.data(), sell.col.size() * 4, hipMemcpyHostToDevice));
+  std::vector<double> sinv_pad(na, 0.0), diag_pad(na, 0.0);
+  std::copy(sell.sinv.begin(), sell.sinv.begin() + std::min<int64_t>(na, sell.sinv.size()), sinv_pad.begin());
+  std::copy(sell.diag.begin(), sell.diag.begin() + std::min<int64_t>(na, sell.diag.size()), diag_pad.begin());
+  if (T == 8) {
+    if ((rc = upload_raw<double>(H_, H_->Ahat, sell.A_hat))) return rc;
+    if ((rc = upload_raw<double>(H_, H_->Krhs, sell.K_rhs))) return rc;
+    if ((rc = upload_raw<double>(H_, H_->sinv, sinv_pad))) return rc;
+    if ((rc = upload_raw<double>(H_, H_->diag, diag_pad))) return rc;
+  } else {
+    if ((rc = upload_raw<float>(H_, H_->Ahat, sell.A_hat))) return rc;
+    if ((rc = upload_raw<float>(H_, H_->Krhs, sell.K_rhs))) return rc;
+    if ((rc = upload_raw<float>(H_, H_->sinv, sinv_pad))) return rc;
+    if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
+  }
+  // Vectors.
+  for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at}) {
+    HIPCHK(H_, b->alloc(na * T));
+    HIPCHK(H_, hipMemset(b->p, 0, na * T));
+  }
+  const int ns = nstate_of(H_->model);
+  HIPCHK(H_, H_->states.alloc((size_t)(ns - 1) * na * T));
+  // Reduction scratch: one FP64 partial per block and value.
+  const int64_t maxblocks = std::max<int64_t>(na / BLK, (H_->n_slices + WAVES - 1) / WAVES) + 1;
+  HIPCHK(H_, H_->part.alloc((size_t)maxblocks * MAX_ECG * 8));
+  HIPCHK(H_, H_->counter.alloc(sizeof(unsigned)));
+  HIPCHK(H_, hipMemset(H_->counter.p, 0, sizeof(unsigned)));
+  HIPCHK(H_, H_->cg.alloc(sizeof(CgState)));
+  HIPCHK(H_, hipMemset(H_->cg.p, 0, sizeof(CgState)));
+  HIPCHK(H_, H_->newton.alloc(sizeof(unsigned long long)));
+  HIPCHK(H_, hipMemset(H_->newton.p, 0, sizeof(unsigned long long)));
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_cg, sizeof(CgState), hipHostMallocDefault));
+  // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi).
+  {
+    int nsets = model->n_param_sets > 0 ? model->n_param_sets : 1;
+    std::vector<double> sc((size_t)nsets * VG_TT_NSCALE, 1.0);
+    if (model->n_param_sets > 0) {
+      if (!model->param_scales) return H_->fail(VG_ERR_ARG, "param_scales is NULL");
+      std::copy(model->param_scales, model->param_scales + sc.size(), sc.begin());
+    }
+    H_->nsets = nsets;
+    if (T == 8) {
+      if ((rc = upload_raw<double>(H_, H_->scales, sc))) return rc;
+    } else {
+      if ((rc = upload_raw<float>(H_, H_->scales, sc))) return rc;
+    }
+    if (model->param_index) {
+      std::vector<uint16_t> pi(na, 0);
+      for (int64_t r = 0; r < H_->n; r++) {
+        const uint16_t v = model->param_index[H_->new2old[r]];
+        if (v >= nsets) return H_->fail(VG_ERR_ARG, "param_index out of range");
+        pi[r] = v;
+      }
+      HIPCHK(H_, H_->pidx.alloc(na * 2));
+      HIPCHK(H_, hipMemcpy(H_->pidx.p, pi.data(), na * 2, hipMemcpyHostToDevice));
+    }
+  }
+  // Initial states and voltage: the model's own initial table, with V from it
+  // (LR: its rest state) unless the driver's V0 = 0 is requested.
+  {
+    std::vector<double> st((size_t)H_->n * ns);
+    std::vector<double> V0(H_->n);
+    double v0;
+    std::vector<double> one(ns);
+    if (H_->model == VG_MODEL_LR) {
+      if (H_->quirks & VG_QUIRK_V0_ZERO) {
+        for (int k = 0; k < ns; k++) one[k] = LR_INIT[k];
+        v0 = 0.0;
+      } else {
+        for (int k = 0; k < 7; k++) one[k] = LR_REST[k];
+        one[7] = LR_REST_V;
+        v0 = LR_REST_V;
+      }
+    } else {
+      for (int k = 0; k < ns; k++) one[k] = TT_INIT[k];
+      v0 = (H_->quirks & VG_QUIRK_V0_ZERO) ? 0.0 : TT_INIT[0];
+    }
+    for (int64_t i = 0; i < H_->n; i++) {
+      std::copy(one.begin(), one.end(), st.begin() + i * ns);
+      V0[i] = v0;
+    }
+    if ((rc = set_states_impl(H_, ns, st.data()))) return rc;
+    if ((rc = upload(H_, H_->Va.p, V0.data(), v0))) return rc;
+    if ((rc = upload(H_, H_->Vb.p, V0.data(), v0))) return rc;
+  }
+  H_->stim_dirty = true;
+  return VG_OK;
+}
+
+void vg_destroy(vg_handle *h) { delete h; }
+
+int vg_get_stream(vg_handle *h, void **s) {
+  if (!h || !s) return VG_ERR_ARG;
+  *s = (void *)h->stream;
+  return VG_OK;
+}
+
+int vg_set_stimulus_nodes(vg_handle *h, const int32_t *nodes, int64_t n, double istim) {
+  if (!h || (n > 0 && !nodes)) return VG_ERR_ARG;
+  for (int64_t k = 0; k < n; k++) {
+    if (nodes[k] < 0 || nodes[k] >= h->n) return h->fail(VG_ERR_ARG, "stimulus node out of range");
+    h->istim_host[nodes[k]] = istim;  // CardiacBody.cc:308-318
+  }
+  h->stim_dirty = true;
+  return VG_OK;
+}
+
+int vg_set_stimulus_sphere(vg_handle *h, const double c[3], double radius, double istim) {
+  if (!h || !c) return VG_ERR_ARG;
+  // CardiacBody.cc:283-305: Purkinje nodes are skipped; others inside get istim,
+  // outside get 0.
+  for (int64_t i = 0; i < h->n; i++) {
+    if (h->purkinje[i]) continue;
+    double d2 = 0;
+    for (int k = 0; k < 3; k++) d2 += (c[k] - h->xyz[3 * i + k]) * (c[k] - h->xyz[3 * i + k]);
+    h->istim_host[i] = (std::sqrt(d2) <= radius) ? istim : 0.0;
+  }
+  h->stim_dirty = true;
+  return VG_OK;
+}
+
+int vg_set_stimulus_axis(vg_handle *h, int axis, double limit, double istim) {
+  if (!h || axis < 0 || axis > 2) return VG_ERR_ARG;
+  // CardiacBody.cc:243-262: nodes with x[axis] <= limit get istim, others unchanged.
+  for (int64_t i = 0; i < h->n; i++)
+    if (h->xyz[3 * i + axis] <= limit) h->istim_host[i] = istim;
+  h->stim_dirty = true;
+  return VG_OK;
+}
+
+int vg_set_stimulus_plane(vg_handle *h, const double o[3], const double nrm[3], double istim) {
+  if (!h || !o || !nrm) return VG_ERR_ARG;
+  const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  if (!(len > 0)) return h->fail(VG_ERR_ARG, "zero plane normal");
+  // CardiacBody.cc:265-281: below the plane -> istim, above -> 0, Purkinje skipped.
+  for (int64_t i = 0; i < h->n; i++) {
+    if (h->purkinje[i]) continue;
+    double d = 0;
+    for (int k = 0; k < 3; k++) d += (h->xyz[3 * i + k] - o[k]) * nrm[k] / len;
+    h->istim_host[i] = (d < 0.) ? istim : 0.0;
+  }
+  h->stim_dirty = true;
+  return VG_OK;
+}
+
+int vg_clear_stimulus(vg_handle *h) {
+  if (!h) return VG_ERR_ARG;
+  std::fill(h->istim_host.begin(), h->istim_host.end(), 0.0);
+  h->stim_dirty = true;
+  return VG_OK;
+}
+
+int vg_step(vg_handle *h, int nsteps) {
+  if (!h || nsteps < 0) return VG_ERR_ARG;
+  for (int k = 0; k < nsteps; k++) {
+    const int rc = h->tsz == 8 ? step_t<double>(h) : step_t<float>(h);
+    if (rc) return rc;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hipfail(e, "kernel launch");
+  return VG_OK;
+}
+
+int vg_synchronize(vg_handle *h) {
+  if (!h) return VG_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return VG_OK;
+}
+
+int vg_get_voltage(vg_handle *h, double *V) {
+  if (!h || !V) return VG_ERR_ARG;
+  return download(h, h->Va.p, V);
+}
+
+int vg_set_voltage(vg_handle *h, const double *V) {
+  if (!h || !V) return VG_ERR_ARG;
+  int rc = upload(h, h->Va.p, V, V[0]);
+  if (rc) return rc;
+  return upload(h, h->Vb.p, V, V[0]);
+}
+
+int vg_get_states(vg_handle *h, int *nvar, double *st) {
+  if (!h || !nvar) return VG_ERR_ARG;
+  const int ns = nstate_of(h->model);
+  *nvar = ns;
+  if (!st) return VG_OK;
+  const int vslot = h->model == VG_MODEL_LR ? 7 : 0;
+  const int64_t n = h->n, na = h->n_alloc;
+  std::vector<double> soa((size_t)(ns - 1) * na);
+  if (h->tsz == 8) {
+    HIPCHK(h, hipMemcpy(soa.data(), h->states.p, soa.size() * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(soa.size());
+    HIPCHK(h, hipMemcpy(f.data(), h->states.p, f.size() * 4, hipMemcpyDeviceToHost));
+    std::copy(f.begin(), f.end(), soa.begin());
+  }
+  std::vector<double> V(n);
+  int rc = download(h, h->Va.p, V.data());
+  if (rc) return rc;
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t o = h->new2old[r];
+    int k2 = 0;
+    for (int k = 0; k < ns; k++) {
+      if (k == vslot) {
+        st[o * ns + k] = V[o];
+        continue;
+      }
+      st[o * ns + k] = soa[(size_t)k2 * na + r];
+      k2++;
+    }
+  }
+  return VG_OK;
+}
+
+int vg_set_states(vg_handle *h, int nvar, const double *st) {
+  if (!h || !st) return VG_ERR_ARG;
+  return set_states_impl(h, nvar, st);
+}
+
+int vg_get_lumped_capacitance(vg_handle *h, double *CL) {
+  if (!h || !CL) return VG_ERR_ARG;
+  std::copy(h->CL.begin(), h->CL.end(), CL);
+  return VG_OK;
+}
+
+int vg_activation_enable(vg_handle *h, double thr) {
+  if (!h) return VG_ERR_ARG;
+  std::vector<double> m1(h->n, -1.0);
+  int rc = upload(h, h->at.p, m1.data(), -1.0);
+  if (rc) return rc;
+  h->at_enabled = true;
+  h->at_thr = thr;
+  return VG_OK;
+}
+
+int vg_get_activation(vg_handle *h, double *t) {
+  if (!h || !t) return VG_ERR_ARG;
+  if (!h->at_enabled) return h->fail(VG_ERR_ARG, "activation tracking not enabled");
+  return download(h, h->at.p, t);
+}
+
+}  // extern "C"
+
+// The ECG lead field needs the element list, which the handle does not keep,
+// so the mesh is passed again.
+extern "C" int vg_ecg_setup(vg_handle *h, const vg_mesh *mesh, const double *el, int ne,
+                            const double D[3], double radius) {
+  if (!h || !mesh || !el || !D || ne < 1 || ne > MAX_ECG) return VG_ERR_ARG;
+  if (mesh->n_nodes != h->n) return h->fail(VG_ERR_ARG, "mesh does not match the handle");
+  std::vector<double> Hm;
+  std::string e = build_ecg_matrix(*mesh, h->quirks, el, ne, D, radius, Hm);
+  if (!e.empty()) return h->fail(VG_ERR_ARG, e);
+  const int64_t na = h->n_alloc;
+  std::vector<double> Hp((size_t)ne * na, 0.0);
+  for (int k = 0; k < ne; k++)
+    for (int64_t r = 0; r < h->n; r++) Hp[(size_t)k * na + r] = Hm[(size_t)k * h->n + h->new2old[r]];
+  int rc = h->tsz == 8 ? upload_raw<double>(h, h->H, Hp) : upload_raw<float>(h, h->H, Hp);
+  if (rc) return rc;
+  HIPCHK(h, h->ecg_out.alloc(MAX_ECG * 8));
+  h->ne = ne;
+  return VG_OK;
+}
+
+extern "C" {
+
+int vg_ecg(vg_handle *h, double *out) {
+  if (!h || !out) return VG_ERR_ARG;
+  if (h->ne == 0) return h->fail(VG_ERR_ARG, "ECG not set up");
+  if (h->tsz == 8)
+    hipLaunchKernelGGL(k_ecg<double>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream, h->Va.as<double>(),
+                       h->n, h->H.as<double>(), h->ne, h->n_alloc, red_of(h), h->ecg_out.as<double>());
+  else
+    hipLaunchKernelGGL(k_ecg<float>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream, h->Va.as<float>(),
+                       h->n, h->H.as<float>(), h->ne, h->n_alloc, red_of(h), h->ecg_out.as<double>());
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, h->ecg_out.p, h->ne * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return VG_OK;
+}
+
+int vg_ecg_record(vg_handle *h, int every, int max_frames) {
+  if (!h || every < 1 || max_frames < 1) return VG_ERR_ARG;
+  if (h->ne == 0) return h->fail(VG_ERR_ARG, "ECG not set up");
+  HIPCHK(h, h->ecg_frames.alloc((size_t)max_frames * h->ne * 8));
+  h->ecg_every = every;
+  h->ecg_max = max_frames;
+  h->ecg_count = 0;
+  return VG_OK;
+}
+
+int vg_ecg_frames(vg_handle *h, double *out, int *n_frames) {
+  if (!h || !n_frames) return VG_ERR_ARG;
+  *n_frames = h->ecg_count;
+  if (!out || h->ecg_count == 0) return VG_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(out, h->ecg_frames.p, (size_t)h->ecg_count * h->ne * 8, hipMemcpyDeviceToHost));
+  return VG_OK;
+}
+
+int vg_profile_enable(vg_handle *h, int enable) {
+  if (!h) return VG_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  drain_profile(h);
+  for (auto &c : h->prof) {
+    c.total_ms = 0;
+    c.launches = 0;
+  }
+  h->profiling = enable != 0;
+  return VG_OK;
+}
+
+int vg_profile_get(vg_handle *h, int kc, double *total_ms, int64_t *launches) {
+  if (!h || kc < 0 || kc >= VG_KC_COUNT) return VG_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  drain_profile(h);
+  if (total_ms) *total_ms = h->prof[kc].total_ms;
+  if (launches) *launches = h->prof[kc].launches;
+  return VG_OK;
+}
+
+int vg_stats_get(vg_handle *h, vg_stats *o) {
+  if (!h || !o) return VG_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  unsigned long long nw = 0;
+  HIPCHK(h, hipMemcpy(&nw, h->newton.p, sizeof(nw), hipMemcpyDeviceToHost));
+  o->steps = h->steps;
+  o->cg_iterations = h->cg_iters;
+  o->cg_solves = h->cg_solves;
+  o->cg_not_converged = h->cg_notconv;
+  o->newton_iterations = (int64_t)nw;
+  o->n_nodes = h->n;
+  o->nnz = h->nnz;
+  o->sell_padded_nnz = h->padded;
+  o->last_cg_relres = h->last_relres;
+  o->sim_time = h->t;
+  return VG_OK;
+}
+
+// ---- independent cells -------------------------------------------------------
+int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
+  if (!out || n < 1) return VG_ERR_ARG;
+  if (model != VG_MODEL_LR && model != VG_MODEL_TT) return VG_ERR_UNSUPPORTED;
+  if (prec != VG_PREC_FP64 && prec != VG_PREC_MIXED) return VG_ERR_ARG;
+  std::unique_ptr<vg_cells> c(new vg_cells());
+  c->prec = prec;
+  c->model = model;
+  c->integ = integ;
+  c->n = n;
+  c->n_alloc = ((n + BLK - 1) / BLK) * BLK;
+  const size_t T = prec == VG_PREC_FP64 ? 8 : 4;
+  const int ns = nstate_of(model);
+  if (c->states.alloc((size_t)(ns - 1) * c->n_alloc * T) != hipSuccess) return VG_ERR_HIP;
+  if (c->V.alloc(n * 8) != hipSuccess || c->stim.alloc(n * 8) != hipSuccess ||
+      c->dvdt.alloc(n * 8) != hipSuccess)
+    return VG_ERR_HIP;
+  std::vector<double> one(VG_TT_NSCALE, 1.0);
+  if (T == 8) {
+    if (c->scales.alloc(one.size() * 8) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->scales.p, one.data(), one.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(one.begin(), one.end());
+    if (c->scales.alloc(f.size() * 4) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->scales.p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  }
+  // Initial states: the model's initial table (LR: reference table).
+  std::vector<double> st((size_t)n * ns);
+  for (int64_t i = 0; i < n; i++)
+    for (int k = 0; k < ns; k++) st[i * ns + k] = model == VG_MODEL_LR ? LR_INIT[k] : TT_INIT[k];
+  *out = c.release();
+  return vg_cells_set_states(*out, ns, st.data());
+}
+
+void vg_cells_destroy(vg_cells *c) { delete c; }
+
+int vg_cells_set_scales(vg_cells *c, int nsets, const double *sc, const uint16_t *pidx) {
+  if (!c || nsets < 1 || !sc) return VG_ERR_ARG;
+  const size_t T = c->prec == VG_PREC_FP64 ? 8 : 4;
+  std::vector<double> v(sc, sc + (size_t)nsets * VG_TT_NSCALE);
+  if (c->scales.p) {
+    (void)hipFree(c->scales.p);
+    c->scales.p = nullptr;
+  }
+  if (T == 8) {
+    if (c->scales.alloc(v.size() * 8) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->scales.p, v.data(), v.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    if (c->scales.alloc(f.size() * 4) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->scales.p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  }
+  c->nsets = nsets;
+  if (pidx) {
+    for (int64_t i = 0; i < c->n; i++)
+      if (pidx[i] >= nsets) return VG_ERR_ARG;
+    if (!c->pidx.p && c->pidx.alloc(c->n * 2) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->pidx.p, pidx, c->n * 2, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  }
+  return VG_OK;
+}
+
+int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, const double *istim,
+                         double *dvdt) {
+  if (!c || !V || !dvdt) return VG_ERR_ARG;
+  const int64_t n = c->n;
+  if (hipMemcpy(c->V.p, V, n * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  if (istim && hipMemcpy(c->stim.p, istim, n * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  const unsigned nb = (unsigned)(c->n_alloc / BLK);
+  const uint16_t *pi = c->pidx.p ? c->pidx.as<uint16_t>() : nullptr;
+  const double *sp = istim ? c->stim.as<double>() : nullptr;
+#define VG_CELLS(T, M, I)                                                                      \
+  hipLaunchKernelGGL((k_cells<T, M, I>), dim3(nb), dim3(BLK), 0, 0, c->V.as<double>(), sp,      \
+                     c->dvdt.as<double>(), c->states.as<T>(), pi, c->scales.as<T>(), n,         \
+                     c->n_alloc, Cm, dt, 500)
+  if (c->prec == VG_PREC_FP64) {
+    if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
+    else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(double, VG_MODEL_TT, VG_INTEG_REFERENCE);
+    else VG_CELLS(double, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
+  } else {
+    if (c->model == VG_MODEL_LR) VG_CELLS(float, VG_MODEL_LR, VG_INTEG_REFERENCE);
+    else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(float, VG_MODEL_TT, VG_INTEG_REFERENCE);
+    else VG_CELLS(float, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
+  }
+#undef VG_CELLS
+  if (hipGetLastError() != hipSuccess) return VG_ERR_HIP;
+  if (hipMemcpy(dvdt, c->dvdt.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
+  return VG_OK;
+}
+
+int vg_cells_get_states(vg_cells *c, int *nvar, double *st) {
+  if (!c || !nvar) return VG_ERR_ARG;
+  const int ns = nstate_of(c->model);
+  *nvar = ns;
+  if (!st) return VG_OK;
+  const int vslot = c->model == VG_MODEL_LR ? 7 : 0;
+  const int64_t n = c->n, na = c->n_alloc;
+  std::vector<double> soa((size_t)(ns - 1) * na);
+  if (c->prec == VG_PREC_FP64) {
+    if (hipMemcpy(soa.data(), c->states.p, soa.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(soa.size());
+    if (hipMemcpy(f.data(), c->states.p, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
+    std::copy(f.begin(), f.end(), soa.begin());
+  }
+  std::vector<double> V(n);
+  if (hipMemcpy(V.data(), c->V.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
+  for (int64_t i = 0; i < n; i++) {
+    int k2 = 0;
+    for (int k = 0; k < ns; k++) {
+      if (k == vslot) {
+        st[i * ns + k] = V[i];
+        continue;
+      }
+      st[i * ns + k] = soa[(size_t)k2 * na + i];
+      k2++;
+    }
+  }
+  return VG_OK;
+}
+
+int vg_cells_set_states(vg_cells *c, int nvar, const double *st) {
+  if (!c || !st) return VG_ERR_ARG;
+  const int ns = nstate_of(c->model);
+  if (nvar != ns) return VG_ERR_ARG;
+  const int vslot = c->model == VG_MODEL_LR ? 7 : 0;
+  const int64_t n = c->n, na = c->n_alloc;
+  std::vector<double> soa((size_t)(ns - 1) * na, 0.0);
+  std::vector<double> V(n);
+  for (int64_t i = 0; i < n; i++) {
+    int k2 = 0;
+    for (int k = 0; k < ns; k++) {
+      if (k == vslot) {
+        V[i] = st[i * ns + k];
+        continue;
+      }
+      soa[(size_t)k2 * na + i] = st[i * ns + k];
+      k2++;
+    }
+  }
+  for (int k = 0; k < ns - 1; k++)
+    for (int64_t i = n; i < na; i++) soa[(size_t)k * na + i] = soa[(size_t)k * na];
+  if (c->prec == VG_PREC_FP64) {
+    if (hipMemcpy(c->states.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(soa.begin(), soa.end());
+    if (hipMemcpy(c->states.p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  }
+  if (hipMemcpy(c->V.p, V.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+  return VG_OK;
+}
+
+}  // extern "C"

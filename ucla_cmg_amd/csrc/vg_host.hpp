@@ -1,0 +1,49 @@
+// vg_host.hpp -- host-side operator construction for libvoomgpu.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/voomgpu.h"
+
+namespace vg {
+
+// CSR of K (conductivity, Cm included), M (consistent mass) and the lumped
+// capacitance, all on one pattern, in the caller's node numbering.
+struct HostCsr {
+  int64_t n = 0;
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> col;
+  std::vector<double> K, M, CL;
+};
+
+// Returns an empty string on success, else the reason.
+std::string assemble_csr(const vg_mesh &mesh, const vg_model &model, HostCsr &out);
+std::string validate_mesh(const vg_mesh &mesh);
+
+// SELL-64 layout: rows in slices of 64 lanes, one lane per row; slice s holds
+// width[s] entries per lane stored lane-fastest from offset[s].
+constexpr int SELL_C = 64;
+struct HostSell {
+  int64_t n = 0, n_slices = 0, padded = 0;
+  std::vector<int64_t> offset;   // [n_slices + 1]
+  std::vector<int32_t> width;    // [n_slices]
+  std::vector<int32_t> col;      // [padded]
+  std::vector<double> A_hat;     // D^-1/2 (Cm M + dt/4 K) D^-1/2
+  std::vector<double> K_rhs;     // -(dt/2) D^-1/2 K
+  std::vector<double> sinv;      // [n_pad] D^-1/2
+  std::vector<double> diag;      // [n_pad] D = diag(Cm M + dt/4 K)
+  std::vector<int32_t> new2old, old2new;
+};
+
+// Builds the scaled CN operators in SELL-64 form.  Rows are reordered by
+// descending length inside windows of `sigma` rows when sigma > 1.
+std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma, HostSell &out);
+
+// Lead-field matrix of the pseudo-ECG: ecg_e = sum_n H[e][n] V_n with
+// H[e][n] = sum over quadrature points of w (dN_n/dx)^T D grad(1/r_e).
+std::string build_ecg_matrix(const vg_mesh &mesh, unsigned quirks, const double *electrodes,
+                             int ne, const double Decg[3], double radius,
+                             std::vector<double> &H /* [ne * n_nodes] */);
+
+}  // namespace vg

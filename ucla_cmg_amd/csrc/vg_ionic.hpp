@@ -1,0 +1,420 @@
+// vg_ionic.hpp -- per-node ionic models for gfx950 (device code).
+//
+// Luo-Rudy (IonicModel/LR.cc) and ten Tusscher 2006 (IonicModel/Tusscher.cc,
+// constants IonicModel/SetUpIonicConstants.h:88-147) written as register-
+// resident device functions over one node's state.  T is float or double;
+// FAST selects hardware exp/log approximations (used only with float).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace vg {
+
+// ---- math shims --------------------------------------------------------------
+template <class T, bool FAST> struct Mth;
+template <bool FAST> struct Mth<double, FAST> {
+  __device__ static double ex(double x) { return exp(x); }
+  __device__ static double lg(double x) { return log(x); }
+  __device__ static double sq(double x) { return sqrt(x); }
+  __device__ static double pw(double x, double y) { return pow(x, y); }
+  __device__ static double em1(double x) { return expm1(x); }
+};
+template <> struct Mth<float, true> {
+  __device__ static float ex(float x) { return __expf(x); }
+  __device__ static float lg(float x) { return __logf(x); }
+  __device__ static float sq(float x) { return __fsqrt_rn(x); }
+  __device__ static float pw(float x, float y) { return __powf(x, y); }
+  __device__ static float em1(float x) { return expm1f(x); }
+};
+template <> struct Mth<float, false> {
+  __device__ static float ex(float x) { return expf(x); }
+  __device__ static float lg(float x) { return logf(x); }
+  __device__ static float sq(float x) { return sqrtf(x); }
+  __device__ static float pw(float x, float y) { return powf(x, y); }
+  __device__ static float em1(float x) { return expm1f(x); }
+};
+
+// ======================= Luo-Rudy 1991 ========================================
+// State: 0 m, 1 h, 2 j, 3 d, 4 f, 5 X, 6 Ca_i, 7 V (LR.cc:18-26 with the gate
+// order of make_parameter1 :166-181).
+constexpr int LR_NSTATE = 8;
+constexpr double LR_INIT[LR_NSTATE] = {.001, 0.0001, 0.99, .001, 0.9, .001, 0.0000001, 0};
+
+// LuoRudy::Compute_Ion + make_parameter1 (LR.cc:65-183), evaluated in T.
+template <class T, bool FAST>
+__device__ inline T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T gna_scale,
+                                   T gsi_scale) {
+  using M = Mth<T, FAST>;
+  const T c_K0 = T(5.4), c_Ki = T(145.0), RT_F = T(26.73);
+  const T Gna = T(12.00) * gna_scale;
+  const T Gsi = T(0.07) * gsi_scale;
+  const T GK = T(0.705) * M::sq(c_K0 / T(5.4));
+  const T E_K1 = RT_F * M::lg(c_K0 / c_Ki);
+  const T G_K1 = T(.6047) * M::sq(c_K0 / T(5.4));
+  const T r = T(11e-4), L = T(100E-4);
+  const T Xi = T(2.) * (r + L) / (r * L);
+
+  T alpha_m, beta_m, alpha_h, beta_h, alpha_j, beta_j;
+  if (fabs(v + T(47.13)) < T(0.001)) alpha_m = T(3.2);
+  else alpha_m = T(0.32) * (v + T(47.13)) / (T(1.0) - M::ex(T(-0.1) * (v + T(47.13))));
+  beta_m = T(0.08) * M::ex(-v / T(11.));
+  if (v > T(-40.)) {
+    alpha_h = T(0);
+    alpha_j = T(0);
+    beta_h = T(1.) / (T(0.13) * (T(1) + M::ex(-(v + T(10.66)) / T(11.1))));
+    beta_j = T(0.3) * M::ex(T(-2.535e-7) * v) / (T(1) + M::ex(T(-0.1) * (v + T(32))));
+  } else {
+    alpha_h = T(0.135) * M::ex(-(T(80) + v) / T(6.8));
+    alpha_j = (T(-1.2714e5) * M::ex(T(0.2444) * v) - T(3.474e-5) * M::ex(T(-0.04391) * v)) *
+              (v + T(37.78)) / (T(1) + M::ex(T(0.311) * (v + T(79.23))));
+    beta_h = T(3.56) * M::ex(T(0.079) * v) + T(3.1e5) * M::ex(T(0.35) * v);
+    beta_j = T(0.1212) * M::ex(T(-0.01052) * v) / (T(1) + M::ex(T(-0.1378) * (v + T(40.14))));
+  }
+  const T alpha_d = T(.095) * M::ex(T(-.01) * (v - T(5))) / (T(1) + M::ex(T(-.072) * (v - T(5))));
+  const T beta_d = T(.07) * M::ex(T(-.017) * (v + T(44))) / (T(1) + M::ex(T(.05) * (v + T(44))));
+  const T alpha_f = T(.012) * M::ex(T(-.008) * (v + T(28))) / (T(1) + M::ex(T(.15) * (v + T(28))));
+  const T beta_f = T(.0065) * M::ex(T(-.02) * (v + T(30))) / (T(1) + M::ex(T(-.2) * (v + T(30))));
+  const T alpha_x = T(.0005) * M::ex(T(.083) * (v + T(50))) / (T(1) + M::ex(T(.057) * (v + T(50))));
+  const T beta_x = T(.0013) * M::ex(T(-.06) * (v + T(20))) / (T(1) + M::ex(T(-.04) * (v + T(20))));
+  T x1;
+  if (v > T(-100)) {
+    if (fabs(v + T(77.)) < T(0.001)) x1 = T(0.608889);
+    else x1 = T(2.837) * (M::ex(T(0.04) * (v + T(77))) - T(1)) / ((v + T(77)) * M::ex(T(0.04) * (v + T(35))));
+  } else {
+    x1 = T(1.);
+  }
+  const T alpha_k1 = T(1.02) / (T(1) + M::ex(T(0.2385) * (v - E_K1 - T(59.215))));
+  const T beta1 = T(0.49124) * M::ex(T(0.08032) * (v - E_K1 + T(5.476)));
+  const T beta2 = M::ex(T(0.06175) * (v - E_K1 - T(594.31)));
+  const T beta3 = T(1) + M::ex(T(-0.5143) * (v - E_K1 + T(4.753)));
+  const T beta_k1 = (beta1 + beta2) / beta3;
+  const T k1_inf = alpha_k1 / (alpha_k1 + beta_k1);
+  const T I_k1 = G_K1 * k1_inf * (v - E_K1);
+  const T kp = T(1.) / (T(1) + M::ex((T(7.488) - v) / T(5.98)));
+  const T I_kp = T(0.0183) * kp * (v - E_K1);
+  const T I_b = T(0.0392) * (v + T(59.87));
+
+  // Rush-Larsen with the rate sums alpha+beta (LR.cc:83-85).
+  const T inf[6] = {alpha_m / (alpha_m + beta_m), alpha_h / (alpha_h + beta_h),
+                    alpha_j / (alpha_j + beta_j), alpha_d / (alpha_d + beta_d),
+                    alpha_f / (alpha_f + beta_f), alpha_x / (alpha_x + beta_x)};
+  const T sum[6] = {alpha_m + beta_m, alpha_h + beta_h, alpha_j + beta_j,
+                    alpha_d + beta_d, alpha_f + beta_f, alpha_x + beta_x};
+#pragma unroll
+  for (int i = 0; i < 6; i++) s[i] = inf[i] - (inf[i] - s[i]) * M::ex(-dt * sum[i]);
+  s[7] = v;
+  const T Es = T(7.7) - T(13.0287) * M::lg(s[6]);
+  const T i_na = Gna * s[0] * s[0] * s[0] * s[1] * s[2] * (v - T(54.4));
+  const T i_si = Gsi * s[3] * s[4] * (v - Es);
+  const T i_k = GK * x1 * s[5] * (v + T(77.0));
+  s[6] = s[6] + (T(-.0001) * i_si + T(0.07) * (T(0.0001) - s[6])) * dt;
+  const T i_tot = -((I_k1 + I_kp + I_b) + i_k + i_na + i_si);
+  return i_tot + istim / (Xi * Cm);
+}
+
+// ======================= ten Tusscher 2006 =====================================
+// State (CellML order used by Tusscher.cc): 0 V, 1 K_i, 2 Na_i, 3 Ca_i, 4 Xr1,
+// 5 Xr2, 6 Xs, 7 m, 8 h, 9 j, 10 Ca_ss, 11 d, 12 f, 13 f2, 14 fCass, 15 s,
+// 16 r, 17 Ca_SR, 18 R'.
+constexpr int TT_NSTATE = 19;
+constexpr double TT_INIT[TT_NSTATE] = {-85.423, 136.89, 8.604,    0.000126, 0.00621,
+                                       0.4712,  0.0095, 0.00172,  0.7444,   0.7045,
+                                       0.00036, 3.373e-5, 0.7888, 0.9755,   0.9953,
+                                       0.999998, 2.42e-8, 3.64,   0.9073};
+// SetUpTusscherParameters (SetUpIonicConstants.h:93-145).
+constexpr double TTC[53] = {
+    8314.472, 310,      96485.3415, 0.185,     0.016404,  10,       1000,
+    1,        52,       0.03,       5.4,       140,       2,        5.405,
+    0.153,    0.392,    14.838,     0.00029,   0.0000398, 0.000592, 0.294,
+    2.724,    1,        40,         1000,      0.1,       2.5,      0.35,
+    1.38,     87.5,     0.1238,     0.0005,    0.0146,    0.15,     0.045,
+    0.06,     0.005,    1.5,        2.5,       1,         0.102,    0.0038,
+    0.00025,  0.00036,  0.006375,   0.2,       0.001,     10,       0.3,
+    0.4,      0.00025,  0.001094,   0.00005468};
+constexpr double TT_XI = 2000.;  // Tusscher.h:72
+
+// Gate order of UpdateStateVariables (Tusscher.cc:122,125).
+constexpr int TT_GATE_STATE[12] = {4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 15, 16};
+constexpr int TT_NR_STATE[6] = {1, 2, 3, 10, 17, 18};
+
+template <class T> struct TTG {  // per-node conductances (after scaling)
+  T gNa, gKr, gKs, gCaL, gto;
+};
+
+template <class T> __device__ inline TTG<T> tt_conductances(const T *scale) {
+  TTG<T> g;
+  g.gNa = T(TTC[16]) * scale[0];
+  g.gKr = T(TTC[14]) * scale[1];
+  g.gKs = T(TTC[15]) * scale[2];
+  g.gCaL = T(TTC[18]) * scale[3];
+  g.gto = T(TTC[20]) * scale[4];
+  return g;
+}
+
+// Steady states and time constants of the 12 gates (Tusscher.cc:5-55), in the
+// order of TT_GATE_STATE.  Depends on V and, for fCass, Ca_ss.
+template <class T, bool FAST>
+__device__ inline void tt_gates(T V, T Cass, T inf[12], T tau[12]) {
+  using M = Mth<T, FAST>;
+  inf[7] = T(1.0) / (T(1.0) + M::ex((V + T(20.0)) / T(7.0)));
+  tau[7] = T(1102.5) * M::ex(-((V + T(27.0)) * (V + T(27.0))) / T(225.0)) +
+           T(200.0) / (T(1.0) + M::ex((T(13.0) - V) / T(10.0))) +
+           T(180.0) / (T(1.0) + M::ex((V + T(30.0)) / T(10.0))) + T(20.0);
+  inf[8] = T(0.67) / (T(1.0) + M::ex((V + T(35.0)) / T(7.0))) + T(0.33);
+  tau[8] = T(562.0) * M::ex(-((V + T(27.0)) * (V + T(27.0))) / T(240.0)) +
+           T(31.0) / (T(1.0) + M::ex((T(25.0) - V) / T(10.0))) +
+           T(80.0) / (T(1.0) + M::ex((V + T(30.0)) / T(10.0)));
+  const T q = (Cass / T(0.05)) * (Cass / T(0.05));
+  inf[9] = T(0.6) / (T(1.0) + q) + T(0.4);
+  tau[9] = T(80.0) / (T(1.0) + q) + T(2.0);
+  inf[10] = T(1.0) / (T(1.0) + M::ex((V + T(20.0)) / T(5.0)));
+  tau[10] = T(85.0) * M::ex(-((V + T(45.0)) * (V + T(45.0))) / T(320.0)) +
+            T(5.0) / (T(1.0) + M::ex((V - T(20.0)) / T(5.0))) + T(3.0);
+  inf[11] = T(1.0) / (T(1.0) + M::ex((T(20.0) - V) / T(6.0)));
+  tau[11] = T(9.5) * M::ex(-((V + T(40.0)) * (V + T(40.0))) / T(1800.0)) + T(0.8);
+  inf[0] = T(1.0) / (T(1.0) + M::ex((T(-26.0) - V) / T(7.0)));
+  tau[0] = (T(450.0) / (T(1.0) + M::ex((T(-45.0) - V) / T(10.0)))) *
+           (T(6.0) / (T(1.0) + M::ex((V + T(30.0)) / T(11.5))));
+  inf[1] = T(1.0) / (T(1.0) + M::ex((V + T(88.0)) / T(24.0)));
+  tau[1] = (T(3.0) / (T(1.0) + M::ex((T(-60.0) - V) / T(20.0)))) *
+           (T(1.12) / (T(1.0) + M::ex((V - T(60.0)) / T(20.0))));
+  inf[2] = T(1.0) / (T(1.0) + M::ex((T(-5.0) - V) / T(14.0)));
+  tau[2] = (T(1400.0) / M::sq(T(1.0) + M::ex((T(5.0) - V) / T(6.0)))) *
+               (T(1.0) / (T(1.0) + M::ex((V - T(35.0)) / T(15.0)))) + T(80.0);
+  {
+    const T e = T(1.0) + M::ex((T(-56.86) - V) / T(9.03));
+    inf[3] = T(1.0) / (e * e);
+  }
+  tau[3] = (T(1.0) / (T(1.0) + M::ex((T(-60.0) - V) / T(5.0)))) *
+           (T(0.1) / (T(1.0) + M::ex((V + T(35.0)) / T(5.0))) +
+            T(0.1) / (T(1.0) + M::ex((V - T(50.0)) / T(200.0))));
+  {
+    const T e = T(1.0) + M::ex((V + T(71.55)) / T(7.43));
+    inf[4] = T(1.0) / (e * e);
+    inf[5] = inf[4];
+  }
+  if (V < T(-40.0)) {
+    const T ah = T(0.057) * M::ex(-(V + T(80.0)) / T(6.8));
+    const T bh = T(2.7) * M::ex(T(0.079) * V) + T(310000.) * M::ex(T(0.3485) * V);
+    tau[4] = T(1.0) / (ah + bh);
+    const T aj = ((T(-25428.0) * M::ex(T(0.2444) * V) - T(6.948e-06) * M::ex(T(-0.04391) * V)) *
+                  (V + T(37.78))) /
+                 (T(1.0) + M::ex(T(0.311) * (V + T(79.23))));
+    const T bj = (T(0.02424) * M::ex(T(-0.01052) * V)) / (T(1.0) + M::ex(T(-0.1378) * (V + T(40.14))));
+    tau[5] = T(1.0) / (aj + bj);
+  } else {
+    const T bh = T(0.77) / (T(0.13) * (T(1.0) + M::ex((V + T(10.66)) / T(-11.1))));
+    tau[4] = T(1.0) / (T(0.0) + bh);
+    const T bj = (T(0.6) * M::ex(T(0.057) * V)) / (T(1.0) + M::ex(T(-0.1) * (V + T(32.0))));
+    tau[5] = T(1.0) / (T(0.0) + bj);
+  }
+  inf[6] = T(1.0) / (T(1.0) + M::ex((T(-8.0) - V) / T(7.5)));
+  tau[6] = (T(1.4) / (T(1.0) + M::ex((T(-35.0) - V) / T(13.0))) + T(0.25)) *
+               (T(1.4) / (T(1.0) + M::ex((V + T(5.0)) / T(5.0)))) +
+           T(1.0) / (T(1.0) + M::ex((T(50.0) - V) / T(20.0)));
+}
+
+// Currents and the rates of V and the six concentrations (Tusscher.cc:56-95).
+// R7[0] = dV/dt, R7[1..6] = d/dt of K_i, Na_i, Ca_i, Ca_ss, Ca_SR, R'.
+// istim_term is the reference's _Algebraic[12] = -istim/(C_m Xi).
+// LIMIT evaluates the I_CaL driving-force ratio through expm1 so it stays
+// finite at V = 15 mV; without it the reference's expression is used verbatim.
+template <class T, bool FAST, bool LIMIT>
+__device__ inline void tt_currents(const T *S, T istim_term, const TTG<T> &g, T R7[7]) {
+  using M = Mth<T, FAST>;
+  const T R = T(TTC[0]), Tk = T(TTC[1]), F = T(TTC[2]);
+  const T Cmc = T(TTC[3]), Vc = T(TTC[4]);
+  const T Ko = T(TTC[10]), Nao = T(TTC[11]), Cao = T(TTC[12]);
+  const T RTF = (R * Tk) / F;
+  const T V = S[0];
+  const T FVRT = (V * F) / (R * Tk);
+  const T i_NaK = (((T(TTC[21]) * Ko) / (Ko + T(TTC[22]))) * S[2] / (S[2] + T(TTC[23]))) /
+                  (T(1.0) + T(0.1245) * M::ex(T(-0.1) * FVRT) + T(0.0353) * M::ex(-FVRT));
+  const T E_Na = RTF * M::lg(Nao / S[2]);
+  const T i_Na = g.gNa * (S[7] * S[7] * S[7]) * S[8] * S[9] * (V - E_Na);
+  const T i_b_Na = T(TTC[17]) * (V - E_Na);
+  const T gam = T(TTC[27]);
+  const T i_NaCa =
+      (T(TTC[24]) * (M::ex(gam * FVRT) * (S[2] * S[2] * S[2]) * Cao -
+                     M::ex((gam - T(1.0)) * FVRT) * (Nao * Nao * Nao) * S[3] * T(TTC[26]))) /
+      ((T(TTC[29]) * T(TTC[29]) * T(TTC[29]) + Nao * Nao * Nao) * (T(TTC[28]) + Cao) *
+       (T(1.0) + T(TTC[25]) * M::ex((gam - T(1.0)) * FVRT)));
+  const T E_K = RTF * M::lg(Ko / S[1]);
+  const T a_K1 = T(0.1) / (T(1.0) + M::ex(T(0.06) * ((V - E_K) - T(200.0))));
+  const T b_K1 = (T(3.0) * M::ex(T(0.0002) * ((V - E_K) + T(100.0))) +
+                  M::ex(T(0.1) * ((V - E_K) - T(10.0)))) /
+                 (T(1.0) + M::ex(T(-0.5) * (V - E_K)));
+  const T xK1 = a_K1 / (a_K1 + b_K1);
+  const T sqKo = M::sq(Ko / T(5.4));
+  const T i_K1 = T(TTC[13]) * xK1 * sqKo * (V - E_K);
+  const T i_to = g.gto * S[16] * S[15] * (V - E_K);
+  const T i_Kr = g.gKr * sqKo * S[4] * S[5] * (V - E_K);
+  const T E_Ks = RTF * M::lg((Ko + T(TTC[9]) * Nao) / (S[1] + T(TTC[9]) * S[2]));
+  const T i_Ks = g.gKs * (S[6] * S[6]) * (V - E_Ks);
+  T i_CaL;
+  if (LIMIT) {
+    const T u = T(2.0) * (V - T(15.0)) * F / (R * Tk);
+    const T eu = M::ex(u);
+    const T ratio = (fabs(u) < T(1e-4)) ? (T(1.0) - T(0.5) * u) : u / M::em1(u);
+    i_CaL = g.gCaL * S[11] * S[12] * S[13] * S[14] * T(2.0) * F * ratio *
+            (T(0.25) * S[10] * eu - Cao);
+  } else {
+    const T eu = M::ex((T(2.0) * (V - T(15.0)) * F) / (R * Tk));
+    i_CaL = ((g.gCaL * S[11] * S[12] * S[13] * S[14] * T(4.0) * (V - T(15.0)) * (F * F)) /
+             (R * Tk)) *
+            (T(0.25) * S[10] * eu - Cao) / (eu - T(1.0));
+  }
+  const T E_Ca = ((T(0.5) * R * Tk) / F) * M::lg(Cao / S[3]);
+  const T i_b_Ca = T(TTC[19]) * (V - E_Ca);
+  const T i_p_K = (T(TTC[32]) * (V - E_K)) / (T(1.0) + M::ex((T(25.0) - V) / T(5.98)));
+  const T i_p_Ca = (T(TTC[30]) * S[3]) / (S[3] + T(TTC[31]));
+  R7[0] = -(i_K1 + i_to + i_Kr + i_Ks + i_CaL + i_NaK + i_Na + i_b_Na + i_NaCa + i_b_Ca +
+            i_p_K + i_p_Ca + istim_term);
+  R7[1] = ((T(-1.0) * ((i_K1 + i_to + i_Kr + i_Ks + i_p_K + istim_term) - T(2.0) * i_NaK)) /
+           (T(1.0) * Vc * F)) *
+          Cmc;
+  R7[2] = ((T(-1.0) * (i_Na + i_b_Na + T(3.0) * i_NaK + T(3.0) * i_NaCa)) / (T(1.0) * Vc * F)) * Cmc;
+  const T Vsr = T(TTC[51]), Vss = T(TTC[52]);
+  const T i_up = T(TTC[44]) / (T(1.0) + (T(TTC[42]) * T(TTC[42])) / (S[3] * S[3]));
+  const T i_leak = T(TTC[43]) * (S[17] - S[3]);
+  const T i_xfer = T(TTC[41]) * (S[10] - S[3]);
+  const T bi = T(1.0) / (T(1.0) + (T(TTC[45]) * T(TTC[46])) / ((S[3] + T(TTC[46])) * (S[3] + T(TTC[46]))));
+  R7[3] = bi * ((((i_leak - i_up) * Vsr) / Vc + i_xfer) -
+                (T(1.0) * ((i_b_Ca + i_p_Ca) - T(2.0) * i_NaCa) * Cmc) / (T(2.0) * T(1.0) * Vc * F));
+  const T kcasr = T(TTC[38]) - (T(TTC[38]) - T(TTC[39])) /
+                                  (T(1.0) + (T(TTC[37]) / S[17]) * (T(TTC[37]) / S[17]));
+  const T k2 = T(TTC[34]) * kcasr;
+  R7[6] = -k2 * S[10] * S[18] + T(TTC[36]) * (T(1.0) - S[18]);
+  const T k1 = T(TTC[33]) / kcasr;
+  const T O = (k1 * (S[10] * S[10]) * S[18]) / (T(TTC[35]) + k1 * (S[10] * S[10]));
+  const T i_rel = T(TTC[40]) * O * (S[17] - S[10]);
+  const T bsr = T(1.0) / (T(1.0) + (T(TTC[47]) * T(TTC[48])) / ((S[17] + T(TTC[48])) * (S[17] + T(TTC[48]))));
+  R7[5] = bsr * (i_up - (i_rel + i_leak));
+  const T bss = T(1.0) / (T(1.0) + (T(TTC[49]) * T(TTC[50])) / ((S[10] + T(TTC[50])) * (S[10] + T(TTC[50]))));
+  R7[4] = bss * (((T(-1.0) * i_CaL * Cmc) / (T(2.0) * T(1.0) * Vss * F) + (i_rel * Vsr) / Vss) -
+                 (i_xfer * Vc) / Vss);
+}
+
+// Rush-Larsen gates + forward-Euler concentrations with one rate evaluation:
+// gates first (at V and the old Ca_ss), then currents at the new gates and old
+// concentrations, mirroring the reference's gates-then-concentrations order.
+// Returns dV/dt; the caller advances V.
+template <class T, bool FAST>
+__device__ inline T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TTG<T> &g) {
+  using M = Mth<T, FAST>;
+  S[0] = V;
+  T inf[12], tau[12];
+  tt_gates<T, FAST>(V, S[10], inf, tau);
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const int k = TT_GATE_STATE[i];
+    S[k] = inf[i] - (inf[i] - S[k]) * M::ex(-dt / tau[i]);
+  }
+  T R7[7];
+  tt_currents<T, FAST, true>(S, istim_term, g, R7);
+#pragma unroll
+  for (int i = 0; i < 6; i++) S[TT_NR_STATE[i]] += dt * R7[1 + i];
+  return R7[0];
+}
+
+// 6x6 solve of the system a column-major LAPACK dgesv sees for the row-major
+// array J (entry (r,c) = J[c*6+r]); partial pivoting, B overwritten.
+template <class T> __device__ inline void tt_solve6(const T J[36], T B[6]) {
+  T a[6][6];
+#pragma unroll
+  for (int r = 0; r < 6; r++)
+#pragma unroll
+    for (int c = 0; c < 6; c++) a[r][c] = J[c * 6 + r];
+  for (int k = 0; k < 6; k++) {
+    int p = k;
+    T best = fabs(a[k][k]);
+    for (int r = k + 1; r < 6; r++)
+      if (fabs(a[r][k]) > best) {
+        best = fabs(a[r][k]);
+        p = r;
+      }
+    if (p != k) {
+      for (int c = 0; c < 6; c++) {
+        const T t = a[k][c];
+        a[k][c] = a[p][c];
+        a[p][c] = t;
+      }
+      const T t = B[k];
+      B[k] = B[p];
+      B[p] = t;
+    }
+    for (int r = k + 1; r < 6; r++) {
+      const T l = a[r][k] / a[k][k];
+      for (int c = k + 1; c < 6; c++) a[r][c] -= l * a[k][c];
+      B[r] -= l * B[k];
+    }
+  }
+  for (int k = 5; k >= 0; k--) {
+    T s = B[k];
+    for (int c = k + 1; c < 6; c++) s -= a[k][c] * B[c];
+    B[k] = s / a[k][k];
+  }
+}
+
+// The reference integrator (Tusscher.cc:116-195): backward-Euler gates, then
+// Newton on the six concentrations with a central-difference Jacobian solved
+// as J^T, stopping at ||dx||2 <= 1e-6 (capped at max_newton passes).
+template <class T>
+__device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
+                                      const TTG<T> &g, int max_newton, int *iters) {
+  S[0] = V;
+  T StateN[6], oldR[6], fxph[6], fxmh[6], J[36], B[6], R7[7];
+#pragma unroll
+  for (int i = 0; i < 6; i++) StateN[i] = S[TT_NR_STATE[i]];
+  {
+    T inf[12], tau[12];
+    tt_gates<T, false>(V, S[10], inf, tau);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      const int k = TT_GATE_STATE[i];
+      const T b = T(-1.) / tau[i];
+      const T a = inf[i] / tau[i];
+      S[k] = (S[k] + dt * a) / (T(1) - dt * b);
+    }
+  }
+  tt_currents<T, false, false>(S, istim_term, g, R7);
+#pragma unroll
+  for (int i = 0; i < 6; i++) oldR[i] = R7[1 + i];
+  T resid = T(100.);
+  int it = 0;
+  while (resid > T(1e-6) && it < max_newton) {
+    for (int i = 0; i < 6; i++) {
+      const int ki = TT_NR_STATE[i];
+      const T eps = T(1e-3) * S[ki];
+      const T twoeps = T(2.) * eps;
+      S[ki] += eps;
+      tt_currents<T, false, false>(S, istim_term, g, R7);
+#pragma unroll
+      for (int j = 0; j < 6; j++) fxph[j] = S[TT_NR_STATE[j]] - StateN[j] - dt * R7[1 + j];
+      S[ki] -= T(2) * eps;
+      tt_currents<T, false, false>(S, istim_term, g, R7);
+#pragma unroll
+      for (int j = 0; j < 6; j++) fxmh[j] = S[TT_NR_STATE[j]] - StateN[j] - dt * R7[1 + j];
+      S[ki] += eps;
+#pragma unroll
+      for (int j = 0; j < 6; j++) J[j * 6 + i] = (fxph[j] - fxmh[j]) / twoeps;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) B[i] = S[TT_NR_STATE[i]] - StateN[i] - dt * oldR[i];
+    tt_solve6<T>(J, B);
+#pragma unroll
+    for (int i = 0; i < 6; i++) S[TT_NR_STATE[i]] -= B[i];
+    tt_currents<T, false, false>(S, istim_term, g, R7);
+#pragma unroll
+    for (int i = 0; i < 6; i++) oldR[i] = R7[1 + i];
+    resid = T(0);
+#pragma unroll
+    for (int i = 0; i < 6; i++) resid += B[i] * B[i];
+    resid = sqrt(resid);
+    it++;
+  }
+  *iters = it;
+  return R7[0];
+}
+
+}  // namespace vg

@@ -1,0 +1,218 @@
+"""Synthetic meshes for the monodomain workloads.
+
+The reference reads meshes from files (``Mesh/Mesh.cc:142-416``: ``.node``,
+``.ele``, ``.fibers``, ``.purk``, ``.cell``, ``.angle``).  No mesh ships with it,
+so the workloads are generated here, deterministically:
+
+* ``slab``: a structured box of hexahedra, each split into 5 P1 tets with the
+  split mirrored on alternate cells (parity of i+j+k) so faces match.
+* ``lv_shell``: a truncated prolate-ellipsoid left-ventricle shell on a
+  (transmural x circumferential x longitudinal) grid, each hex split into 6
+  tets around its main diagonal (Kuhn split, conforming without parity).
+
+Every tet is returned positively oriented, since the reference integrates with
+the signed Jacobian determinant (``Mesh/MixedMesh3D.cc:126-136``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class Mesh:
+    """Mixed P1-tet + 2-node bar mesh, 0-based.
+
+    ``conn`` has 4 slots per element; bars use the first two (``elem_nn`` 2).
+    ``fibers`` holds rows f, s, n per element (9 numbers, the layout of the
+    reference's ``.fibers`` file) or is ``None`` for an isotropic mesh.
+    """
+
+    xyz: np.ndarray
+    elem_nn: np.ndarray
+    conn: np.ndarray
+    fibers: Optional[np.ndarray] = None
+    tissue: Optional[np.ndarray] = None
+    pmj_node: Optional[np.ndarray] = None
+    pmj_data: Optional[np.ndarray] = None
+    radius: float = 0.04
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.xyz.shape[0])
+
+    @property
+    def n_elems(self) -> int:
+        return int(self.elem_nn.shape[0])
+
+
+def _orient(xyz: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """Swap the last two nodes of negatively oriented tets."""
+    p = xyz[tets]
+    d = np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])
+    neg = d < 0
+    t = tets.copy()
+    t[neg, 2], t[neg, 3] = tets[neg, 3], tets[neg, 2]
+    return t
+
+
+# Corner offsets (a,b,c) of a hex, indexed by a + 2b + 4c.
+def _v(a, b, c):
+    return a + 2 * b + 4 * c
+
+
+# 5-tet split of an "even" cell: four corner tets and one central tet.
+_FIVE_EVEN = [
+    (_v(0, 0, 0), _v(1, 0, 0), _v(0, 1, 0), _v(0, 0, 1)),
+    (_v(1, 1, 0), _v(1, 0, 0), _v(0, 1, 0), _v(1, 1, 1)),
+    (_v(1, 0, 1), _v(1, 0, 0), _v(0, 0, 1), _v(1, 1, 1)),
+    (_v(0, 1, 1), _v(0, 1, 0), _v(0, 0, 1), _v(1, 1, 1)),
+    (_v(1, 0, 0), _v(0, 1, 0), _v(0, 0, 1), _v(1, 1, 1)),
+]
+# The mirror image, used on odd cells so shared faces carry the same diagonal.
+_FIVE_ODD = [
+    (_v(1, 0, 0), _v(0, 0, 0), _v(1, 1, 0), _v(1, 0, 1)),
+    (_v(0, 1, 0), _v(0, 0, 0), _v(1, 1, 0), _v(0, 1, 1)),
+    (_v(0, 0, 1), _v(0, 0, 0), _v(1, 0, 1), _v(0, 1, 1)),
+    (_v(1, 1, 1), _v(1, 1, 0), _v(1, 0, 1), _v(0, 1, 1)),
+    (_v(0, 0, 0), _v(1, 1, 0), _v(1, 0, 1), _v(0, 1, 1)),
+]
+# Kuhn 6-tet split around the 000-111 diagonal.
+_SIX = [
+    (_v(0, 0, 0), _v(1, 0, 0), _v(1, 1, 0), _v(1, 1, 1)),
+    (_v(0, 0, 0), _v(1, 0, 0), _v(1, 0, 1), _v(1, 1, 1)),
+    (_v(0, 0, 0), _v(0, 1, 0), _v(1, 1, 0), _v(1, 1, 1)),
+    (_v(0, 0, 0), _v(0, 1, 0), _v(0, 1, 1), _v(1, 1, 1)),
+    (_v(0, 0, 0), _v(0, 0, 1), _v(1, 0, 1), _v(1, 1, 1)),
+    (_v(0, 0, 0), _v(0, 0, 1), _v(0, 1, 1), _v(1, 1, 1)),
+]
+
+
+def slab(nx=41, ny=41, nz=9, h=0.025, fiber_axis=(1.0, 0.0, 0.0)) -> Mesh:
+    """Structured slab, node id = i + nx*j + nx*ny*k, 5 tets per hex.
+
+    The default is the 1.0 x 1.0 x 0.2 cm slab at h = 0.025 cm (15,129 nodes,
+    64,000 tets) with fibers along x.
+    """
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = (i + nx * j + nx * ny * k).astype(np.int64)
+    xyz = np.zeros((nx * ny * nz, 3))
+    xyz[idx.ravel(), 0] = (i * h).ravel()
+    xyz[idx.ravel(), 1] = (j * h).ravel()
+    xyz[idx.ravel(), 2] = (k * h).ravel()
+    ci, cj, ck = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1),
+                             indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    corners = np.stack([
+        (ci + a) + nx * (cj + b) + nx * ny * (ck + c)
+        for c in (0, 1) for b in (0, 1) for a in (0, 1)
+    ], axis=1)  # column order a + 2b + 4c
+    even = ((ci + cj + ck) % 2) == 0
+    tets = np.empty((ci.size, 5, 4), dtype=np.int64)
+    te = np.array(_FIVE_EVEN)
+    to = np.array(_FIVE_ODD)
+    tets[even] = corners[even][:, te]
+    tets[~even] = corners[~even][:, to]
+    tets = _orient(xyz, tets.reshape(-1, 4))
+    ne = tets.shape[0]
+    f = np.asarray(fiber_axis, dtype=np.float64)
+    f = f / np.linalg.norm(f)
+    # Complete f to an orthonormal frame (s, n).
+    trial = np.array([0.0, 1.0, 0.0]) if abs(f[1]) < 0.9 else np.array([0.0, 0.0, 1.0])
+    s = trial - f * (trial @ f)
+    s /= np.linalg.norm(s)
+    n = np.cross(f, s)
+    fib = np.tile(np.concatenate([f, s, n]), (ne, 1))
+    return Mesh(xyz=xyz, elem_nn=np.full(ne, 4, np.int8), conn=tets.astype(np.int32),
+                fibers=fib, meta={"kind": "slab", "shape": (nx, ny, nz), "h": h})
+
+
+def slab_corner_nodes(nx=41, ny=41, nz=9, m=3) -> np.ndarray:
+    """Ids of the m x m x nz corner column (i < m, j < m, all k)."""
+    i, j, k = np.meshgrid(np.arange(m), np.arange(m), np.arange(nz), indexing="ij")
+    return np.sort((i + nx * j + nx * ny * k).ravel()).astype(np.int32)
+
+
+def lv_shell(n_trans=26, n_circ=240, n_long=165, base_radius=2.5, apex_depth=7.0,
+             wall=1.0, theta_apex_deg=6.0, fiber_endo_deg=60.0, fiber_epi_deg=-60.0) -> Mesh:
+    """Truncated prolate-ellipsoid LV shell with a small apical opening.
+
+    The outer surface has equatorial radius ``base_radius`` and long semi-axis
+    ``apex_depth``; the inner one is ``wall`` smaller on both axes.  The base
+    plane is z = 0 and the apex points to -z.  A structured grid in (depth,
+    phi, theta) needs no degenerate cells only if the apical pole is cut out,
+    so theta starts at ``theta_apex_deg``.  Fibers rotate linearly from
+    ``fiber_endo_deg`` at the endocardium to ``fiber_epi_deg`` at the
+    epicardium, measured from the circumferential direction toward the
+    longitudinal direction in the wall's tangent plane.
+
+    Defaults give 1,029,600 nodes and 5,771,520 tets.
+    ``meta`` carries per-node transmural depth (0 endo .. 1 epi) and
+    normalised apicobasal height (0 apex .. 1 base).
+    """
+    nt, nc, nl = n_trans, n_circ, n_long
+    if nc % 2:
+        raise ValueError("n_circ must be even")
+    d = np.linspace(0.0, 1.0, nt)                     # transmural 0 endo .. 1 epi
+    phi = np.arange(nc) * (2 * np.pi / nc)
+    th = np.linspace(np.deg2rad(theta_apex_deg), np.pi / 2, nl)  # apex .. base
+    a_in, c_in = base_radius - wall, apex_depth - wall
+    a = a_in + d * wall
+    c = c_in + d * wall
+    D, P, T = np.meshgrid(np.arange(nt), np.arange(nc), np.arange(nl), indexing="ij")
+    idx = (D + nt * P + nt * nc * T).astype(np.int64)
+    ar, cr = a[D], c[D]
+    sp, cp = np.sin(phi[P]), np.cos(phi[P])
+    st, ct = np.sin(th[T]), np.cos(th[T])
+    xyz = np.zeros((nt * nc * nl, 3))
+    xyz[idx.ravel(), 0] = (ar * st * cp).ravel()
+    xyz[idx.ravel(), 1] = (ar * st * sp).ravel()
+    xyz[idx.ravel(), 2] = (-cr * ct).ravel()
+    depth = np.zeros(nt * nc * nl)
+    depth[idx.ravel()] = d[D].ravel()
+    height = np.zeros(nt * nc * nl)
+    height[idx.ravel()] = ((th[T] - th[0]) / (th[-1] - th[0])).ravel()
+
+    ci, cj, ck = np.meshgrid(np.arange(nt - 1), np.arange(nc), np.arange(nl - 1), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+
+    def node(i, j, k):
+        return i + nt * (j % nc) + nt * nc * k
+
+    corners = np.stack([node(ci + a_, cj + b_, ck + c_)
+                        for c_ in (0, 1) for b_ in (0, 1) for a_ in (0, 1)], axis=1)
+    tets = corners[:, np.array(_SIX)].reshape(-1, 4)
+    tets = _orient(xyz, tets)
+    # Fibers at each element centroid.
+    cen = xyz[tets].mean(axis=1)
+    dep = depth[tets].mean(axis=1)
+    ang = np.deg2rad(fiber_endo_deg + (fiber_epi_deg - fiber_endo_deg) * dep)
+    r_xy = np.hypot(cen[:, 0], cen[:, 1])
+    e_c = np.stack([-cen[:, 1] / r_xy, cen[:, 0] / r_xy, np.zeros_like(r_xy)], axis=1)
+    # Outward normal of the confocal-ish ellipsoid through the centroid.
+    aa = a_in + dep * wall
+    cc = c_in + dep * wall
+    nrm = np.stack([cen[:, 0] / aa ** 2, cen[:, 1] / aa ** 2, cen[:, 2] / cc ** 2], axis=1)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    e_l = np.cross(nrm, e_c)
+    e_l /= np.linalg.norm(e_l, axis=1, keepdims=True)
+    f = np.cos(ang)[:, None] * e_c + np.sin(ang)[:, None] * e_l
+    s = nrm
+    n = np.cross(f, s)
+    fib = np.concatenate([f, s, n], axis=1)
+    ne = tets.shape[0]
+    return Mesh(xyz=xyz, elem_nn=np.full(ne, 4, np.int8), conn=tets.astype(np.int32),
+                fibers=fib,
+                meta={"kind": "lv_shell", "shape": (nt, nc, nl), "depth": depth,
+                      "height": height})
+
+
+def electrodes_around(center, radius, n, axis_z=0.0) -> np.ndarray:
+    """n electrode positions on a circle of ``radius`` around ``center``."""
+    ang = np.arange(n) * (2 * np.pi / n)
+    c = np.asarray(center, dtype=np.float64)
+    return np.stack([c[0] + radius * np.cos(ang), c[1] + radius * np.sin(ang),
+                     np.full(n, c[2] + axis_z)], axis=1)

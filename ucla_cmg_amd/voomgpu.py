@@ -1,0 +1,421 @@
+"""Python host mirror of libvoomgpu (ctypes over the C ABI in include/voomgpu.h).
+
+The classes keep the reference's operator vocabulary so a driver loop reads
+like ``EPSimulator.cc:507-596``:
+
+* :class:`CardiacSolver` -- ``CardiacBody`` + ``EP_CrankNicholson`` on one
+  device: ``Solve``, ``setStimulusCurrent`` (4 overloads), ``InitializeVoltage``,
+  ``getVoltage``, ``getStateVariables`` / ``setStateVariables``, ``computeECG``.
+* :class:`IonicCells` -- ``IonicModel::Compute_Ion`` on independent cells.
+
+There is no CPU fallback: every call runs the HIP library, and constructing a
+solver without the built library or a visible GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoomgpu.so")
+
+VG_OK = 0
+MODEL_LR = 0
+MODEL_TT = 2
+PREC_FP64 = 0
+PREC_MIXED = 1
+INTEG_REFERENCE = 0
+INTEG_RUSH_LARSEN = 1
+QUIRK_V0_ZERO = 1
+QUIRK_PMJ_STICKY = 2
+QUIRK_BAR_1X_VOLUME = 4
+TT_NSCALE = 5
+KC_RHS, KC_CG_SPMV, KC_CG_UPDATE, KC_IONIC, KC_APPLY = range(5)
+KC_NAMES = ["rhs", "cg_spmv", "cg_update", "ionic", "apply"]
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTED = [
+    "vg_version", "vg_create", "vg_destroy", "vg_last_error", "vg_get_stream",
+    "vg_set_stimulus_nodes", "vg_set_stimulus_sphere", "vg_set_stimulus_axis",
+    "vg_set_stimulus_plane", "vg_clear_stimulus", "vg_step", "vg_synchronize",
+    "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
+    "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
+    "vg_ecg_setup", "vg_ecg", "vg_ecg_record", "vg_ecg_frames", "vg_profile_enable",
+    "vg_profile_get", "vg_stats_get", "vg_assemble_csr", "vg_cells_create",
+    "vg_cells_destroy", "vg_cells_set_scales", "vg_cells_compute_ion",
+    "vg_cells_get_states", "vg_cells_set_states",
+]
+
+
+class VgMesh(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64), ("xyz", C.c_void_p), ("n_elems", C.c_int64),
+        ("elem_nn", C.c_void_p), ("conn", C.c_void_p), ("fibers", C.c_void_p),
+        ("tissue", C.c_void_p), ("n_pmj", C.c_int64), ("pmj_node", C.c_void_p),
+        ("pmj_data", C.c_void_p), ("radius", C.c_double),
+    ]
+
+
+class VgModel(C.Structure):
+    _fields_ = [
+        ("cell_model", C.c_int), ("precision", C.c_int), ("integrator", C.c_int),
+        ("D", C.c_double * 3), ("Dp", C.c_double * 3), ("DpLeft", C.c_double * 3),
+        ("Cm", C.c_double), ("dt", C.c_double), ("dt_ionic", C.c_double),
+        ("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("quirks", C.c_uint),
+        ("n_param_sets", C.c_int), ("param_scales", C.c_void_p),
+        ("param_index", C.c_void_p), ("sell_sigma", C.c_int),
+    ]
+
+
+class VgStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("cg_iterations", C.c_int64), ("cg_solves", C.c_int64),
+        ("cg_not_converged", C.c_int64), ("newton_iterations", C.c_int64),
+        ("n_nodes", C.c_int64), ("nnz", C.c_int64), ("sell_padded_nnz", C.c_int64),
+        ("last_cg_relres", C.c_double), ("sim_time", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libvoomgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    H = C.c_void_p
+    sig = {
+        "vg_version": ([], C.c_char_p),
+        "vg_create": ([C.POINTER(VgMesh), C.POINTER(VgModel), C.POINTER(C.c_void_p)], C.c_int),
+        "vg_destroy": ([H], None),
+        "vg_last_error": ([H], C.c_char_p),
+        "vg_get_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
+        "vg_set_stimulus_nodes": ([H, C.c_void_p, C.c_int64, C.c_double], C.c_int),
+        "vg_set_stimulus_sphere": ([H, dp, C.c_double, C.c_double], C.c_int),
+        "vg_set_stimulus_axis": ([H, C.c_int, C.c_double, C.c_double], C.c_int),
+        "vg_set_stimulus_plane": ([H, dp, dp, C.c_double], C.c_int),
+        "vg_clear_stimulus": ([H], C.c_int),
+        "vg_step": ([H, C.c_int], C.c_int),
+        "vg_synchronize": ([H], C.c_int),
+        "vg_get_voltage": ([H, C.c_void_p], C.c_int),
+        "vg_set_voltage": ([H, C.c_void_p], C.c_int),
+        "vg_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
+        "vg_set_states": ([H, C.c_int, C.c_void_p], C.c_int),
+        "vg_get_lumped_capacitance": ([H, C.c_void_p], C.c_int),
+        "vg_activation_enable": ([H, C.c_double], C.c_int),
+        "vg_get_activation": ([H, C.c_void_p], C.c_int),
+        "vg_ecg_setup": ([H, C.POINTER(VgMesh), C.c_void_p, C.c_int, dp, C.c_double], C.c_int),
+        "vg_ecg": ([H, C.c_void_p], C.c_int),
+        "vg_ecg_record": ([H, C.c_int, C.c_int], C.c_int),
+        "vg_ecg_frames": ([H, C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+        "vg_profile_enable": ([H, C.c_int], C.c_int),
+        "vg_profile_get": ([H, C.c_int, dp, C.POINTER(C.c_int64)], C.c_int),
+        "vg_stats_get": ([H, C.POINTER(VgStats)], C.c_int),
+        "vg_assemble_csr": ([C.POINTER(VgMesh), C.POINTER(VgModel), C.POINTER(C.c_int64),
+                             C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p], C.c_int),
+        "vg_cells_create": ([C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
+        "vg_cells_destroy": ([H], None),
+        "vg_cells_set_scales": ([H, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+        "vg_cells_compute_ion": ([H, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_void_p], C.c_int),
+        "vg_cells_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
+        "vg_cells_set_states": ([H, C.c_int, C.c_void_p], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+class _MeshView:
+    """Keeps contiguous copies of a mesh's arrays alive behind a VgMesh."""
+
+    def __init__(self, mesh):
+        self.xyz = np.ascontiguousarray(mesh.xyz, np.float64)
+        self.elem_nn = np.ascontiguousarray(mesh.elem_nn, np.int8)
+        self.conn = np.ascontiguousarray(mesh.conn, np.int32)
+        self.fibers = None if mesh.fibers is None else np.ascontiguousarray(mesh.fibers, np.float64)
+        self.tissue = None if mesh.tissue is None else np.ascontiguousarray(mesh.tissue, np.int8)
+        pn, pd = getattr(mesh, "pmj_node", None), getattr(mesh, "pmj_data", None)
+        self.pmj_node = None if pn is None else np.ascontiguousarray(pn, np.int32)
+        self.pmj_data = None if pd is None else np.ascontiguousarray(pd, np.float64)
+        m = VgMesh()
+        m.n_nodes = self.xyz.shape[0]
+        m.xyz = _p(self.xyz)
+        m.n_elems = self.elem_nn.shape[0]
+        m.elem_nn = _p(self.elem_nn)
+        m.conn = _p(self.conn)
+        m.fibers = _p(self.fibers)
+        m.tissue = _p(self.tissue)
+        m.n_pmj = 0 if self.pmj_node is None else self.pmj_node.shape[0]
+        m.pmj_node = _p(self.pmj_node)
+        m.pmj_data = _p(self.pmj_data)
+        m.radius = float(getattr(mesh, "radius", 0.04))
+        self.struct = m
+
+
+def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERENCE,
+               D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
+               dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
+               param_index=None, sell_sigma=0):
+    """Builds a VgModel; returns (struct, keep-alive list)."""
+    m = VgModel()
+    m.cell_model = cell_model
+    m.precision = precision
+    m.integrator = integrator
+    m.D = _d3(D)
+    m.Dp = _d3(Dp)
+    m.DpLeft = _d3(DpLeft if DpLeft is not None else Dp)
+    m.Cm = Cm
+    m.dt = dt
+    m.dt_ionic = dt if dt_ionic is None else dt_ionic
+    m.cg_tol = cg_tol
+    m.cg_maxit = cg_maxit
+    m.quirks = quirks
+    keep = []
+    if param_scales is not None:
+        ps = np.ascontiguousarray(param_scales, np.float64).reshape(-1, TT_NSCALE)
+        keep.append(ps)
+        m.n_param_sets = ps.shape[0]
+        m.param_scales = _p(ps)
+    if param_index is not None:
+        pi = np.ascontiguousarray(param_index, np.uint16)
+        keep.append(pi)
+        m.param_index = _p(pi)
+    m.sell_sigma = sell_sigma
+    return m, keep
+
+
+def assemble_csr(mesh, **model_kw):
+    """Host-only assembly of K (with Cm), M and C_L through the library."""
+    L = load_library()
+    mv = _MeshView(mesh)
+    mod, keep = make_model(**model_kw)
+    n, nnz = C.c_int64(), C.c_int64()
+    rc = L.vg_assemble_csr(C.byref(mv.struct), C.byref(mod), C.byref(n), C.byref(nnz),
+                           None, None, None, None, None)
+    if rc != VG_OK:
+        raise ValueError(f"vg_assemble_csr failed ({rc})")
+    rp = np.zeros(n.value + 1, np.int64)
+    col = np.zeros(nnz.value, np.int32)
+    K = np.zeros(nnz.value)
+    M = np.zeros(nnz.value)
+    CL = np.zeros(n.value)
+    rc = L.vg_assemble_csr(C.byref(mv.struct), C.byref(mod), C.byref(n), C.byref(nnz),
+                           _p(rp), _p(col), _p(K), _p(M), _p(CL))
+    if rc != VG_OK:
+        raise ValueError(f"vg_assemble_csr failed ({rc})")
+    return rp, col, K, M, CL
+
+
+class VoomError(RuntimeError):
+    pass
+
+
+class CardiacSolver:
+    """``CardiacBody`` + ``EP_CrankNicholson`` resident on the current GPU."""
+
+    def __init__(self, mesh, **model_kw):
+        self.L = load_library()
+        self._mesh = _MeshView(mesh)
+        self._model, self._keep = make_model(**model_kw)
+        h = C.c_void_p()
+        rc = self.L.vg_create(C.byref(self._mesh.struct), C.byref(self._model), C.byref(h))
+        self.h = h.value
+        if rc != VG_OK:
+            msg = self.L.vg_last_error(self.h).decode() if self.h else "vg_create failed"
+            if self.h:
+                self.L.vg_destroy(self.h)
+                self.h = None
+            raise VoomError(f"vg_create: {msg} ({rc})")
+        self.n = self._mesh.xyz.shape[0]
+        self.cell_model = self._model.cell_model
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != VG_OK:
+            raise VoomError(f"{what}: {self.L.vg_last_error(self.h).decode()} ({rc})")
+
+    # -- CardiacBody::setStimulusCurrent (CardiacBody.cc:243-324) -------------
+    def setStimulusCurrent(self, *args):
+        """Overloads: (nodeSet, istim), (center[3], radius, istim),
+        (axis:int, limit, istim), (origin[3], normal[3], istim)."""
+        if len(args) == 2:
+            nodes = np.ascontiguousarray(args[0], np.int32)
+            self._chk(self.L.vg_set_stimulus_nodes(self.h, _p(nodes), nodes.size, float(args[1])),
+                      "set_stimulus_nodes")
+        elif len(args) == 3 and isinstance(args[0], (int, np.integer)):
+            self._chk(self.L.vg_set_stimulus_axis(self.h, int(args[0]), float(args[1]),
+                                                  float(args[2])), "set_stimulus_axis")
+        elif len(args) == 3 and np.ndim(args[1]) == 0:
+            self._chk(self.L.vg_set_stimulus_sphere(self.h, _d3(args[0]), float(args[1]),
+                                                    float(args[2])), "set_stimulus_sphere")
+        elif len(args) == 3:
+            self._chk(self.L.vg_set_stimulus_plane(self.h, _d3(args[0]), _d3(args[1]),
+                                                   float(args[2])), "set_stimulus_plane")
+        else:
+            raise TypeError("unsupported setStimulusCurrent overload")
+
+    def clearStimulus(self):
+        self._chk(self.L.vg_clear_stimulus(self.h), "clear_stimulus")
+
+    # -- EP_CrankNicholson::Solve ---------------------------------------------
+    def Solve(self, nsteps: int = 1):
+        self._chk(self.L.vg_step(self.h, int(nsteps)), "step")
+
+    def synchronize(self):
+        self._chk(self.L.vg_synchronize(self.h), "synchronize")
+
+    def getVoltage(self) -> np.ndarray:
+        v = np.zeros(self.n)
+        self._chk(self.L.vg_get_voltage(self.h, _p(v)), "get_voltage")
+        return v
+
+    def InitializeVoltage(self, V):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(V, np.float64), (self.n,)))
+        self._chk(self.L.vg_set_voltage(self.h, _p(v)), "set_voltage")
+
+    def getStateVariables(self) -> np.ndarray:
+        nv = C.c_int()
+        self._chk(self.L.vg_get_states(self.h, C.byref(nv), None), "get_states")
+        st = np.zeros((self.n, nv.value))
+        self._chk(self.L.vg_get_states(self.h, C.byref(nv), _p(st)), "get_states")
+        return st
+
+    def setStateVariables(self, st):
+        st = np.ascontiguousarray(st, np.float64)
+        self._chk(self.L.vg_set_states(self.h, st.shape[1], _p(st)), "set_states")
+
+    def lumpedCapacitance(self) -> np.ndarray:
+        c = np.zeros(self.n)
+        self._chk(self.L.vg_get_lumped_capacitance(self.h, _p(c)), "get_lumped_capacitance")
+        return c
+
+    def enableActivation(self, threshold=-40.0):
+        self._chk(self.L.vg_activation_enable(self.h, threshold), "activation_enable")
+
+    def activationTimes(self) -> np.ndarray:
+        a = np.zeros(self.n)
+        self._chk(self.L.vg_get_activation(self.h, _p(a)), "get_activation")
+        return a
+
+    def setupECG(self, electrodes, D=(1e-3, 5e-4, 2.5e-4), radius=0.0275):
+        el = np.ascontiguousarray(electrodes, np.float64).reshape(-1, 3)
+        self.n_electrodes = el.shape[0]
+        self._chk(self.L.vg_ecg_setup(self.h, C.byref(self._mesh.struct), _p(el), el.shape[0],
+                                      _d3(D), radius), "ecg_setup")
+
+    def computeECG(self) -> np.ndarray:
+        out = np.zeros(self.n_electrodes)
+        self._chk(self.L.vg_ecg(self.h, _p(out)), "ecg")
+        return out
+
+    def recordECG(self, steps_per_frame: int, max_frames: int):
+        self._ecg_max = max_frames
+        self._chk(self.L.vg_ecg_record(self.h, steps_per_frame, max_frames), "ecg_record")
+
+    def ecgFrames(self) -> np.ndarray:
+        nf = C.c_int()
+        out = np.zeros((self._ecg_max, self.n_electrodes))
+        self._chk(self.L.vg_ecg_frames(self.h, _p(out), C.byref(nf)), "ecg_frames")
+        return out[: nf.value]
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self._chk(self.L.vg_get_stream(self.h, C.byref(s)), "get_stream")
+        return s.value or 0
+
+    def profile(self, enable: bool):
+        self._chk(self.L.vg_profile_enable(self.h, int(enable)), "profile_enable")
+
+    def profileGet(self, kc: int):
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(self.L.vg_profile_get(self.h, kc, C.byref(ms), C.byref(n)), "profile_get")
+        return ms.value, n.value
+
+    def stats(self) -> dict:
+        s = VgStats()
+        self._chk(self.L.vg_stats_get(self.h, C.byref(s)), "stats")
+        return {k: getattr(s, k) for k, _ in VgStats._fields_}
+
+
+class IonicCells:
+    """``IonicModel::Compute_Ion`` for n independent cells on the device."""
+
+    def __init__(self, cell_model, n, precision=PREC_FP64, integrator=INTEG_REFERENCE):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.vg_cells_create(cell_model, precision, integrator, n, C.byref(h))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_create failed ({rc})")
+        self.h = h.value
+        self.n = n
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.vg_cells_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_scales(self, scales, index=None):
+        sc = np.ascontiguousarray(scales, np.float64).reshape(-1, TT_NSCALE)
+        pi = None if index is None else np.ascontiguousarray(index, np.uint16)
+        rc = self.L.vg_cells_set_scales(self.h, sc.shape[0], _p(sc), _p(pi))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_set_scales failed ({rc})")
+
+    def compute_ion(self, Cm, dt, V, istim=None) -> np.ndarray:
+        V = np.ascontiguousarray(np.broadcast_to(np.asarray(V, np.float64), (self.n,)))
+        st = None if istim is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(istim, np.float64), (self.n,)))
+        out = np.zeros(self.n)
+        rc = self.L.vg_cells_compute_ion(self.h, Cm, dt, _p(V), _p(st), _p(out))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_compute_ion failed ({rc})")
+        return out
+
+    def get_states(self) -> np.ndarray:
+        nv = C.c_int()
+        self.L.vg_cells_get_states(self.h, C.byref(nv), None)
+        st = np.zeros((self.n, nv.value))
+        rc = self.L.vg_cells_get_states(self.h, C.byref(nv), _p(st))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_get_states failed ({rc})")
+        return st
+
+    def set_states(self, st):
+        st = np.ascontiguousarray(st, np.float64)
+        rc = self.L.vg_cells_set_states(self.h, st.shape[1], _p(st))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_set_states failed ({rc})")
