@@ -69,6 +69,10 @@ extern "C" {
 #define VG_QUIRK_BAR_1X_VOLUME 4u /* integrate bars with pi r^2 L instead of
                                      the reference's 2 pi r^2 L
                                      (MixedMesh3D.cc:97-107) */
+#define VG_QUIRK_NEWTON_JT 8u     /* tT reference integrator: solve each Newton
+                                     step with J^T like the reference
+                                     (Tusscher.cc:172-181); leaves the physical
+                                     range during an upstroke at dt = 0.02 */
 
 /* Per-node conductance scalings of the ten Tusscher model (vg_model). */
 #define VG_TT_SCALE_GNA 0
@@ -194,6 +198,7 @@ int vg_cells_create(int cell_model, int precision, int integrator, int64_t n_cel
 void vg_cells_destroy(vg_cells *c);
 int vg_cells_set_scales(vg_cells *c, int n_param_sets, const double *param_scales,
                         const uint16_t *param_index);
+int vg_cells_set_quirks(vg_cells *c, unsigned quirks);
 int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V,
                          const double *istim, double *dVdt);
 int vg_cells_get_states(vg_cells *c, int *nvar, double *states);

@@ -1,0 +1,213 @@
+// voomgpu.hpp -- C++ host classes over the C ABI (voomgpu.h) that keep the
+// reference's interface, so a driver written against voomEP's CardiacBody /
+// EPSolver / IonicModel (Body/CardiacBody.h:82-353, Solvers/EPSolver.h:51-308,
+// IonicModel/IonicModel.h:16-50) runs on the GPU with std::vector in place of
+// Epetra vectors.  Header-only; link with -lvoomgpu.  Errors throw
+// voomgpu::Error carrying vg_last_error().
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "voomgpu.h"
+
+namespace voomgpu {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// Mesh arrays owned by the caller's containers (0-based ids).
+struct Mesh {
+  std::vector<double> xyz;      // 3 per node
+  std::vector<int8_t> elem_nn;  // 4 (tet) or 2 (bar)
+  std::vector<int32_t> conn;    // 4 slots per element
+  std::vector<double> fibers;   // 9 per element, or empty = isotropic
+  std::vector<int8_t> tissue;   // per node, or empty
+  std::vector<int32_t> pmj_node;
+  std::vector<double> pmj_data;  // 4 per PMJ: cosines, length
+  double radius = 0.04;
+
+  vg_mesh view() const {
+    vg_mesh m{};
+    m.n_nodes = (int64_t)(xyz.size() / 3);
+    m.xyz = xyz.data();
+    m.n_elems = (int64_t)elem_nn.size();
+    m.elem_nn = elem_nn.data();
+    m.conn = conn.data();
+    m.fibers = fibers.empty() ? nullptr : fibers.data();
+    m.tissue = tissue.empty() ? nullptr : tissue.data();
+    m.n_pmj = (int64_t)pmj_node.size();
+    m.pmj_node = pmj_node.empty() ? nullptr : pmj_node.data();
+    m.pmj_data = pmj_data.empty() ? nullptr : pmj_data.data();
+    m.radius = radius;
+    return m;
+  }
+};
+
+inline vg_model default_model(int cell_model) {
+  vg_model m{};
+  m.cell_model = cell_model;
+  m.precision = VG_PREC_FP64;
+  m.integrator = VG_INTEG_REFERENCE;
+  // EPSimulator.cc:84-113 / CardiacBody.cc:22-27 defaults.
+  for (int i = 0; i < 3; i++) {
+    m.D[i] = 0.001;
+    m.Dp[i] = 0.0016;
+    m.DpLeft[i] = 0.0016;
+  }
+  m.Cm = 1.0;
+  m.dt = 0.1;
+  m.dt_ionic = 0.1;
+  m.cg_tol = 1e-7;  // EP_CrankNicholson.cc:104
+  m.cg_maxit = 1000;
+  return m;
+}
+
+// CardiacBody + EP_CrankNicholson on one GPU.
+class GpuCNSolver {
+ public:
+  GpuCNSolver(const Mesh &mesh, const vg_model &model) : mesh_(mesh) {
+    const vg_mesh mv = mesh_.view();
+    const int rc = vg_create(&mv, &model, &h_);
+    if (rc != VG_OK) {
+      std::string msg = h_ ? vg_last_error(h_) : "vg_create failed";
+      if (h_) vg_destroy(h_);
+      h_ = nullptr;
+      throw Error(rc, "vg_create: " + msg);
+    }
+    n_ = (size_t)mv.n_nodes;
+  }
+  ~GpuCNSolver() {
+    if (h_) vg_destroy(h_);
+  }
+  GpuCNSolver(const GpuCNSolver &) = delete;
+  GpuCNSolver &operator=(const GpuCNSolver &) = delete;
+
+  // EP_CrankNicholson::Solve (EP_CrankNicholson.cc:78-156), n times.
+  void Solve(int nsteps = 1) { check(vg_step(h_, nsteps), "Solve"); }
+
+  // CardiacBody::setStimulusCurrent overloads (CardiacBody.cc:243-324).
+  void setStimulusCurrent(const std::vector<int32_t> &nodeSet, double istim = 0.0) {
+    check(vg_set_stimulus_nodes(h_, nodeSet.data(), (int64_t)nodeSet.size(), istim), "stimulus");
+  }
+  void setStimulusCurrent(const double xc[3], double rad, double stim = 0.0) {
+    check(vg_set_stimulus_sphere(h_, xc, rad, stim), "stimulus");
+  }
+  void setStimulusCurrent(int axis, double limit, double stim = 0.0) {
+    check(vg_set_stimulus_axis(h_, axis, limit, stim), "stimulus");
+  }
+  void setStimulusCurrent(const double xo[3], const double xn[3], double stim = 0.0) {
+    check(vg_set_stimulus_plane(h_, xo, xn, stim), "stimulus");
+  }
+
+  // EPSolver::InitializeVoltage and the voltage view written by WriteOutput.
+  void InitializeVoltage(const std::vector<double> &V) {
+    check(vg_set_voltage(h_, V.data()), "InitializeVoltage");
+  }
+  void InitializeVoltage(double v) { InitializeVoltage(std::vector<double>(n_, v)); }
+  std::vector<double> getVoltage() {
+    std::vector<double> V(n_);
+    check(vg_get_voltage(h_, V.data()), "getVoltage");
+    return V;
+  }
+
+  // CardiacBody::getStateVariables / setStateVariables (CardiacBody.cc:438-463).
+  void getStateVariables(std::vector<int> &numVar, std::vector<std::vector<double>> &data) {
+    int nv = 0;
+    check(vg_get_states(h_, &nv, nullptr), "getStateVariables");
+    std::vector<double> flat(n_ * (size_t)nv);
+    check(vg_get_states(h_, &nv, flat.data()), "getStateVariables");
+    numVar.assign(n_, nv);
+    data.assign(n_, std::vector<double>(nv));
+    for (size_t i = 0; i < n_; i++)
+      for (int k = 0; k < nv; k++) data[i][k] = flat[i * nv + k];
+  }
+  void setStateVariables(const std::vector<std::vector<double>> &data) {
+    if (data.size() != n_ || data.empty()) throw Error(VG_ERR_ARG, "state count");
+    const int nv = (int)data[0].size();
+    std::vector<double> flat(n_ * (size_t)nv);
+    for (size_t i = 0; i < n_; i++)
+      for (int k = 0; k < nv; k++) flat[i * nv + k] = data[i][k];
+    check(vg_set_states(h_, nv, flat.data()), "setStateVariables");
+  }
+
+  std::vector<double> lumpedCapacitance() {
+    std::vector<double> c(n_);
+    check(vg_get_lumped_capacitance(h_, c.data()), "lumpedCapacitance");
+    return c;
+  }
+
+  // Pseudo-ECG (ComputeECG.h:30-189), electrodes as 3 numbers each.
+  void setupECG(const std::vector<double> &electrodes, const double D[3], double radius) {
+    const vg_mesh mv = mesh_.view();
+    ne_ = (int)(electrodes.size() / 3);
+    check(vg_ecg_setup(h_, &mv, electrodes.data(), ne_, D, radius), "setupECG");
+  }
+  std::vector<double> computeECG() {
+    std::vector<double> e(ne_);
+    check(vg_ecg(h_, e.data()), "computeECG");
+    return e;
+  }
+
+  vg_stats stats() {
+    vg_stats s{};
+    check(vg_stats_get(h_, &s), "stats");
+    return s;
+  }
+  vg_handle *handle() { return h_; }
+  size_t numNodes() const { return n_; }
+
+ private:
+  void check(int rc, const char *what) {
+    if (rc != VG_OK) throw Error(rc, std::string(what) + ": " + vg_last_error(h_));
+  }
+  Mesh mesh_;
+  vg_handle *h_ = nullptr;
+  size_t n_ = 0;
+  int ne_ = 0;
+};
+
+// IonicModel-shaped single cell (IonicModel.h:33-34): one Compute_Ion per
+// call, run on the device.  Meant for unit tests and cell studies; a tissue
+// runs all its cells in one launch inside GpuCNSolver.
+class GpuIonicModel {
+ public:
+  GpuIonicModel(int cell_model, int precision = VG_PREC_FP64,
+                int integrator = VG_INTEG_REFERENCE) {
+    const int rc = vg_cells_create(cell_model, precision, integrator, 1, &c_);
+    if (rc != VG_OK) throw Error(rc, "vg_cells_create");
+  }
+  ~GpuIonicModel() {
+    if (c_) vg_cells_destroy(c_);
+  }
+  GpuIonicModel(const GpuIonicModel &) = delete;
+  GpuIonicModel &operator=(const GpuIonicModel &) = delete;
+
+  // Returns dV/dt and advances the internal states by dt at fixed V.  Xi and
+  // userXi are accepted for signature compatibility; like the reference
+  // driver (which never sets a user Xi) the model's own ratio is used.
+  double Compute_Ion(double /*Xi*/, bool /*userXi*/, double C_m, double dt, double Volt,
+                     double istim) {
+    double dv = 0.0;
+    const int rc = vg_cells_compute_ion(c_, C_m, dt, &Volt, &istim, &dv);
+    if (rc != VG_OK) throw Error(rc, "Compute_Ion");
+    return dv;
+  }
+  std::vector<double> getInternalParameters(int &nData) {
+    vg_cells_get_states(c_, &nData, nullptr);
+    std::vector<double> s(nData);
+    vg_cells_get_states(c_, &nData, s.data());
+    return s;
+  }
+  void setInternalParameters(const std::vector<double> &data) {
+    const int rc = vg_cells_set_states(c_, (int)data.size(), data.data());
+    if (rc != VG_OK) throw Error(rc, "setInternalParameters");
+  }
+
+ private:
+  vg_cells *c_ = nullptr;
+};
+
+}  // namespace voomgpu

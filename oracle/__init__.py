@@ -81,7 +81,7 @@ class _Sim(C.Structure):
         ("A", C.c_void_p), ("B", C.c_void_p), ("rhs", C.c_void_p), ("r", C.c_void_p),
         ("z", C.c_void_p), ("p", C.c_void_p), ("q", C.c_void_p), ("diag", C.c_void_p),
         ("cg_iters_total", C.c_int64), ("newton_iters_total", C.c_int64),
-        ("nthreads", C.c_int),
+        ("nthreads", C.c_int), ("tt_newton_jt", C.c_int),
     ]
 
 
@@ -101,11 +101,13 @@ def _declare(L):
     L.or_tt_init.argtypes = [_dp]
     L.or_tt_rates.argtypes = [_dp, _dp, _dp, _dp]
     L.or_tt_compute_ion.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double,
-                                    C.c_double, C.POINTER(C.c_int)]
+                                    C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.or_tt_compute_ion.restype = C.c_double
     L.or_tt_newton_system.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
     L.or_solve6_lapack_layout.argtypes = [_dp, _dp]
     L.or_solve6_lapack_layout.restype = C.c_int
+    L.or_solve6.argtypes = [_dp, _dp, C.c_int]
+    L.or_solve6.restype = C.c_int
     L.or_sim_init.argtypes = [C.POINTER(_Sim)]
     L.or_sim_init.restype = C.c_int
     L.or_sim_free.argtypes = [C.POINTER(_Sim)]
@@ -236,10 +238,12 @@ def tt_rates(const, state, alg12=0.0):
     return alg, rates
 
 
-def tt_compute_ion(const, state, Cm, dt, V, istim):
+def tt_compute_ion(const, state, Cm, dt, V, istim, jt=False):
+    """Advances ``state`` in place; returns (dV/dt, Newton passes).  ``jt``
+    solves the Newton steps with J^T exactly as the reference does."""
     it = C.c_int()
     dv = lib().or_tt_compute_ion(np.ascontiguousarray(const, np.float64), state, Cm, dt,
-                                 V, istim, C.byref(it))
+                                 V, istim, int(bool(jt)), C.byref(it))
     return dv, it.value
 
 
@@ -255,6 +259,12 @@ def tt_newton_system(const, state, stateN, oldRates, dt, alg12=0.0):
 def solve6_lapack_layout(J, B):
     Bc = np.array(B, dtype=np.float64, copy=True)
     info = lib().or_solve6_lapack_layout(np.ascontiguousarray(J, np.float64), Bc)
+    return Bc, info
+
+
+def solve6(J, B, transpose):
+    Bc = np.array(B, dtype=np.float64, copy=True)
+    info = lib().or_solve6(np.ascontiguousarray(J, np.float64), Bc, int(bool(transpose)))
     return Bc, info
 
 
@@ -275,7 +285,8 @@ class Simulation:
 
     def __init__(self, mesh, model, dt, dt_ion=None, Cm=1.0, cg_tol=1e-10, cg_maxit=1000,
                  D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, DpLeft=None, V0=None, states=None,
-                 tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None):
+                 tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None,
+                 newton_jt=False):
         self.csr = csr if csr is not None else assemble(mesh, D=D, Dp=Dp, DpLeft=DpLeft, Cm=Cm,
                                                         pmj_sticky=pmj_sticky)
         n = self.csr.n
@@ -315,6 +326,7 @@ class Simulation:
         s.tt_const = _ptr(self.tt_const)
         s.param_idx = _ptr(self.param_idx)
         s.nthreads = nthreads
+        s.tt_newton_jt = int(bool(newton_jt))
         self.s = s
         if lib().or_sim_init(C.byref(s)) != 0:
             raise MemoryError("or_sim_init")

@@ -22,8 +22,8 @@
  *                      reference's initial state; constants and initial
  *                      state against tests/golden/ionic_tables.json
  *   or_tt_compute_ion  backward-Euler gate closed form; Newton fixed point
- *                      solves x - x^n - dt f(x) = 0; J^T linear solve against
- *                      numpy.linalg.solve
+ *                      solves x - x^n - dt f(x) = 0; J and J^T linear solves
+ *                      against numpy.linalg.solve
  *   or_lr_compute_ion  Rush-Larsen gate closed form at fixed V; Ca_i forward
  *                      Euler closed form; constants against the golden table.
  *                      The individual alpha/beta rate expressions of LR are
@@ -97,12 +97,13 @@ void or_tt_constants(double *c);
 void or_tt_init(double *s);
 void or_tt_rates(const double *c, double *state, double *alg, double *rates);
 double or_tt_compute_ion(const double *c, double *s, double Cm, double dt,
-                         double V, double istim, int *newton_iters);
+                         double V, double istim, int jt, int *newton_iters);
 /* One Newton linear solve exactly as the reference forms it (J^T). */
 void or_tt_newton_system(const double *c, double *s, const double *stateN,
                          const double *oldRates, double dt, double alg12,
                          double Jac[36], double B[6]);
 int or_solve6_lapack_layout(const double Jac[36], double B[6]);
+int or_solve6(const double Jac[36], double B[6], int transpose);
 
 /* ---- whole-step driver ------------------------------------------------- */
 typedef struct {
@@ -121,6 +122,7 @@ typedef struct {
   int64_t cg_iters_total;
   int64_t newton_iters_total;
   int nthreads;
+  int tt_newton_jt;     /* 1 = solve Newton steps with J^T like the reference */
 } or_sim;
 
 int or_sim_init(or_sim *s);

@@ -398,8 +398,8 @@ int or_cn_step(or_sim *s) {
         if (s->tt_const && s->param_idx)
           c = s->tt_const + (int64_t)OR_TT_NCONST * s->param_idx[i];
         int it = 0;
-        const double dv = or_tt_compute_ion(c, s->states + OR_TT_NSTATE * i,
-                                            s->Cm, dt, s->V[i], s->istim[i], &it);
+        const double dv = or_tt_compute_ion(c, s->states + OR_TT_NSTATE * i, s->Cm, dt,
+                                            s->V[i], s->istim[i], s->tt_newton_jt, &it);
         newton += it;
         s->V[i] += dt * dv;
       }

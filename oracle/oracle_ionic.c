@@ -279,14 +279,15 @@ static const int TT_IND[12] = {4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 15, 16};
 static const int TT_DIND[12] = {34, 35, 36, 37, 38, 39, 42, 20, 21, 22, 23, 24};
 static const int TT_NR[6] = {1, 2, 3, 10, 17, 18};
 
-/* Gaussian elimination with partial pivoting on the 6x6 system exactly as a
- * column-major LAPACK dgesv sees the reference's row-major array: entry
- * (r,c) of the LAPACK matrix is Jac[c*6 + r].  Returns 0, or k+1 for an exact
- * zero pivot in column k (LAPACK's INFO convention). */
-int or_solve6_lapack_layout(const double Jac[36], double B[6]) {
+/* Gaussian elimination with partial pivoting on a 6x6 system held row-major
+ * in Jac (Jac[j*6+i] = dg_j/dx_i).  transpose = 1 solves the system exactly
+ * as a column-major LAPACK dgesv sees that array (entry (r,c) = Jac[c*6+r],
+ * i.e. J^T, the reference's behaviour); transpose = 0 solves J d = B.
+ * Returns 0, or k+1 for an exact zero pivot in column k (LAPACK's INFO). */
+int or_solve6(const double Jac[36], double B[6], int transpose) {
   double a[6][6];
   for (int r = 0; r < 6; r++)
-    for (int c = 0; c < 6; c++) a[r][c] = Jac[c * 6 + r];
+    for (int c = 0; c < 6; c++) a[r][c] = transpose ? Jac[c * 6 + r] : Jac[r * 6 + c];
   int info = 0;
   for (int k = 0; k < 6; k++) {
     int p = k;
@@ -326,6 +327,8 @@ int or_solve6_lapack_layout(const double Jac[36], double B[6]) {
   return 0;
 }
 
+int or_solve6_lapack_layout(const double Jac[36], double B[6]) { return or_solve6(Jac, B, 1); }
+
 /* The FD Jacobian and residual of one Newton pass (Tusscher.cc:157-178).
  * On return the state is restored the way the reference restores it. */
 void or_tt_newton_system(const double *C, double *S, const double *stateN,
@@ -352,11 +355,15 @@ void or_tt_newton_system(const double *C, double *S, const double *stateN,
 
 /* Tusscher::Compute_Ion + UpdateStateVariables (Tusscher.cc:116-213):
  * backward-Euler gates, then Newton on the six concentrations with a central
- * FD Jacobian solved as J^T (the row-major/column-major mix-up of :172-181),
- * stopping at ||dx||2 <= 1e-6.  The reference has no iteration cap; this
- * restatement stops at 500 passes and reports the count. */
+ * FD Jacobian, stopping at ||dx||2 <= 1e-6.  jt = 1 solves each step with
+ * J^T as the reference does (row-major array handed to column-major dgesv,
+ * :172-181); that iteration leaves the physical range during a normal action
+ * potential at dt = 0.02 ms (Ca_i < 0, then NaN), so jt = 0 (a true Newton
+ * step towards the same backward-Euler fixed point) is the default of the
+ * whole-step driver.  The reference has no iteration cap; this restatement
+ * stops at 500 passes and reports the count. */
 double or_tt_compute_ion(const double *C, double *S, double Cm, double dt,
-                         double volt, double istim, int *newton_iters) {
+                         double volt, double istim, int jt, int *newton_iters) {
   double alg[OR_TT_NALG], rates[OR_TT_NSTATE];
   memset(alg, 0, sizeof(alg));
   memset(rates, 0, sizeof(rates));
@@ -389,7 +396,7 @@ double or_tt_compute_ion(const double *C, double *S, double Cm, double dt,
       for (int j = 0; j < 6; j++) Jac[j * 6 + i] = (fxph[j] - fxmh[j]) / twoeps;
     }
     for (int i = 0; i < 6; i++) B[i] = S[TT_NR[i]] - StateN[i] - dt * oldRates[i];
-    or_solve6_lapack_layout(Jac, B);
+    or_solve6(Jac, B, jt);
     for (int i = 0; i < 6; i++) S[TT_NR[i]] -= B[i];
     or_tt_rates(C, S, alg, rates);
     for (int i = 0; i < 6; i++) oldRates[i] = rates[TT_NR[i]];

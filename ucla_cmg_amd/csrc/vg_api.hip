@@ -126,6 +126,7 @@ struct vg_handle {
 
 struct vg_cells {
   int prec, model, integ;
+  int newton_jt = 0;
   int64_t n, n_alloc;
   DevBuf states, scales, pidx, V, stim, dvdt;
   int nsets = 1;
@@ -240,7 +241,7 @@ int cg_solve(vg_handle *h) {
 
 template <class T, int MODEL, int INTEG>
 void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
-  IonicArgs a{h->n, h->n_alloc, h->nsets, h->Cm, 500};
+  IonicArgs a{h->n, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0};
   hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
                      Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
@@ -869,6 +870,12 @@ int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
 
 void vg_cells_destroy(vg_cells *c) { delete c; }
 
+int vg_cells_set_quirks(vg_cells *c, unsigned quirks) {
+  if (!c) return VG_ERR_ARG;
+  c->newton_jt = (quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0;
+  return VG_OK;
+}
+
 int vg_cells_set_scales(vg_cells *c, int nsets, const double *sc, const uint16_t *pidx) {
   if (!c || nsets < 1 || !sc) return VG_ERR_ARG;
   const size_t T = c->prec == VG_PREC_FP64 ? 8 : 4;
@@ -907,7 +914,7 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, con
 #define VG_CELLS(T, M, I)                                                                      \
   hipLaunchKernelGGL((k_cells<T, M, I>), dim3(nb), dim3(BLK), 0, 0, c->V.as<double>(), sp,      \
                      c->dvdt.as<double>(), c->states.as<T>(), pi, c->scales.as<T>(), n,         \
-                     c->n_alloc, Cm, dt, 500)
+                     c->n_alloc, Cm, dt, 500, c->newton_jt)
   if (c->prec == VG_PREC_FP64) {
     if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(double, VG_MODEL_TT, VG_INTEG_REFERENCE);

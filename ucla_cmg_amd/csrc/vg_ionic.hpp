@@ -317,14 +317,15 @@ __device__ inline T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TT
   return R7[0];
 }
 
-// 6x6 solve of the system a column-major LAPACK dgesv sees for the row-major
-// array J (entry (r,c) = J[c*6+r]); partial pivoting, B overwritten.
-template <class T> __device__ inline void tt_solve6(const T J[36], T B[6]) {
+// 6x6 solve with partial pivoting of J d = B for the row-major array J
+// (J[j*6+i] = dg_j/dx_i), or, with transpose, of the system a column-major
+// LAPACK dgesv sees for that array (entry (r,c) = J[c*6+r]); B overwritten.
+template <class T> __device__ inline void tt_solve6(const T J[36], T B[6], bool transpose) {
   T a[6][6];
 #pragma unroll
   for (int r = 0; r < 6; r++)
 #pragma unroll
-    for (int c = 0; c < 6; c++) a[r][c] = J[c * 6 + r];
+    for (int c = 0; c < 6; c++) a[r][c] = transpose ? J[c * 6 + r] : J[r * 6 + c];
   for (int k = 0; k < 6; k++) {
     int p = k;
     T best = fabs(a[k][k]);
@@ -357,11 +358,14 @@ template <class T> __device__ inline void tt_solve6(const T J[36], T B[6]) {
 }
 
 // The reference integrator (Tusscher.cc:116-195): backward-Euler gates, then
-// Newton on the six concentrations with a central-difference Jacobian solved
-// as J^T, stopping at ||dx||2 <= 1e-6 (capped at max_newton passes).
+// Newton on the six concentrations with a central-difference Jacobian,
+// stopping at ||dx||2 <= 1e-6 (capped at max_newton passes).  jt solves each
+// step with J^T as the reference's row-major/column-major mix-up does
+// (:172-181); by default the step solves J, which converges to the same
+// backward-Euler solution and stays physical through the upstroke.
 template <class T>
 __device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
-                                      const TTG<T> &g, int max_newton, int *iters) {
+                                      const TTG<T> &g, int max_newton, bool jt, int *iters) {
   S[0] = V;
   T StateN[6], oldR[6], fxph[6], fxmh[6], J[36], B[6], R7[7];
 #pragma unroll
@@ -401,7 +405,7 @@ __device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) B[i] = S[TT_NR_STATE[i]] - StateN[i] - dt * oldR[i];
-    tt_solve6<T>(J, B);
+    tt_solve6<T>(J, B, jt);
 #pragma unroll
     for (int i = 0; i < 6; i++) S[TT_NR_STATE[i]] -= B[i];
     tt_currents<T, false, false>(S, istim_term, g, R7);

@@ -235,6 +235,7 @@ struct IonicArgs {
   int nsets;
   double Cm;
   int max_newton;
+  int newton_jt;
 };
 
 template <class T, int MODEL, int INTEG>
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
         T dv;
         if constexpr (INTEG == VG_INTEG_REFERENCE) {
           int nit = 0;
-          dv = tt_step_reference<T>(s, V, it, dt, g, a.max_newton, &nit);
+          dv = tt_step_reference<T>(s, V, it, dt, g, a.max_newton, a.newton_jt != 0, &nit);
           newton += nit;
         } else {
           dv = tt_step_rl<T, sizeof(T) == 4>(s, V, it, dt, g);
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, int64_t n,
                                                int64_t n_alloc, double Cm_d, double dt_d,
-                                               int max_newton) {
+                                               int max_newton, int newton_jt) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const T V = (T)Vd[i], stim = stim_d ? (T)stim_d[i] : T(0), Cm = (T)Cm_d, dt = (T)dt_d;
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
     T dv;
     if constexpr (INTEG == VG_INTEG_REFERENCE) {
       int nit = 0;
-      dv = tt_step_reference<T>(s, V, it, dt, g, max_newton, &nit);
+      dv = tt_step_reference<T>(s, V, it, dt, g, max_newton, newton_jt != 0, &nit);
     } else {
       dv = tt_step_rl<T, sizeof(T) == 4>(s, V, it, dt, g);
     }

@@ -32,6 +32,7 @@ INTEG_RUSH_LARSEN = 1
 QUIRK_V0_ZERO = 1
 QUIRK_PMJ_STICKY = 2
 QUIRK_BAR_1X_VOLUME = 4
+QUIRK_NEWTON_JT = 8
 TT_NSCALE = 5
 KC_RHS, KC_CG_SPMV, KC_CG_UPDATE, KC_IONIC, KC_APPLY = range(5)
 KC_NAMES = ["rhs", "cg_spmv", "cg_update", "ionic", "apply"]
@@ -45,7 +46,7 @@ EXPORTED = [
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
     "vg_ecg_setup", "vg_ecg", "vg_ecg_record", "vg_ecg_frames", "vg_profile_enable",
     "vg_profile_get", "vg_stats_get", "vg_assemble_csr", "vg_cells_create",
-    "vg_cells_destroy", "vg_cells_set_scales", "vg_cells_compute_ion",
+    "vg_cells_destroy", "vg_cells_set_scales", "vg_cells_set_quirks", "vg_cells_compute_ion",
     "vg_cells_get_states", "vg_cells_set_states",
 ]
 
@@ -125,6 +126,7 @@ def load_library(path: str = LIB_PATH):
         "vg_cells_create": ([C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
         "vg_cells_destroy": ([H], None),
         "vg_cells_set_scales": ([H, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+        "vg_cells_set_quirks": ([H, C.c_uint], C.c_int),
         "vg_cells_compute_ion": ([H, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                   C.c_void_p], C.c_int),
         "vg_cells_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
@@ -394,6 +396,11 @@ class IonicCells:
         rc = self.L.vg_cells_set_scales(self.h, sc.shape[0], _p(sc), _p(pi))
         if rc != VG_OK:
             raise VoomError(f"vg_cells_set_scales failed ({rc})")
+
+    def set_quirks(self, quirks: int):
+        rc = self.L.vg_cells_set_quirks(self.h, quirks)
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_set_quirks failed ({rc})")
 
     def compute_ion(self, Cm, dt, V, istim=None) -> np.ndarray:
         V = np.ascontiguousarray(np.broadcast_to(np.asarray(V, np.float64), (self.n,)))
