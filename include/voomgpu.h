@@ -49,6 +49,9 @@ extern "C" {
  * (CardiacBody.cc:988-1093). */
 #define VG_MODEL_LR 0
 #define VG_MODEL_TT 2
+/* Corrias rabbit Purkinje (IonicModel/Purkinje.cc); in a tissue the model of
+ * every Purkinje node (tissue 11/12), here also selectable for vg_cells_*. */
+#define VG_MODEL_PURKINJE 10
 
 /* Arithmetic of the hot path. */
 #define VG_PREC_FP64 0  /* every array and operation in FP64 */
@@ -159,8 +162,15 @@ int vg_synchronize(vg_handle *h);
 
 int vg_get_voltage(vg_handle *h, double *V);
 int vg_set_voltage(vg_handle *h, const double *V);
+/* Uniform layout [n_nodes * nvar] (nvar of the body's model); only for
+ * meshes without Purkinje nodes. */
 int vg_get_states(vg_handle *h, int *nvar, double *states);
 int vg_set_states(vg_handle *h, int nvar, const double *states);
+/* Per-node layout of getStateVariables(numVar, data): num_var[n_nodes] and
+ * the node vectors packed in node order (Purkinje nodes carry 22 values).
+ * Call with data NULL to get num_var first. */
+int vg_get_states_ragged(vg_handle *h, int32_t *num_var, double *data);
+int vg_set_states_ragged(vg_handle *h, const int32_t *num_var, const double *data);
 int vg_get_lumped_capacitance(vg_handle *h, double *CL);
 
 int vg_activation_enable(vg_handle *h, double threshold_mV);

@@ -82,6 +82,7 @@ class _Sim(C.Structure):
         ("z", C.c_void_p), ("p", C.c_void_p), ("q", C.c_void_p), ("diag", C.c_void_p),
         ("cg_iters_total", C.c_int64), ("newton_iters_total", C.c_int64),
         ("nthreads", C.c_int), ("tt_newton_jt", C.c_int),
+        ("pk_flag", C.c_void_p), ("pk_states", C.c_void_p),
     ]
 
 
@@ -327,6 +328,16 @@ class Simulation:
         s.param_idx = _ptr(self.param_idx)
         s.nthreads = nthreads
         s.tt_newton_jt = int(bool(newton_jt))
+        tissue = getattr(mesh, "tissue", None)
+        self.pk_flag = None
+        self.pk_states = None
+        if tissue is not None and np.any((tissue == 11) | (tissue == 12)):
+            self.pk_flag = np.ascontiguousarray(((tissue == 11) | (tissue == 12)).astype(np.int8))
+            self.pk_states = np.tile(pk_init(), (n, 1))
+            s.pk_flag = self.pk_flag.ctypes.data
+            s.pk_states = self.pk_states.ctypes.data
+            if V0 is None or np.ndim(V0) == 0:
+                self.V[self.pk_flag == 1] = self.pk_states[0, 0]
         self.s = s
         if lib().or_sim_init(C.byref(s)) != 0:
             raise MemoryError("or_sim_init")
@@ -361,3 +372,51 @@ class Simulation:
         if hit.any():
             frac = (thr - Vold[hit]) / (self.V[hit] - Vold[hit])
             self.act[hit] = t0 + frac * self.dt
+
+
+# ---- Corrias rabbit Purkinje ------------------------------------------------
+PK_NSTATE = 22
+PK_NCONST = 63
+PK_NALG = 69
+
+
+def _declare_pk(L):
+    L.or_pk_constants.argtypes = [_dp]
+    L.or_pk_init.argtypes = [_dp]
+    L.or_pk_rates.argtypes = [_dp, _dp, _dp, _dp]
+    L.or_pk_compute_ion.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.or_pk_compute_ion.restype = C.c_double
+
+
+def pk_constants():
+    L = lib()
+    _declare_pk(L)
+    c = np.zeros(PK_NCONST)
+    L.or_pk_constants(c)
+    return c
+
+
+def pk_init():
+    L = lib()
+    _declare_pk(L)
+    s = np.zeros(PK_NSTATE)
+    L.or_pk_init(s)
+    return s
+
+
+def pk_rates(const, state, alg0=0.0):
+    L = lib()
+    _declare_pk(L)
+    alg = np.zeros(PK_NALG)
+    alg[0] = alg0
+    rates = np.zeros(PK_NSTATE)
+    L.or_pk_rates(np.ascontiguousarray(const, np.float64), np.ascontiguousarray(state, np.float64),
+                  alg, rates)
+    return alg, rates
+
+
+def pk_compute_ion(const, state, Cm, dt, V, istim):
+    """Advances ``state`` in place (Purkinje::Compute_Ion); returns dV/dt."""
+    L = lib()
+    _declare_pk(L)
+    return L.or_pk_compute_ion(np.ascontiguousarray(const, np.float64), state, Cm, dt, V, istim)

@@ -105,6 +105,15 @@ void or_tt_newton_system(const double *c, double *s, const double *stateN,
 int or_solve6_lapack_layout(const double Jac[36], double B[6]);
 int or_solve6(const double Jac[36], double B[6], int transpose);
 
+#define OR_PK_NSTATE 22
+#define OR_PK_NCONST 63
+#define OR_PK_NALG 69
+void or_pk_constants(double *c);
+void or_pk_init(double *s);
+void or_pk_rates(const double *c, const double *state, double *alg, double *rates);
+double or_pk_compute_ion(const double *c, double *s, double Cm, double dt, double V,
+                         double istim);
+
 /* ---- whole-step driver ------------------------------------------------- */
 typedef struct {
   int model;            /* OR_LR or OR_TT */
@@ -123,6 +132,8 @@ typedef struct {
   int64_t newton_iters_total;
   int nthreads;
   int tt_newton_jt;     /* 1 = solve Newton steps with J^T like the reference */
+  const int8_t *pk_flag; /* [n] 1 = Purkinje node (runs the Purkinje model), or NULL */
+  double *pk_states;    /* [n*22] AoS, used where pk_flag is set */
 } or_sim;
 
 int or_sim_init(or_sim *s);

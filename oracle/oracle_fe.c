@@ -382,9 +382,24 @@ int or_cn_step(or_sim *s) {
   int64_t newton = 0;
   while (total < s->dt) {
     const int64_t n = s->n;
+    if (s->pk_flag) {
+      /* CardiacBody::New (CardiacBody.cc:993-1002): Purkinje nodes run the
+       * Purkinje model whatever the body's model is. */
+      double pkc[OR_PK_NCONST];
+      or_pk_constants(pkc);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; i++) {
+        if (!s->pk_flag[i]) continue;
+        const double dv = or_pk_compute_ion(pkc, s->pk_states + OR_PK_NSTATE * i, s->Cm, dt,
+                                            s->V[i], s->istim[i]);
+        s->V[i] += dt * dv;
+      }
+    }
+    const int8_t *pkf = s->pk_flag;
     if (s->model == OR_LR) {
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < n; i++) {
+        if (pkf && pkf[i]) continue;
         const double dv = or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm,
                                             dt, s->V[i], s->istim[i]);
         s->V[i] += dt * dv;
@@ -394,6 +409,7 @@ int or_cn_step(or_sim *s) {
       or_tt_constants(tc);
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : newton)
       for (int64_t i = 0; i < n; i++) {
+        if (pkf && pkf[i]) continue;
         const double *c = tc;
         if (s->tt_const && s->param_idx)
           c = s->tt_const + (int64_t)OR_TT_NCONST * s->param_idx[i];

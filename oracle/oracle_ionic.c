@@ -408,3 +408,174 @@ double or_tt_compute_ion(const double *C, double *S, double Cm, double dt,
   if (newton_iters) *newton_iters = it;
   return rates[0];
 }
+
+/* ================= Corrias rabbit Purkinje (IonicModel/Purkinje.cc) ========== */
+/* SetUpPurkinjeParameters (IonicModel/SetUpIonicConstants.h:16-85). */
+void or_pk_constants(double *C) {
+  static const double k[57] = {
+      310.0, 96485.3415, 8314.472, 2.0,    140,     5.4,    69.0,     13266.5, 0.6,
+      0.2,   0.06,       700,      9000000, 500,    0.5,    -4320.0,  1.0,     0.9,
+      0.7,   5.4,        5.0,      350.0,  10.0,    3.0,    0.005,    2.0,     1140.0,
+      15.0,  2.0,        20.0,     1.5,    3.0,     3,      0.001,    0.001,   0.5,
+      442.2, 0.01,       0.01,     0.0001, 5.0,     0.0001, 1.5,      0.188709677,
+      0.045290323, 2500.0, 0.001, 120.0,  0.000246, 1.7,    1.6,      120.0,   0.000246,
+      1.7,   1.6,        10.0,     5000.0};
+  memcpy(C, k, sizeof(k));
+  C[57] = C[1] / (C[2] * C[0]);
+  C[58] = (C[2] * C[0]) / C[1];
+  C[59] = C[9] * C[7];
+  C[60] = C[8] * C[7];
+  C[61] = (C[8] + C[9]) * C[7];
+  C[62] = C[10] * C[7];
+}
+
+/* Purkinje constructor (Purkinje.h:44-50). */
+void or_pk_init(double *s) {
+  static const double v[OR_PK_NSTATE] = {
+      -88.34, 0.00001, 0.000032, 0.17,     6.7,      140,      0.001337, 0.01,
+      0.000003, 0.1,   0.7,      0.0,      0.7,      0.000007, 0.978861, 0.000012,
+      0.864489, 0.25,  1.0,      0.0,      0.011099, 0.011099};
+  memcpy(s, v, sizeof(v));
+}
+
+#define S state
+#define A alg
+#define R rates
+/* Purkinje::ComputeRates (Purkinje.cc:5-104); alg[0] (stimulus, pA) is an
+ * input. */
+void or_pk_rates(const double *C, const double *S, double *A, double *R) {
+  A[1] = 1.0 / (1.0 + exp((S[0] + 47.8) / -5.5));
+  R[6] = (A[1] - S[6]) / C[16];
+  A[3] = 1.0 / (1.0 + exp((S[0] + 14.6) / -5.5));
+  R[8] = (A[3] - S[8]) / C[18];
+  A[6] = 1.0 / (1.0 + exp((S[0] + -7.0) / -9.0));
+  R[11] = (A[6] - S[11]) / C[20];
+  A[7] = 1.0 / (1.0 + exp((S[0] + 27.5) / 8.0));
+  R[12] = (A[7] - S[12]) / C[21];
+  A[8] = 1.0 / (1.0 + exp((S[0] + 25.0) / -5.0));
+  R[13] = (A[8] - S[13]) / C[24];
+  A[9] = 1.0 / (1.0 + exp((S[0] + 69.0) / 3.96));
+  R[14] = (A[9] - S[14]) / C[25];
+  A[10] = 1.0 / (1.0 + exp((S[0] + 30.0) / -5.0));
+  R[15] = (A[10] - S[15]) / C[27];
+  A[2] = 1.0 / (1.0 + exp((S[0] + 67.9) / 3.87));
+  A[17] = 1.42271 * exp(-0.05119 * S[0]);
+  R[7] = (A[2] - S[7]) / A[17];
+  A[4] = 1.0 / (1.0 + exp((S[0] + 31.0) / 5.54));
+  A[18] = 25.1 / (0.04 + 0.7 * exp(-1.0 * pow(0.028 * (S[0] + 14.5), 2.0)));
+  R[9] = (A[4] - S[9]) / A[18];
+  A[5] = 0.4 + 0.6 / (1.0 + pow(S[1] / 0.0001, 2.0));
+  A[19] = 2.0 + 80.0 / (1.0 + pow(S[1] / 0.0001, 2.0));
+  R[10] = (A[5] - S[10]) / A[19];
+  A[11] = 0.1 + 0.9 / (1.0 + exp((S[0] + 75.6) / 6.3));
+  A[20] = 120.0 + 1.0 * exp((S[0] + 100.0) / 25.0);
+  R[16] = (A[11] - S[16]) / A[20];
+  A[13] = 1.0 / (1.0 + exp((S[0] - 1.5) / -16.7));
+  A[22] = (fabs(S[0] + 30.0) < 0.0145
+               ? 417.946
+               : 1.0 / ((7.19e-05 * (S[0] + 30.0)) / (1.0 - exp(-0.148 * (S[0] + 30.0))) +
+                        (0.000131 * (S[0] + 30.0)) / (exp(0.0687 * (S[0] + 30.0)) - 1.0)));
+  R[18] = (A[13] - S[18]) / A[22];
+  A[14] = 1.0 / (1.0 + exp((S[0] + 109.0) / 10.0));
+  A[23] = 6000.0 / (exp(-1.0 * (2.9 + 0.04 * S[0])) + exp(1.0 * (3.6 + 0.11 * S[0])));
+  R[20] = (A[14] - S[20]) / A[23];
+  A[15] = 1.0 / (1.0 + exp((S[0] + 109.0) / 10.0));
+  A[24] = 6000.0 / (exp(-1.0 * (2.9 + 0.04 * S[0])) + exp(1.0 * (3.6 + 0.11 * S[0])));
+  R[21] = (A[15] - S[21]) / A[24];
+  A[12] = 1.0 / (1.0 + exp((S[0] + 50.0) / -7.5));
+  A[21] = (fabs(S[0] + 7.0) > 0.001 ? (0.00138 * 1.0 * (S[0] + 7.0)) / (1.0 - exp(-0.123 * (S[0] + 7.0)))
+                                    : 0.00138 / 0.123);
+  A[26] = (fabs(S[0] + 10.0) > 0.001 ? (6.1e-05 * 1.0 * (S[0] + 10.0)) / (exp(0.145 * (S[0] + 10.0)) - 1.0)
+                                     : 0.00061 / 0.145);
+  A[29] = 1.0 / (A[21] + A[26]);
+  R[17] = (A[12] - S[17]) / A[29];
+  A[27] = A[13];
+  A[30] = 4.0 * A[22];
+  R[19] = (A[27] - S[19]) / A[30];
+  A[41] = 1.0 / (1.0 + exp((92.0 + S[0]) / 10.0));
+  A[42] = C[58] * log(C[5] / S[5]);
+  A[43] = C[29] * pow(C[5] / 5.4, 0.8) * A[41] * (S[0] - A[42]);
+  A[32] = C[58] * log(C[5] / S[5]);
+  A[33] = C[22] * S[11] * S[12] * (S[0] - A[32]);
+  A[34] = 1.0 / (1.0 + exp((5.0 - S[0]) / 17.0));
+  A[35] = C[58] * log(C[5] / S[5]);
+  A[36] = C[23] * A[34] * (S[0] - A[35]);
+  A[44] = 1.0 / (1.0 + exp((33.0 + S[0]) / 22.4));
+  A[45] = C[58] * log(C[5] / S[5]);
+  A[46] = C[30] * pow(C[5] / 5.4, 1.0) * A[44] * S[17] * (S[0] - A[45]);
+  A[47] = C[58] * log(C[5] / S[5]);
+  A[48] = C[31] * S[18] * S[19] * (S[0] - A[47]);
+  A[53] = C[58] * log(C[5] / S[5]);
+  A[54] = C[37] * (S[0] - A[53]);
+  A[50] = 1.0 / (1.0 + exp((S[0] + 80.0) / -45.0));
+  A[51] = 1.0 / (1.0 + exp((S[0] + 0.0) / 125.0));
+  A[52] = C[36] * A[50] * A[51] * (1.0 / (1.0 + pow(1.9 / C[5], 1.45))) *
+          (1.0 / (1.0 + pow(31.98 / S[4], 1.0)));
+  A[60] = C[58] * log(C[5] / S[5]);
+  A[62] = C[43] * S[20] * (S[0] - A[60]);
+  R[5] = ((-1.0 * A[33] + -1.0 * A[36] + -1.0 * A[46] + -1.0 * A[48] + -1.0 * A[43] +
+           -1.0 * A[54] + -1.0 * A[62] + -1.0 * A[0] + 2.0 * A[52]) * 1000.0) / (C[1] * C[61]);
+  A[28] = 0.5 * C[58] * log(C[3] / S[1]);
+  A[31] = C[19] * S[8] * S[9] * S[10] * (S[0] - A[28]);
+  A[16] = 0.5 * C[58] * log(C[3] / S[1]);
+  A[25] = C[17] * S[6] * S[7] * (S[0] - A[16]);
+  A[37] = C[58] * log(C[4] / S[4]);
+  A[38] = C[26] * S[13] * S[14] * (S[0] - A[37]);
+  A[39] = C[58] * log(C[4] / S[4]);
+  A[40] = C[28] * S[15] * S[16] * (S[0] - A[39]);
+  A[59] = C[40] * (1.0 / (1.0 + pow(C[41] / S[1], C[42])));
+  A[57] = 0.5 * C[58] * log(C[3] / S[1]);
+  A[58] = C[39] * (S[0] - A[57]);
+  A[64] = C[58] * log(C[4] / S[4]);
+  A[66] = C[44] * S[21] * (S[0] - A[64]);
+  A[49] = (512.009 * C[33] *
+           (exp((C[35] * (C[32] - 2.0) * S[0]) / C[58]) * pow(S[4], C[32]) * C[3] -
+            S[1] * exp(((C[35] - 1.0) * (C[32] - 2.0) * S[0]) / C[58]) * pow(C[4], C[32]))) /
+          ((1.0 + C[34] * (S[1] * pow(C[4], C[32]) + C[3] * pow(S[4], C[32]))) *
+           (1.0 + S[1] / 0.0069));
+  A[55] = C[58] * log(C[4] / S[4]);
+  A[56] = C[38] * (S[0] - A[55]);
+  R[0] = ((-1.0 * 1.0) / C[6]) *
+         (A[43] + A[33] + A[36] + A[46] + A[48] + A[54] + A[52] + A[31] + A[25] + A[38] +
+          A[40] + A[59] + A[58] + A[66] + A[62] + A[49] + A[56] + A[0]);
+  A[61] = C[45] * (S[3] / (1.0 + pow(C[46] / S[1], 2.0)));
+  A[63] = C[47] * ((pow(S[1] / C[48], C[50]) - pow(S[3] / C[49], C[50])) /
+                   (1.0 + pow(S[1] / C[48], C[50]) + pow(S[3] / C[49], C[50])));
+  A[65] = C[51] * ((pow(S[2] / C[52], C[54]) - pow(S[3] / C[53], C[54])) /
+                   (1.0 + pow(S[2] / C[52], C[54]) + pow(S[3] / C[53], C[54])));
+  A[67] = C[55] * (S[3] - S[2]);
+  R[3] = ((-1.0 * A[61] + -1.0 * A[67] + A[65] + A[63]) * 1000.0) / (2.0 * C[1] * C[62]);
+  R[4] = ((-1.0 * A[38] + -1.0 * A[40] + -3.0 * A[52] + -3.0 * A[49] + -1.0 * A[66] +
+           -1.0 * A[56]) * 1000.0) / (C[1] * C[61]);
+  A[68] = C[56] * (S[1] - S[2]);
+  R[1] = ((-1.0 * A[31] + -1.0 * A[25] + -1.0 * A[58] + 1.0 * A[61] + -1.0 * A[63] +
+           -1.0 * A[68] + 2.0 * A[49] + -1.0 * A[59]) * 1000.0) / (2.0 * C[1] * C[59]);
+  R[2] = ((-1.0 * A[65] + A[68] + A[67]) * 1000.0) / (2.0 * C[1] * C[60]);
+}
+#undef S
+#undef A
+#undef R
+
+/* Purkinje::Compute_Ion + UpdateStateVariables (Purkinje.cc:179-210): forward
+ * Euler on all 22 states (V included), with 10 sub-steps of dt/10 when
+ * |dV/dt| > 1.5 mV/ms; the last evaluated dV/dt is returned. */
+double or_pk_compute_ion(const double *C, double *S, double Cm, double dt, double volt,
+                         double istim) {
+  double alg[OR_PK_NALG], rates[OR_PK_NSTATE];
+  memset(alg, 0, sizeof(alg));
+  memset(rates, 0, sizeof(rates));
+  const double Xi = 2500., area = 4.612E-5;
+  (void)Cm;
+  S[0] = volt;
+  alg[0] = -1E6 * istim * area / Xi;
+  or_pk_rates(C, S, alg, rates);
+  if (fabs(rates[0]) > 1.5) {
+    for (int i = 0; i < 10; i++) {
+      or_pk_rates(C, S, alg, rates);
+      for (int j = 0; j < OR_PK_NSTATE; j++) S[j] += rates[j] * dt / 10;
+    }
+  } else {
+    for (int j = 0; j < OR_PK_NSTATE; j++) S[j] += rates[j] * dt;
+  }
+  return rates[0];
+}

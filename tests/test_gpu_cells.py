@@ -29,9 +29,15 @@ def test_lr_fp64_matches_oracle(gpu_device):
     n = 4
     dt = 0.01
     cells = vg.IonicCells(vg.MODEL_LR, n, vg.PREC_FP64)
+    # Bring the reference's initial table to rest first (its h gate starts
+    # at 1e-4, which blocks the upstroke), then pace.
     st = oracle.lr_init()
+    V = -84.0
+    for _ in range(50000):
+        V += dt * oracle.lr_compute_ion(st, 1.0, dt, V, 0.0)
+    st[7] = V
     cells.set_states(np.tile(st, (n, 1)))
-    V0 = np.full(n, -84.0)
+    V0 = np.full(n, V)
     so = st.copy()
     istim = 50.0 * oracle.lr_xi()
     tg, to = _pace(lambda V, s: cells.compute_ion(1.0, dt, V, s),
@@ -60,6 +66,26 @@ def test_tt_reference_integrator_fp64_matches_oracle(gpu_device):
     np.testing.assert_allclose(tg[:, 0], to, rtol=0, atol=1e-6)
     st_g = cells.get_states()[0]
     np.testing.assert_allclose(st_g[1:], so[1:], rtol=1e-7, atol=1e-12)
+
+
+def test_purkinje_fp64_matches_oracle(gpu_device):
+    """Corrias Purkinje: forward Euler on all states with adaptive x10
+    sub-steps (Purkinje.cc:179-192), against the oracle."""
+    n = 2
+    dt = 0.01
+    c = oracle.pk_constants()
+    cells = vg.IonicCells(vg.MODEL_PURKINJE, n, vg.PREC_FP64)
+    so = oracle.pk_init()
+    cells.set_states(np.tile(so, (n, 1)))
+    V0 = np.full(n, so[0])
+    istim = 50.0 * 2500.0
+    tg, to = _pace(lambda V, s: cells.compute_ion(1.0, dt, V, s),
+                   lambda V, s: oracle.pk_compute_ion(c, so, 1.0, dt, V, s),
+                   5000, dt, 100, istim, V0)
+    print(f"Purkinje fp64: peak {to.max():.2f} mV, max |dV| {np.abs(tg[:, 0] - to).max():.3e}")
+    assert to.max() > 20.0
+    np.testing.assert_allclose(tg[:, 0], to, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(cells.get_states()[0], so, rtol=1e-8, atol=1e-14)
 
 
 def _apd90(t, V):

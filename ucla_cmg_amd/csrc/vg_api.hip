@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -48,7 +49,17 @@ struct ProfileClass {
   int64_t launches = 0;
 };
 
-int nstate_of(int model) { return model == VG_MODEL_LR ? LR_NSTATE : TT_NSTATE; }
+// States per node as getInternalParameters reports them, the slot holding V
+// (kept on the device as the voltage vector instead, -1 if the model keeps its
+// own V copy) and the number of state rows stored on the device.
+int nstate_of(int model) {
+  return model == VG_MODEL_LR ? LR_NSTATE : model == VG_MODEL_PURKINJE ? PK_NSTATE : TT_NSTATE;
+}
+int vslot_of(int model) { return model == VG_MODEL_LR ? 7 : model == VG_MODEL_PURKINJE ? -1 : 0; }
+int ndev_of(int model) { return nstate_of(model) - (vslot_of(model) >= 0 ? 1 : 0); }
+const double *init_table(int model) {
+  return model == VG_MODEL_LR ? LR_INIT : model == VG_MODEL_PURKINJE ? PK_INIT : TT_INIT;
+}
 
 // Ionic sub-step sequence of EP_CrankNicholson::Solve (:114-143).
 bool substeps(double dT, double dt_ion, SubSteps &s) {
@@ -86,6 +97,16 @@ struct vg_handle {
   // device
   DevBuf off, width, col, Ahat, Krhs, sinv, diag;
   DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
+  DevBuf r2, w2, s2;  // ping-pong partners of r, w, s for the fused CG
+  // Rows [0, n_myo) run the body's cell model, rows [n_myo, n) the Purkinje
+  // model (their states in pk_states, SoA with n_pk_alloc lanes per state).
+  int64_t n_myo = 0, n_pk = 0, n_pk_alloc = 0;
+  DevBuf pk_states;
+  bool fused = true;  // one launch per CG iteration (VG_CG_FUSED=0: two)
+  // Persistent cooperative solve (one launch per half-step, no host sync).
+  bool coop = false;
+  unsigned coop_blocks = 0;
+  DevBuf bar_count, bar_gen;
   DevBuf cg, part, counter, newton;
   DevBuf H, ecg_out, ecg_frames;
   CgState *h_cg = nullptr;  // pinned
@@ -214,14 +235,60 @@ void launch_cg_pair(vg_handle *h) {
   }
 }
 
+template <class T>
+void launch_cg_init(vg_handle *h) {
+  ProfScope ps(h, VG_KC_CG_UPDATE);
+  hipLaunchKernelGGL(k_cg_spmv_dots<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
+                     h->r.as<T>(), h->w.as<T>(), h->n_slices, h->cg.as<CgState>(), red_of(h));
+}
+
+template <class T>
+void launch_cg_fused(vg_handle *h) {
+  ProfScope ps(h, VG_KC_CG_SPMV);
+  hipLaunchKernelGGL(k_cg_fused<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
+                     h->r.as<T>(), h->r2.as<T>(), h->w.as<T>(), h->w2.as<T>(), h->s.as<T>(),
+                     h->s2.as<T>(), h->p.as<T>(), h->x.as<T>(), h->diag.as<T>(), h->n_slices,
+                     h->cg.as<CgState>(), red_of(h));
+}
+
+// The whole half-step (right-hand side + CG to convergence) in one
+// cooperative launch; nothing comes back to the host.
+template <class T>
+int launch_cg_solve(vg_handle *h, const T *V) {
+  ProfScope ps(h, VG_KC_CG_SPMV);
+  const int64_t *off = h->off.as<int64_t>();
+  const int *width = h->width.as<int>(), *col = h->col.as<int>();
+  const T *Ahat = h->Ahat.as<T>(), *Krhs = h->Krhs.as<T>(), *diag = h->diag.as<T>();
+  T *rA = h->r.as<T>(), *rB = h->r2.as<T>(), *wA = h->w.as<T>(), *wB = h->w2.as<T>();
+  T *sA = h->s.as<T>(), *sB = h->s2.as<T>(), *p = h->p.as<T>(), *x = h->x.as<T>();
+  int64_t ns = h->n_slices;
+  CgState *cg = h->cg.as<CgState>();
+  Reduce red = red_of(h);
+  Barrier bar{h->bar_count.as<unsigned>(), h->bar_gen.as<unsigned>()};
+  double tol2 = h->tol * h->tol;
+  int maxit = h->maxit;
+  void *args[] = {&off, &width, &col, &Ahat, &Krhs, &V,   &rA,  &rB,   &wA,  &wB,    &sA,
+                  &sB,  &p,     &x,   &diag, &ns,   &cg,  &red, &bar, &tol2, &maxit};
+  HIPCHK(h, hipLaunchCooperativeKernel((const void *)k_cg_solve<T>, dim3(h->coop_blocks), dim3(BLK),
+                                       args, 0, h->stream));
+  return VG_OK;
+}
+
 // Runs CG to convergence: launches the expected number of iterations, then
-// reads the device state back and continues if needed.
+// reads the device state back and continues if needed.  Launches after
+// convergence return at once.
 template <class T>
 int cg_solve(vg_handle *h) {
   int launched = 0;
   int batch = h->cg_guess;
+  if (h->fused) launch_cg_init<T>(h);
   for (;;) {
-    for (int k = 0; k < batch; k++) launch_cg_pair<T>(h);
+    for (int k = 0; k < batch; k++) {
+      if (h->fused) launch_cg_fused<T>(h);
+      else launch_cg_pair<T>(h);
+    }
     launched += batch;
     HIPCHK(h, hipMemcpyAsync(h->h_cg, h->cg.p, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -241,7 +308,7 @@ int cg_solve(vg_handle *h) {
 
 template <class T, int MODEL, int INTEG>
 void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
-  IonicArgs a{h->n, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0};
+  IonicArgs a{h->n_myo, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0};
   hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
                      Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
@@ -253,10 +320,17 @@ void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
 template <class T>
 void launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   ProfScope ps(h, VG_KC_IONIC);
-  if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
-  else if (h->integ == VG_INTEG_REFERENCE)
-    launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
-  else launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN>(h, Vin, x, Vout);
+  if (h->n_myo > 0) {
+    if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+    else if (h->integ == VG_INTEG_REFERENCE)
+      launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+    else launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN>(h, Vin, x, Vout);
+  }
+  if (h->n_pk > 0)
+    hipLaunchKernelGGL(k_ionic_pk<T>, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, h->stream,
+                       Vin, x, h->sinv.as<T>(), Vout, h->pk_states.as<T>(),
+                       h->stim_active ? h->istim.as<T>() : (const T *)nullptr, h->n_myo, h->n_pk,
+                       h->n_pk_alloc, h->sub, h->cg.as<CgState>());
 }
 
 template <class T>
@@ -311,14 +385,22 @@ int step_t(vg_handle *h) {
   if (rc) return rc;
   const double t0 = h->t;
   // Step 1: half-step diffusion, C_m M + dT/4 K against C_m M - dT/4 K.
-  launch_rhs<T>(h, h->Va.as<T>());
-  if ((rc = cg_solve<T>(h))) return rc;
+  if (h->coop) {
+    if ((rc = launch_cg_solve<T>(h, h->Va.as<T>()))) return rc;
+  } else {
+    launch_rhs<T>(h, h->Va.as<T>());
+    if ((rc = cg_solve<T>(h))) return rc;
+  }
   // Step 2: ionic update over dT in the reference's sub-steps, fused with
   // V = V_a + D^-1/2 x.
   launch_ionic<T>(h, h->Va.as<T>(), h->x.as<T>(), h->Vb.as<T>());
   // Step 3: second half-step.
-  launch_rhs<T>(h, h->Vb.as<T>());
-  if ((rc = cg_solve<T>(h))) return rc;
+  if (h->coop) {
+    if ((rc = launch_cg_solve<T>(h, h->Vb.as<T>()))) return rc;
+  } else {
+    launch_rhs<T>(h, h->Vb.as<T>());
+    if ((rc = cg_solve<T>(h))) return rc;
+  }
   h->steps += 1;
   h->t = h->steps * h->dt;
   const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
@@ -362,32 +444,87 @@ int upload_raw(vg_handle *h, DevBuf &b, const std::vector<double> &v) {
   return VG_OK;
 }
 
-int set_states_impl(vg_handle *h, int nvar, const double *st) {
-  const int ns = nstate_of(h->model);
-  if (nvar != ns) return h->fail(VG_ERR_ARG, "state count mismatch");
-  // Device keeps every state except V (LR slot 7, tT slot 0).
-  const int vslot = h->model == VG_MODEL_LR ? 7 : 0;
-  const int64_t n = h->n, na = h->n_alloc;
-  std::vector<double> soa((size_t)(ns - 1) * na, 0.0);
-  for (int64_t r = 0; r < n; r++) {
+// The cell model of a node: Purkinje nodes always run the Purkinje model
+// (CardiacBody.cc:993-1002), all others the body's model.
+int node_model(const vg_handle *h, int64_t orig) {
+  return h->purkinje[orig] ? VG_MODEL_PURKINJE : h->model;
+}
+
+template <class T>
+int put_soa(vg_handle *h, DevBuf &b, const std::vector<double> &soa) {
+  if (soa.empty()) return VG_OK;
+  std::vector<T> f(soa.begin(), soa.end());
+  HIPCHK(h, hipMemcpy(b.p, f.data(), f.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VG_OK;
+}
+template <class T>
+int get_soa(vg_handle *h, const DevBuf &b, std::vector<double> &soa) {
+  if (soa.empty()) return VG_OK;
+  std::vector<T> f(soa.size());
+  HIPCHK(h, hipMemcpy(f.data(), b.p, f.size() * sizeof(T), hipMemcpyDeviceToHost));
+  std::copy(f.begin(), f.end(), soa.begin());
+  return VG_OK;
+}
+
+// Writes per-node state vectors (getInternalParameters layout, node order of
+// the mesh, offsets[i] = start of node i in data) into the two device SoA
+// arrays: the body's model for rows [0, n_myo), Purkinje for [n_myo, n).
+// The V slot of LR/tT is held by the voltage vector instead.
+int set_states_impl(vg_handle *h, const int64_t *offsets, const double *data) {
+  const int nd = ndev_of(h->model), vs = vslot_of(h->model);
+  const int64_t na = h->n_alloc, npa = h->n_pk_alloc;
+  std::vector<double> soa((size_t)nd * na, 0.0), pk((size_t)PK_NSTATE * npa, 0.0);
+  for (int64_t r = 0; r < h->n; r++) {
     const int64_t o = h->new2old[r];
-    int k2 = 0;
-    for (int k = 0; k < ns; k++) {
-      if (k == vslot) continue;
-      soa[(size_t)k2 * na + r] = st[o * ns + k];
-      k2++;
+    const double *st = data + offsets[o];
+    if (r < h->n_myo) {
+      int k2 = 0;
+      for (int k = 0; k < nstate_of(h->model); k++) {
+        if (k == vs) continue;
+        soa[(size_t)k2++ * na + r] = st[k];
+      }
+    } else {
+      for (int k = 0; k < PK_NSTATE; k++) pk[(size_t)k * npa + (r - h->n_myo)] = st[k];
     }
   }
-  // Padding lanes get the first node's states so every lane computes finite values.
-  for (int k = 0; k < ns - 1; k++)
-    for (int64_t r = n; r < na; r++) soa[(size_t)k * na + r] = soa[(size_t)k * na];
-  if (h->tsz == 8) {
-    HIPCHK(h, hipMemcpy(h->states.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> f(soa.begin(), soa.end());
-    HIPCHK(h, hipMemcpy(h->states.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  // Padding lanes copy a real node's states so every lane computes finite values.
+  if (h->n_myo > 0)
+    for (int k = 0; k < nd; k++)
+      for (int64_t r = h->n_myo; r < na; r++) soa[(size_t)k * na + r] = soa[(size_t)k * na];
+  if (h->n_pk > 0)
+    for (int k = 0; k < PK_NSTATE; k++)
+      for (int64_t r = h->n_pk; r < npa; r++) pk[(size_t)k * npa + r] = pk[(size_t)k * npa];
+  int rc = h->tsz == 8 ? put_soa<double>(h, h->states, soa) : put_soa<float>(h, h->states, soa);
+  if (rc) return rc;
+  if (h->n_pk > 0) rc = h->tsz == 8 ? put_soa<double>(h, h->pk_states, pk) : put_soa<float>(h, h->pk_states, pk);
+  return rc;
+}
+
+int get_states_impl(vg_handle *h, const int64_t *offsets, double *data, const double *V) {
+  const int nd = ndev_of(h->model), vs = vslot_of(h->model);
+  const int64_t na = h->n_alloc, npa = h->n_pk_alloc;
+  std::vector<double> soa((size_t)nd * na), pk((size_t)PK_NSTATE * (h->n_pk > 0 ? npa : 0));
+  int rc = h->tsz == 8 ? get_soa<double>(h, h->states, soa) : get_soa<float>(h, h->states, soa);
+  if (rc) return rc;
+  rc = h->tsz == 8 ? get_soa<double>(h, h->pk_states, pk) : get_soa<float>(h, h->pk_states, pk);
+  if (rc) return rc;
+  for (int64_t r = 0; r < h->n; r++) {
+    const int64_t o = h->new2old[r];
+    double *st = data + offsets[o];
+    if (r < h->n_myo) {
+      int k2 = 0;
+      for (int k = 0; k < nstate_of(h->model); k++) st[k] = (k == vs) ? V[o] : soa[(size_t)k2++ * na + r];
+    } else {
+      for (int k = 0; k < PK_NSTATE; k++) st[k] = pk[(size_t)k * npa + (r - h->n_myo)];
+    }
   }
   return VG_OK;
+}
+
+std::vector<int64_t> state_offsets(const vg_handle *h) {
+  std::vector<int64_t> off(h->n + 1, 0);
+  for (int64_t i = 0; i < h->n; i++) off[i + 1] = off[i] + nstate_of(node_model(h, i));
+  return off;
 }
 
 }  // namespace
@@ -452,8 +589,21 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, e);
   }
+  // Row order: myocardium first, then the Purkinje segment, each in mesh order.
+  std::vector<int32_t> base;
+  base.reserve(csr.n);
+  h->purkinje.assign(csr.n, 0);
+  if (mesh->tissue)
+    for (int64_t i = 0; i < csr.n; i++) h->purkinje[i] = (mesh->tissue[i] == 11 || mesh->tissue[i] == 12);
+  for (int64_t i = 0; i < csr.n; i++)
+    if (!h->purkinje[i]) base.push_back((int32_t)i);
+  h->n_myo = (int64_t)base.size();
+  for (int64_t i = 0; i < csr.n; i++)
+    if (h->purkinje[i]) base.push_back((int32_t)i);
+  h->n_pk = csr.n - h->n_myo;
+  h->n_pk_alloc = ((h->n_pk + BLK - 1) / BLK) * BLK;
   HostSell sell;
-  e = build_sell(csr, h->dt, h->Cm, model->sell_sigma, sell);
+  e = build_sell(csr, h->dt, h->Cm, model->sell_sigma, base, h->n_myo, sell);
   if (!e.empty()) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, e);
@@ -467,9 +617,6 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->old2new = sell.old2new;
   h->CL = csr.CL;
   h->xyz.assign(mesh->xyz, mesh->xyz + 3 * mesh->n_nodes);
-  h->purkinje.assign(h->n, 0);
-  if (mesh->tissue)
-    for (int64_t i = 0; i < h->n; i++) h->purkinje[i] = (mesh->tissue[i] == 11 || mesh->tissue[i] == 12);
   h->istim_host.assign(h->n, 0.0);
 
   int rc;
@@ -512,12 +659,38 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
   }
   // Vectors.
-  for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at}) {
+  // CG driver: the persistent cooperative solve when the device supports it
+  // and every block of it can be resident; VG_CG_MODE=fused|pair selects the
+  // host-driven launch-per-iteration variants (kept for measurement).
+  {
+    int coop_ok = 0, ncu = 0, per_cu = 0;
+    (void)hipDeviceGetAttribute(&coop_ok, hipDeviceAttributeCooperativeLaunch, H_->device);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, H_->device);
+    hipError_t oe = T == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                 &per_cu, (const void *)k_cg_solve<double>, BLK, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                 &per_cu, (const void *)k_cg_solve<float>, BLK, 0);
+    const int64_t want = (H_->n_slices + WAVES - 1) / WAVES;
+    const int64_t resident = (int64_t)ncu * per_cu;
+    const char *mode = getenv("VG_CG_MODE");
+    H_->coop = coop_ok && oe == hipSuccess && resident > 0 && mode && std::string(mode) == "coop";
+    H_->coop_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, resident));
+    if (mode && std::string(mode) == "pair") {
+      H_->coop = false;
+      H_->fused = false;
+    }
+  }
+  HIPCHK(H_, H_->bar_count.alloc(sizeof(unsigned)));
+  HIPCHK(H_, hipMemset(H_->bar_count.p, 0, sizeof(unsigned)));
+  HIPCHK(H_, H_->bar_gen.alloc(sizeof(unsigned)));
+  HIPCHK(H_, hipMemset(H_->bar_gen.p, 0, sizeof(unsigned)));
+  for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at,
+                    &H_->r2, &H_->w2, &H_->s2}) {
     HIPCHK(H_, b->alloc(na * T));
     HIPCHK(H_, hipMemset(b->p, 0, na * T));
   }
-  const int ns = nstate_of(H_->model);
-  HIPCHK(H_, H_->states.alloc((size_t)(ns - 1) * na * T));
+  HIPCHK(H_, H_->states.alloc((size_t)ndev_of(H_->model) * na * T));
+  if (H_->n_pk > 0) HIPCHK(H_, H_->pk_states.alloc((size_t)PK_NSTATE * H_->n_pk_alloc * T));
   // Reduction scratch: one FP64 partial per block and value.
   const int64_t maxblocks = std::max<int64_t>(na / BLK, (H_->n_slices + WAVES - 1) / WAVES) + 1;
   HIPCHK(H_, H_->part.alloc((size_t)maxblocks * MAX_ECG * 8));
@@ -553,13 +726,14 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
       HIPCHK(H_, hipMemcpy(H_->pidx.p, pi.data(), na * 2, hipMemcpyHostToDevice));
     }
   }
-  // Initial states and voltage: the model's own initial table, with V from it
-  // (LR: its rest state) unless the driver's V0 = 0 is requested.
+  // Initial states and voltage: each model's own initial table, with V from
+  // it (LR: its rest state) unless the driver's V0 = 0 is requested for the
+  // body's model; Purkinje nodes start at their own table's V
+  // (Purkinje.h:44-50, getVoltage :79-81).
   {
-    std::vector<double> st((size_t)H_->n * ns);
-    std::vector<double> V0(H_->n);
-    double v0;
+    const int ns = nstate_of(H_->model);
     std::vector<double> one(ns);
+    double v0;
     if (H_->model == VG_MODEL_LR) {
       if (H_->quirks & VG_QUIRK_V0_ZERO) {
         for (int k = 0; k < ns; k++) one[k] = LR_INIT[k];
@@ -573,11 +747,19 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
       for (int k = 0; k < ns; k++) one[k] = TT_INIT[k];
       v0 = (H_->quirks & VG_QUIRK_V0_ZERO) ? 0.0 : TT_INIT[0];
     }
+    const std::vector<int64_t> off = state_offsets(H_);
+    std::vector<double> st(off[H_->n]);
+    std::vector<double> V0(H_->n);
     for (int64_t i = 0; i < H_->n; i++) {
-      std::copy(one.begin(), one.end(), st.begin() + i * ns);
-      V0[i] = v0;
+      if (H_->purkinje[i]) {
+        std::copy(PK_INIT, PK_INIT + PK_NSTATE, st.begin() + off[i]);
+        V0[i] = PK_INIT[0];
+      } else {
+        std::copy(one.begin(), one.end(), st.begin() + off[i]);
+        V0[i] = v0;
+      }
     }
-    if ((rc = set_states_impl(H_, ns, st.data()))) return rc;
+    if ((rc = set_states_impl(H_, off.data(), st.data()))) return rc;
     if ((rc = upload(H_, H_->Va.p, V0.data(), v0))) return rc;
     if ((rc = upload(H_, H_->Vb.p, V0.data(), v0))) return rc;
   }
@@ -681,38 +863,42 @@ int vg_get_states(vg_handle *h, int *nvar, double *st) {
   if (!h || !nvar) return VG_ERR_ARG;
   const int ns = nstate_of(h->model);
   *nvar = ns;
+  if (h->n_pk > 0) return h->fail(VG_ERR_ARG, "mixed cell models: use vg_get_states_ragged");
   if (!st) return VG_OK;
-  const int vslot = h->model == VG_MODEL_LR ? 7 : 0;
-  const int64_t n = h->n, na = h->n_alloc;
-  std::vector<double> soa((size_t)(ns - 1) * na);
-  if (h->tsz == 8) {
-    HIPCHK(h, hipMemcpy(soa.data(), h->states.p, soa.size() * 8, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> f(soa.size());
-    HIPCHK(h, hipMemcpy(f.data(), h->states.p, f.size() * 4, hipMemcpyDeviceToHost));
-    std::copy(f.begin(), f.end(), soa.begin());
-  }
-  std::vector<double> V(n);
+  std::vector<int64_t> off(h->n + 1);
+  for (int64_t i = 0; i <= h->n; i++) off[i] = i * ns;
+  std::vector<double> V(h->n);
   int rc = download(h, h->Va.p, V.data());
   if (rc) return rc;
-  for (int64_t r = 0; r < n; r++) {
-    const int64_t o = h->new2old[r];
-    int k2 = 0;
-    for (int k = 0; k < ns; k++) {
-      if (k == vslot) {
-        st[o * ns + k] = V[o];
-        continue;
-      }
-      st[o * ns + k] = soa[(size_t)k2 * na + r];
-      k2++;
-    }
-  }
-  return VG_OK;
+  return get_states_impl(h, off.data(), st, V.data());
 }
 
 int vg_set_states(vg_handle *h, int nvar, const double *st) {
   if (!h || !st) return VG_ERR_ARG;
-  return set_states_impl(h, nvar, st);
+  if (nvar != nstate_of(h->model)) return h->fail(VG_ERR_ARG, "state count mismatch");
+  if (h->n_pk > 0) return h->fail(VG_ERR_ARG, "mixed cell models: use vg_set_states_ragged");
+  std::vector<int64_t> off(h->n + 1);
+  for (int64_t i = 0; i <= h->n; i++) off[i] = i * nvar;
+  return set_states_impl(h, off.data(), st);
+}
+
+int vg_get_states_ragged(vg_handle *h, int32_t *num_var, double *data) {
+  if (!h || !num_var) return VG_ERR_ARG;
+  const std::vector<int64_t> off = state_offsets(h);
+  for (int64_t i = 0; i < h->n; i++) num_var[i] = (int32_t)(off[i + 1] - off[i]);
+  if (!data) return VG_OK;
+  std::vector<double> V(h->n);
+  int rc = download(h, h->Va.p, V.data());
+  if (rc) return rc;
+  return get_states_impl(h, off.data(), data, V.data());
+}
+
+int vg_set_states_ragged(vg_handle *h, const int32_t *num_var, const double *data) {
+  if (!h || !num_var || !data) return VG_ERR_ARG;
+  const std::vector<int64_t> off = state_offsets(h);
+  for (int64_t i = 0; i < h->n; i++)
+    if (num_var[i] != off[i + 1] - off[i]) return h->fail(VG_ERR_ARG, "state count mismatch at node " + std::to_string(i));
+  return set_states_impl(h, off.data(), data);
 }
 
 int vg_get_lumped_capacitance(vg_handle *h, double *CL) {
@@ -821,15 +1007,18 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   unsigned long long nw = 0;
   HIPCHK(h, hipMemcpy(&nw, h->newton.p, sizeof(nw), hipMemcpyDeviceToHost));
+  CgState c;
+  HIPCHK(h, hipMemcpy(&c, h->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+  // Host-driven solves are counted on the host, persistent ones on the device.
   o->steps = h->steps;
-  o->cg_iterations = h->cg_iters;
-  o->cg_solves = h->cg_solves;
-  o->cg_not_converged = h->cg_notconv;
+  o->cg_iterations = h->cg_iters + (int64_t)c.iters_total;
+  o->cg_solves = h->cg_solves + (int64_t)c.solves_total;
+  o->cg_not_converged = h->cg_notconv + (int64_t)c.notconv_total;
   o->newton_iterations = (int64_t)nw;
   o->n_nodes = h->n;
   o->nnz = h->nnz;
   o->sell_padded_nnz = h->padded;
-  o->last_cg_relres = h->last_relres;
+  o->last_cg_relres = h->coop ? (c.rho0 > 0 ? std::sqrt(c.rho / c.rho0) : 0.0) : h->last_relres;
   o->sim_time = h->t;
   return VG_OK;
 }
@@ -837,7 +1026,8 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
 // ---- independent cells -------------------------------------------------------
 int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   if (!out || n < 1) return VG_ERR_ARG;
-  if (model != VG_MODEL_LR && model != VG_MODEL_TT) return VG_ERR_UNSUPPORTED;
+  if (model != VG_MODEL_LR && model != VG_MODEL_TT && model != VG_MODEL_PURKINJE)
+    return VG_ERR_UNSUPPORTED;
   if (prec != VG_PREC_FP64 && prec != VG_PREC_MIXED) return VG_ERR_ARG;
   std::unique_ptr<vg_cells> c(new vg_cells());
   c->prec = prec;
@@ -847,7 +1037,7 @@ int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   c->n_alloc = ((n + BLK - 1) / BLK) * BLK;
   const size_t T = prec == VG_PREC_FP64 ? 8 : 4;
   const int ns = nstate_of(model);
-  if (c->states.alloc((size_t)(ns - 1) * c->n_alloc * T) != hipSuccess) return VG_ERR_HIP;
+  if (c->states.alloc((size_t)ndev_of(model) * c->n_alloc * T) != hipSuccess) return VG_ERR_HIP;
   if (c->V.alloc(n * 8) != hipSuccess || c->stim.alloc(n * 8) != hipSuccess ||
       c->dvdt.alloc(n * 8) != hipSuccess)
     return VG_ERR_HIP;
@@ -863,7 +1053,7 @@ int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   // Initial states: the model's initial table (LR: reference table).
   std::vector<double> st((size_t)n * ns);
   for (int64_t i = 0; i < n; i++)
-    for (int k = 0; k < ns; k++) st[i * ns + k] = model == VG_MODEL_LR ? LR_INIT[k] : TT_INIT[k];
+    for (int k = 0; k < ns; k++) st[i * ns + k] = init_table(model)[k];
   *out = c.release();
   return vg_cells_set_states(*out, ns, st.data());
 }
@@ -917,10 +1107,12 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, con
                      c->n_alloc, Cm, dt, 500, c->newton_jt)
   if (c->prec == VG_PREC_FP64) {
     if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(double, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(double, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(double, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
   } else {
     if (c->model == VG_MODEL_LR) VG_CELLS(float, VG_MODEL_LR, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(float, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(float, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(float, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
   }
@@ -935,9 +1127,9 @@ int vg_cells_get_states(vg_cells *c, int *nvar, double *st) {
   const int ns = nstate_of(c->model);
   *nvar = ns;
   if (!st) return VG_OK;
-  const int vslot = c->model == VG_MODEL_LR ? 7 : 0;
+  const int vslot = vslot_of(c->model);
   const int64_t n = c->n, na = c->n_alloc;
-  std::vector<double> soa((size_t)(ns - 1) * na);
+  std::vector<double> soa((size_t)ndev_of(c->model) * na);
   if (c->prec == VG_PREC_FP64) {
     if (hipMemcpy(soa.data(), c->states.p, soa.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
   } else {
@@ -965,22 +1157,20 @@ int vg_cells_set_states(vg_cells *c, int nvar, const double *st) {
   if (!c || !st) return VG_ERR_ARG;
   const int ns = nstate_of(c->model);
   if (nvar != ns) return VG_ERR_ARG;
-  const int vslot = c->model == VG_MODEL_LR ? 7 : 0;
+  const int vslot = vslot_of(c->model), nd = ndev_of(c->model);
   const int64_t n = c->n, na = c->n_alloc;
-  std::vector<double> soa((size_t)(ns - 1) * na, 0.0);
+  std::vector<double> soa((size_t)nd * na, 0.0);
   std::vector<double> V(n);
   for (int64_t i = 0; i < n; i++) {
     int k2 = 0;
+    V[i] = st[i * ns + (vslot >= 0 ? vslot : 0)];
     for (int k = 0; k < ns; k++) {
-      if (k == vslot) {
-        V[i] = st[i * ns + k];
-        continue;
-      }
+      if (k == vslot) continue;
       soa[(size_t)k2 * na + i] = st[i * ns + k];
       k2++;
     }
   }
-  for (int k = 0; k < ns - 1; k++)
+  for (int k = 0; k < nd; k++)
     for (int64_t i = n; i < na; i++) soa[(size_t)k * na + i] = soa[(size_t)k * na];
   if (c->prec == VG_PREC_FP64) {
     if (hipMemcpy(c->states.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;

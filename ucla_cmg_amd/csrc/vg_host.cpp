@@ -237,7 +237,8 @@ std::string assemble_csr(const vg_mesh &m, const vg_model &mod, HostCsr &out) {
   return "";
 }
 
-std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma, HostSell &o) {
+std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
+                       const std::vector<int32_t> &base, int64_t n_first, HostSell &o) {
   const int64_t n = c.n;
   o.n = n;
   o.n_slices = (n + SELL_C - 1) / SELL_C;
@@ -248,15 +249,17 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma, HostSe
       if (c.col[k] == i) d[i] = dt / 4.0 * c.K[k] + Cm * c.M[k];
     if (!(d[i] > 0.0)) return "non-positive diagonal of the CN operator at node " + std::to_string(i);
   }
-  o.new2old.resize(n);
-  std::iota(o.new2old.begin(), o.new2old.end(), 0);
+  if ((int64_t)base.size() != n || n_first < 0 || n_first > n) return "bad row order";
+  o.new2old = base;
   if (sigma > 1) {
-    for (int64_t w0 = 0; w0 < n; w0 += sigma) {
-      const int64_t w1 = std::min<int64_t>(n, w0 + sigma);
-      std::stable_sort(o.new2old.begin() + w0, o.new2old.begin() + w1, [&](int32_t a, int32_t b) {
-        return (c.row_ptr[a + 1] - c.row_ptr[a]) > (c.row_ptr[b + 1] - c.row_ptr[b]);
-      });
-    }
+    const int64_t seg[3] = {0, n_first, n};
+    for (int sgi = 0; sgi < 2; sgi++)
+      for (int64_t w0 = seg[sgi]; w0 < seg[sgi + 1]; w0 += sigma) {
+        const int64_t w1 = std::min<int64_t>(seg[sgi + 1], w0 + sigma);
+        std::stable_sort(o.new2old.begin() + w0, o.new2old.begin() + w1, [&](int32_t a, int32_t b) {
+          return (c.row_ptr[a + 1] - c.row_ptr[a]) > (c.row_ptr[b + 1] - c.row_ptr[b]);
+        });
+      }
   }
   o.old2new.assign(n, 0);
   for (int64_t i = 0; i < n; i++) o.old2new[o.new2old[i]] = (int32_t)i;

@@ -36,9 +36,12 @@ struct HostSell {
   std::vector<int32_t> new2old, old2new;
 };
 
-// Builds the scaled CN operators in SELL-64 form.  Rows are reordered by
-// descending length inside windows of `sigma` rows when sigma > 1.
-std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma, HostSell &out);
+// Builds the scaled CN operators in SELL-64 form.  `base` lists the mesh
+// nodes in the solver's row order (rows [0, n_first) and [n_first, n) form two
+// segments); rows are reordered by descending length inside windows of
+// `sigma` rows of one segment when sigma > 1.
+std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
+                       const std::vector<int32_t> &base, int64_t n_first, HostSell &out);
 
 // Lead-field matrix of the pseudo-ECG: ecg_e = sum_n H[e][n] V_n with
 // H[e][n] = sum over quadrature points of w (dN_n/dx)^T D grad(1/r_e).

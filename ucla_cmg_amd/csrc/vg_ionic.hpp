@@ -421,4 +421,155 @@ __device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
   return R7[0];
 }
 
+// ======================= Corrias rabbit Purkinje ==============================
+// State order of Purkinje.cc (22 states, V = 0); constants of
+// SetUpPurkinjeParameters (SetUpIonicConstants.h:16-85), the last six derived.
+constexpr int PK_NSTATE = 22;
+constexpr double PK_INIT[PK_NSTATE] = {-88.34,   0.00001, 0.000032, 0.17,     6.7,      140,
+                                       0.001337, 0.01,    0.000003, 0.1,      0.7,      0.0,
+                                       0.7,      0.000007, 0.978861, 0.000012, 0.864489, 0.25,
+                                       1.0,      0.0,     0.011099, 0.011099};
+constexpr double PKL[57] = {
+    310.0, 96485.3415, 8314.472, 2.0,    140,     5.4,    69.0,     13266.5, 0.6,
+    0.2,   0.06,       700,      9000000, 500,    0.5,    -4320.0,  1.0,     0.9,
+    0.7,   5.4,        5.0,      350.0,  10.0,    3.0,    0.005,    2.0,     1140.0,
+    15.0,  2.0,        20.0,     1.5,    3.0,     3,      0.001,    0.001,   0.5,
+    442.2, 0.01,       0.01,     0.0001, 5.0,     0.0001, 1.5,      0.188709677,
+    0.045290323, 2500.0, 0.001, 120.0,  0.000246, 1.7,    1.6,      120.0,   0.000246,
+    1.7,   1.6,        10.0,     5000.0};
+constexpr double PK_RTF = (PKL[2] * PKL[0]) / PKL[1];            // Constants[58]
+constexpr double PK_VOL_I = PKL[9] * PKL[7];                      // Constants[59]
+constexpr double PK_VOL_SR = PKL[8] * PKL[7];                     // Constants[60]
+constexpr double PK_VOL_MYO = (PKL[8] + PKL[9]) * PKL[7];         // Constants[61]
+constexpr double PK_VOL_SUB = PKL[10] * PKL[7];                   // Constants[62]
+constexpr double PK_XI = 2500.;        // Purkinje.h:53
+constexpr double PK_AREA = 4.612E-5;   // Purkinje.h:55, cm^2
+
+// All 22 rates of Purkinje::ComputeRates (Purkinje.cc:5-104).  stim is the
+// stimulus current in pA (the reference's _Algebraic[0]).
+template <class T, bool FAST>
+__device__ inline void pk_rates(const T S[PK_NSTATE], T stim, T R[PK_NSTATE]) {
+  using M = Mth<T, FAST>;
+  const T V = S[0];
+  auto sig = [](T x) { return T(1.0) / (T(1.0) + M::ex(x)); };
+  // Gates with constant time constants.
+  R[6] = (sig((V + T(47.8)) / T(-5.5)) - S[6]) / T(PKL[16]);
+  R[8] = (sig((V + T(14.6)) / T(-5.5)) - S[8]) / T(PKL[18]);
+  R[11] = (sig((V - T(7.0)) / T(-9.0)) - S[11]) / T(PKL[20]);
+  R[12] = (sig((V + T(27.5)) / T(8.0)) - S[12]) / T(PKL[21]);
+  R[13] = (sig((V + T(25.0)) / T(-5.0)) - S[13]) / T(PKL[24]);
+  R[14] = (sig((V + T(69.0)) / T(3.96)) - S[14]) / T(PKL[25]);
+  R[15] = (sig((V + T(30.0)) / T(-5.0)) - S[15]) / T(PKL[27]);
+  // Gates with voltage- or calcium-dependent time constants.
+  R[7] = (sig((V + T(67.9)) / T(3.87)) - S[7]) / (T(1.42271) * M::ex(T(-0.05119) * V));
+  {
+    const T q = T(0.028) * (V + T(14.5));
+    R[9] = (sig((V + T(31.0)) / T(5.54)) - S[9]) / (T(25.1) / (T(0.04) + T(0.7) * M::ex(-(q * q))));
+  }
+  {
+    const T c = S[1] / T(0.0001);
+    const T den = T(1.0) + c * c;
+    R[10] = ((T(0.4) + T(0.6) / den) - S[10]) / (T(2.0) + T(80.0) / den);
+  }
+  R[16] = ((T(0.1) + T(0.9) / (T(1.0) + M::ex((V + T(75.6)) / T(6.3)))) - S[16]) /
+          (T(120.0) + M::ex((V + T(100.0)) / T(25.0)));
+  const T xs_inf = sig((V - T(1.5)) / T(-16.7));
+  T tau_xs;
+  {
+    const T u = V + T(30.0);
+    tau_xs = (fabs(u) < T(0.0145))
+                 ? T(417.946)
+                 : T(1.0) / ((T(7.19e-05) * u) / (T(1.0) - M::ex(T(-0.148) * u)) +
+                             (T(0.000131) * u) / (M::ex(T(0.0687) * u) - T(1.0)));
+  }
+  R[18] = (xs_inf - S[18]) / tau_xs;
+  R[19] = (xs_inf - S[19]) / (T(4.0) * tau_xs);
+  {
+    const T inf = sig((V + T(109.0)) / T(10.0));
+    const T tau = T(6000.0) / (M::ex(-(T(2.9) + T(0.04) * V)) + M::ex(T(3.6) + T(0.11) * V));
+    R[20] = (inf - S[20]) / tau;
+    R[21] = (inf - S[21]) / tau;
+  }
+  {
+    const T u7 = V + T(7.0), u10 = V + T(10.0);
+    const T a = (fabs(u7) > T(0.001)) ? (T(0.00138) * u7) / (T(1.0) - M::ex(T(-0.123) * u7))
+                                     : T(0.00138) / T(0.123);
+    const T b = (fabs(u10) > T(0.001)) ? (T(6.1e-05) * u10) / (M::ex(T(0.145) * u10) - T(1.0))
+                                      : T(0.00061) / T(0.145);
+    R[17] = (sig((V + T(50.0)) / T(-7.5)) - S[17]) * (a + b);
+  }
+  // Reversal potentials (mV).
+  const T rtf = T(PK_RTF);
+  const T Ko = T(PKL[5]), Nao = T(PKL[4]), Cao = T(PKL[3]);
+  const T EK = rtf * M::lg(Ko / S[5]);
+  const T ENa = rtf * M::lg(Nao / S[4]);
+  const T ECa = T(0.5) * rtf * M::lg(Cao / S[1]);
+  // Currents (pA).
+  const T iK1 = T(PKL[29]) * M::pw(Ko / T(5.4), T(0.8)) * sig((T(92.0) + V) / T(10.0)) * (V - EK);
+  const T ito = T(PKL[22]) * S[11] * S[12] * (V - EK);
+  const T isus = T(PKL[23]) * sig((T(5.0) - V) / T(17.0)) * (V - EK);
+  const T iKr = T(PKL[30]) * (Ko / T(5.4)) * sig((T(33.0) + V) / T(22.4)) * S[17] * (V - EK);
+  const T iKs = T(PKL[31]) * S[18] * S[19] * (V - EK);
+  const T ibK = T(PKL[37]) * (V - EK);
+  const T iNaK = T(PKL[36]) * sig((V + T(80.0)) / T(-45.0)) * sig(V / T(125.0)) *
+                 (T(1.0) / (T(1.0) + M::pw(T(1.9) / Ko, T(1.45)))) *
+                 (T(1.0) / (T(1.0) + T(31.98) / S[4]));
+  const T ifK = T(PKL[43]) * S[20] * (V - EK);
+  const T iCaL = T(PKL[19]) * S[8] * S[9] * S[10] * (V - ECa);
+  const T iCaT = T(PKL[17]) * S[6] * S[7] * (V - ECa);
+  const T iNa = T(PKL[26]) * S[13] * S[14] * (V - ENa);
+  const T iNaL = T(PKL[28]) * S[15] * S[16] * (V - ENa);
+  const T ipCa = T(PKL[40]) * (T(1.0) / (T(1.0) + M::pw(T(PKL[41]) / S[1], T(PKL[42]))));
+  const T ibCa = T(PKL[39]) * (V - ECa);
+  const T ifNa = T(PKL[44]) * S[21] * (V - ENa);
+  const T nH = T(PKL[32]), gam = T(PKL[35]);
+  const T Nao_n = M::pw(Nao, nH), Nai_n = M::pw(S[4], nH);
+  const T iNaCa = (T(512.009) * T(PKL[33]) *
+                   (M::ex((gam * (nH - T(2.0)) * V) / rtf) * Nai_n * Cao -
+                    S[1] * M::ex(((gam - T(1.0)) * (nH - T(2.0)) * V) / rtf) * Nao_n)) /
+                  ((T(1.0) + T(PKL[34]) * (S[1] * Nao_n + Cao * Nai_n)) * (T(1.0) + S[1] / T(0.0069)));
+  const T ibNa = T(PKL[38]) * (V - ENa);
+  R[0] = (T(-1.0) / T(PKL[6])) * (iK1 + ito + isus + iKr + iKs + ibK + iNaK + iCaL + iCaT + iNa +
+                                  iNaL + ipCa + ibCa + ifNa + ifK + iNaCa + ibNa + stim);
+  const T F = T(PKL[1]);
+  R[5] = ((-ito - isus - iKr - iKs - iK1 - ibK - ifK - stim + T(2.0) * iNaK) * T(1000.0)) /
+         (F * T(PK_VOL_MYO));
+  R[4] = ((-iNa - iNaL - T(3.0) * iNaK - T(3.0) * iNaCa - ifNa - ibNa) * T(1000.0)) / (F * T(PK_VOL_MYO));
+  const T jrel = T(PKL[45]) * (S[3] / (T(1.0) + (T(PKL[46]) / S[1]) * (T(PKL[46]) / S[1])));
+  const T nu = T(PKL[50]), nsr = T(PKL[54]);
+  const T ui = M::pw(S[1] / T(PKL[48]), nu), us = M::pw(S[3] / T(PKL[49]), nu);
+  const T jup = T(PKL[47]) * ((ui - us) / (T(1.0) + ui + us));
+  const T vi = M::pw(S[2] / T(PKL[52]), nsr), vs = M::pw(S[3] / T(PKL[53]), nsr);
+  const T jup2 = T(PKL[51]) * ((vi - vs) / (T(1.0) + vi + vs));
+  const T jtr = T(PKL[55]) * (S[3] - S[2]);
+  R[3] = ((-jrel - jtr + jup2 + jup) * T(1000.0)) / (T(2.0) * F * T(PK_VOL_SUB));
+  const T jdiff = T(PKL[56]) * (S[1] - S[2]);
+  R[1] = ((-iCaL - iCaT - ibCa + jrel - jup - jdiff + T(2.0) * iNaCa - ipCa) * T(1000.0)) /
+         (T(2.0) * F * T(PK_VOL_I));
+  R[2] = ((-jup2 + jdiff + jtr) * T(1000.0)) / (T(2.0) * F * T(PK_VOL_SR));
+}
+
+// Purkinje::Compute_Ion + UpdateStateVariables (Purkinje.cc:179-210):
+// forward Euler on all 22 states, V included, in 10 sub-steps of dt/10 when
+// |dV/dt| > 1.5 mV/ms; returns the last dV/dt evaluated.
+template <class T, bool FAST>
+__device__ inline T pk_step(T S[PK_NSTATE], T V, T istim, T dt) {
+  S[0] = V;
+  const T stim = T(-1E6) * istim * T(PK_AREA) / T(PK_XI);
+  T R[PK_NSTATE];
+  pk_rates<T, FAST>(S, stim, R);
+  if (fabs(R[0]) > T(1.5)) {
+    const T h = dt / T(10);
+    for (int k = 0; k < 10; k++) {
+      pk_rates<T, FAST>(S, stim, R);
+#pragma unroll
+      for (int j = 0; j < PK_NSTATE; j++) S[j] += R[j] * h;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PK_NSTATE; j++) S[j] += R[j] * dt;
+  }
+  return R[0];
+}
+
 }  // namespace vg

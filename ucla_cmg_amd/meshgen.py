@@ -149,7 +149,7 @@ def lv_shell(n_trans=26, n_circ=240, n_long=165, base_radius=2.5, apex_depth=7.0
     epicardium, measured from the circumferential direction toward the
     longitudinal direction in the wall's tangent plane.
 
-    Defaults give 1,029,600 nodes and 5,771,520 tets.
+    Defaults give 1,029,600 nodes and 5,904,000 tets.
     ``meta`` carries per-node transmural depth (0 endo .. 1 epi) and
     normalised apicobasal height (0 apex .. 1 base).
     """
@@ -216,3 +216,123 @@ def electrodes_around(center, radius, n, axis_z=0.0) -> np.ndarray:
     c = np.asarray(center, dtype=np.float64)
     return np.stack([c[0] + radius * np.cos(ang), c[1] + radius * np.sin(ang),
                      np.full(n, c[2] + axis_z)], axis=1)
+
+
+def pmj_length(resistance_ohm_m: float) -> float:
+    """Effective PMJ bar length (cm) standing for a junction resistance.
+
+    The reference describes a junction only by the bar's direction and length
+    (``.angle`` file, Mesh/MixedMesh3D.cc:85-88), which set its conductance
+    2 D_p pi r^2 / L.  This build maps a resistance linearly onto that length,
+    0.4 mm per 1 Ohm m (500 Ohm m -> 0.2 cm), a choice calibrated so a single
+    junction still excites the tissue it feeds.
+    """
+    return 4e-4 * float(resistance_ohm_m)
+
+
+def purkinje_tree(mesh: Mesh, generations=8, seg_len=0.06, seed=1, offset=0.02,
+                  pmj_length=0.1, theta_apex_deg=6.0, base_frac=0.92) -> Mesh:
+    """Adds a seeded binary-branching Purkinje network to an ``lv_shell`` mesh.
+
+    The tree runs on a surface ``offset`` cm inside the endocardium: a root
+    near the base splits ``generations`` times (2**generations terminals),
+    each branch heading apex-ward and spreading in circumferential angle with
+    seeded jitter.  Branches are cut into 2-node bars of about ``seg_len`` cm.
+    Each terminal gets one PMJ bar to its nearest endocardial node, keyed in
+    the PMJ table by the terminal (the element's first node) with the
+    direction cosines and the effective length ``pmj_length`` that sets the
+    junction's conductance 2 D_p pi r^2 / L (Mesh/MixedMesh3D.cc:85-107).
+    Purkinje nodes get tissue type 11 (left ventricle).
+
+    Returns a new Mesh: the myocardium first (unchanged ids), then the
+    Purkinje nodes; ``meta['purkinje_root']`` is the root node id and
+    ``meta['pmj_terminals']`` the terminal ids.
+    """
+    if mesh.meta.get("kind") != "lv_shell":
+        raise ValueError("purkinje_tree needs an lv_shell mesh")
+    rng = np.random.default_rng(seed)
+    nt, nc, nl = mesh.meta["shape"]
+    depth = mesh.meta["depth"]
+    endo = np.nonzero(depth == 0.0)[0]
+    xyz_endo = mesh.xyz[endo]
+    # Endocardial ellipsoid axes from the generator's geometry.
+    a_in = np.max(np.hypot(xyz_endo[:, 0], xyz_endo[:, 1]))
+    c_in = -xyz_endo[:, 2].min() / np.cos(np.deg2rad(theta_apex_deg))
+    a_p, c_p = a_in - offset, c_in - offset
+    th_lo = np.deg2rad(theta_apex_deg) * 1.5
+    th_hi = np.pi / 2 * base_frac
+
+    def point(th, ph):
+        return np.array([a_p * np.sin(th) * np.cos(ph), a_p * np.sin(th) * np.sin(ph),
+                         -c_p * np.cos(th)])
+
+    nodes = []      # (theta, phi)
+    bars = []       # (i, j)
+    terminals = []
+
+    def add_node(th, ph):
+        nodes.append((th, ph))
+        return len(nodes) - 1
+
+    def run(i0, th1, ph1):
+        th0, ph0 = nodes[i0]
+        L = np.linalg.norm(point(th1, ph1) - point(th0, ph0))
+        k = max(1, int(round(L / seg_len)))
+        prev = i0
+        for s in range(1, k + 1):
+            f = s / k
+            cur = add_node(th0 + f * (th1 - th0), ph0 + f * (ph1 - ph0))
+            bars.append((prev, cur))
+            prev = cur
+        return prev
+
+    root = add_node(th_hi, 0.0)
+    frontier = [(root, -np.pi, np.pi)]
+    dth = (th_hi - th_lo) / (generations + 1)
+    for g in range(generations):
+        nxt = []
+        for (i, pa, pb) in frontier:
+            th, ph = nodes[i]
+            mid = 0.5 * (pa + pb)
+            for (ca, cb) in ((pa, mid), (mid, pb)):
+                tgt_ph = 0.5 * (ca + cb) + rng.uniform(-0.15, 0.15) * (cb - ca)
+                tgt_th = max(th_lo, th - dth * rng.uniform(0.7, 1.3))
+                end = run(i, tgt_th, tgt_ph)
+                nxt.append((end, ca, cb))
+        frontier = nxt
+    # Terminal twigs towards the apex band so activation reaches all heights.
+    for (i, pa, pb) in frontier:
+        th, ph = nodes[i]
+        tgt_th = th_lo + rng.uniform(0.0, 1.0) * (th - th_lo)
+        terminals.append(run(i, tgt_th, ph + rng.uniform(-0.3, 0.3) * (pb - pa)))
+
+    n_myo = mesh.n_nodes
+    pk_xyz = np.array([point(th, ph) for th, ph in nodes])
+    n_pk = len(nodes)
+    # PMJ bars: terminal -> nearest endocardial node.
+    term = np.array(terminals)
+    d2 = ((pk_xyz[term][:, None, :] - xyz_endo[None, :, :]) ** 2).sum(-1)
+    near = endo[np.argmin(d2, axis=1)]
+    pmj_bars = [(n_myo + t, m) for t, m in zip(term, near)]
+    cable = [(n_myo + i, n_myo + j) for i, j in bars]
+    all_bars = np.array(cable + pmj_bars, dtype=np.int32)
+    nb = all_bars.shape[0]
+    conn_b = np.full((nb, 4), -1, np.int32)
+    conn_b[:, :2] = all_bars
+    dirs = mesh.xyz[near] - pk_xyz[term]
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    pmj_data = np.hstack([dirs, np.full((len(term), 1), pmj_length)])
+    xyz = np.vstack([mesh.xyz, pk_xyz])
+    tissue = np.concatenate([np.zeros(n_myo, np.int8), np.full(n_pk, 11, np.int8)])
+    fib_b = np.tile(np.eye(3).ravel(), (nb, 1))
+    fibers = None if mesh.fibers is None else np.vstack([mesh.fibers, fib_b])
+    meta = dict(mesh.meta)
+    meta["depth"] = np.concatenate([depth, np.full(n_pk, -1.0)])
+    meta["height"] = np.concatenate([mesh.meta["height"], np.full(n_pk, -1.0)])
+    meta["purkinje_root"] = n_myo + root
+    meta["pmj_terminals"] = n_myo + term
+    meta["n_myocardium"] = n_myo
+    return Mesh(xyz=xyz, elem_nn=np.concatenate([mesh.elem_nn, np.full(nb, 2, np.int8)]),
+                conn=np.vstack([mesh.conn, conn_b]), fibers=fibers, tissue=tissue,
+                pmj_node=(n_myo + term).astype(np.int32), pmj_data=pmj_data,
+                radius=mesh.radius, meta=meta)

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libvoomgpu.so")
 VG_OK = 0
 MODEL_LR = 0
 MODEL_TT = 2
+MODEL_PURKINJE = 10
 PREC_FP64 = 0
 PREC_MIXED = 1
 INTEG_REFERENCE = 0
@@ -43,6 +44,7 @@ EXPORTED = [
     "vg_set_stimulus_nodes", "vg_set_stimulus_sphere", "vg_set_stimulus_axis",
     "vg_set_stimulus_plane", "vg_clear_stimulus", "vg_step", "vg_synchronize",
     "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
+    "vg_get_states_ragged", "vg_set_states_ragged",
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
     "vg_ecg_setup", "vg_ecg", "vg_ecg_record", "vg_ecg_frames", "vg_profile_enable",
     "vg_profile_get", "vg_stats_get", "vg_assemble_csr", "vg_cells_create",
@@ -110,6 +112,8 @@ def load_library(path: str = LIB_PATH):
         "vg_set_voltage": ([H, C.c_void_p], C.c_int),
         "vg_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
         "vg_set_states": ([H, C.c_int, C.c_void_p], C.c_int),
+        "vg_get_states_ragged": ([H, C.c_void_p, C.c_void_p], C.c_int),
+        "vg_set_states_ragged": ([H, C.c_void_p, C.c_void_p], C.c_int),
         "vg_get_lumped_capacitance": ([H, C.c_void_p], C.c_int),
         "vg_activation_enable": ([H, C.c_double], C.c_int),
         "vg_get_activation": ([H, C.c_void_p], C.c_int),
@@ -316,6 +320,21 @@ class CardiacSolver:
     def setStateVariables(self, st):
         st = np.ascontiguousarray(st, np.float64)
         self._chk(self.L.vg_set_states(self.h, st.shape[1], _p(st)), "set_states")
+
+    def getStateVariablesRagged(self):
+        """(numVar[n], list of per-node state vectors) like
+        CardiacBody::getStateVariables (CardiacBody.cc:438-451)."""
+        nv = np.zeros(self.n, np.int32)
+        self._chk(self.L.vg_get_states_ragged(self.h, _p(nv), None), "get_states_ragged")
+        data = np.zeros(int(nv.sum()))
+        self._chk(self.L.vg_get_states_ragged(self.h, _p(nv), _p(data)), "get_states_ragged")
+        off = np.concatenate([[0], np.cumsum(nv)])
+        return nv, [data[off[i]:off[i + 1]] for i in range(self.n)]
+
+    def setStateVariablesRagged(self, num_var, states):
+        nv = np.ascontiguousarray(num_var, np.int32)
+        data = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in states]))
+        self._chk(self.L.vg_set_states_ragged(self.h, _p(nv), _p(data)), "set_states_ragged")
 
     def lumpedCapacitance(self) -> np.ndarray:
         c = np.zeros(self.n)
