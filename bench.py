@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="mesh size factor (1.0 = 1.03M nodes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cg-blocks", type=int, default=0,
+                    help="workgroups per CG launch (0 = library default)")
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",
                     help="Purkinje tree with high/low PMJ resistance, or none")
     args = ap.parse_args()
@@ -147,7 +149,7 @@ def main():
         stim, amp = his_stimulus(mesh), 2.0 * 50.0 * 2500.0
     solver = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED,
                               integrator=vg.INTEG_RUSH_LARSEN, param_scales=scales,
-                              param_index=pidx, **MODEL_KW)
+                              param_index=pidx, cg_blocks=args.cg_blocks, **MODEL_KW)
     solver.enableActivation(-40.0)
     solver.setupECG(electrodes10())
     total_steps = args.warmup + args.steps
@@ -187,26 +189,31 @@ def main():
     value = n * args.steps / wall
 
     # Per-kernel times: HIP events around every launch on the solver's stream,
-    # over a separate window after the timed one.
+    # over a window of the same run right after the timed one (the events
+    # cost a little time of their own, so the timed window runs without).
     prof_steps = min(50, args.steps)
     solver.profile(True)
     stp0 = solver.stats()
+    tp0 = time.perf_counter()
     solver.Solve(prof_steps)
     solver.synchronize()
+    prof_wall_ms = (time.perf_counter() - tp0) * 1e3
     prof = {name: solver.profileGet(kc) for kc, name in enumerate(vg.KC_NAMES)}
     prof_its = solver.stats()["cg_iterations"] - stp0["cg_iterations"]
     solver.profile(False)
 
-    # Dominant kernel: the CG SpMV (+ fused dots).  Algorithmic bytes per
-    # launch: every stored nonzero once (FP32 value + int32 column), the
-    # source vector once and the result once (4 B each per row).
-    spmv_ms, spmv_launches = prof["cg_spmv"]
+    # Dominant kernel: the fused CG iteration k_cg_iter<float>.  Algorithmic
+    # bytes per launch: every stored nonzero once (FP32 value + int32
+    # column) and per row the vectors it must read (r, w, s, p, x, diag) and
+    # write (r, w, s, p, x), 4 B each; neighbours' r, w, s are the same
+    # vectors, counted once.
+    it_ms, it_launches = prof["cg_iter"]
     nnz = st1["nnz"]
-    spmv_bytes = 8 * nnz + 8 * n
+    it_bytes = 8 * nnz + 44 * n
     # Launches after convergence exit at once; their few microseconds are
     # charged to the launches that did an iteration (conservative).
-    spmv_avg_s = (spmv_ms / 1e3) / max(1, prof_its) if spmv_launches else float("nan")
-    achieved = spmv_bytes / spmv_avg_s / 1e9
+    it_avg_s = (it_ms / 1e3) / max(1, prof_its) if it_launches else float("nan")
+    achieved = it_bytes / it_avg_s / 1e9
     ionic_ms, ionic_launches = prof["ionic"]
     ionic_bytes = n * (18 * 4 * 2 + 4 + 4 + 4 + 4 + 2)
     kern_tot = sum(v[0] for v in prof.values())
@@ -228,24 +235,28 @@ def main():
                         f"{n} nodes, {mesh.n_elems} elements, plain path (Rush-Larsen ionic), "
                         "dt 0.02 ms, CG tol 1e-7, AT + 10-lead ECG/ms",
             "nodes": n, "tets": mesh.n_elems, "nnz": nnz, "cg_iters_per_step": k_per_step,
+            "cg_blocks": args.cg_blocks,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_spmv_dots<float>",
+            "kernel": "k_cg_iter<float>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_launch": spmv_bytes,
-            "avg_launch_us": spmv_avg_s * 1e6,
+            "bytes_per_launch": it_bytes,
+            "us_per_iteration": it_avg_s * 1e6,
+            "avg_launch_us": (it_ms * 1e3) / max(1, it_launches),
+            "launches_per_iteration": it_launches / max(1, prof_its),
         },
         "kernels_ms": {k: v[0] for k, v in prof.items()},
         "kernel_launches": {k: v[1] for k, v in prof.items()},
         "ionic_GBps": ionic_bytes / ((ionic_ms / 1e3) / max(1, ionic_launches)) / 1e9 if ionic_launches else None,
         "event_ms_timed_region": ev_ms,
-        "kernel_busy_frac": kern_tot / (wall * 1e3),
+        "profiled_ms_per_step": prof_wall_ms / prof_steps,
+        "kernel_busy_frac": kern_tot / prof_wall_ms,
         "setup_s": setup_s,
     }
     if not args.no_cpu_baseline:

@@ -125,7 +125,17 @@ typedef struct {
   const uint16_t *param_index;   /* [n_nodes] set of each node, or NULL */
   int sell_sigma;     /* rows sorted by length in windows of this many rows
                          (0 or 1 = keep the mesh order) */
+  int cg_blocks;      /* workgroups of each SpMV/CG launch, rounded up to a
+                         multiple of 8 (0 = the library's choice) */
+  int cg_x0;          /* VG_X0_*: initial guess of each half-step solve */
 } vg_model;
+
+/* Initial guess of the CG solves.  Either way a solve stops when
+ * ||r||_D <= cg_tol ||K_rhs V||_D, the reference's ||r_0|| when it starts from
+ * x_0 = V^n (EP_CrankNicholson.cc:104): the same accuracy target. */
+#define VG_X0_ZERO 0     /* increment 0 (= the reference's x_0 = V^n): the
+                            reference's Krylov iterates */
+#define VG_X0_PREVIOUS 1 /* the previous increment of the same half-step */
 
 typedef struct {
   int64_t steps;
@@ -186,10 +196,10 @@ int vg_ecg_frames(vg_handle *h, double *out, int *n_frames);
 
 int vg_profile_enable(vg_handle *h, int enable);
 int vg_profile_get(vg_handle *h, int kernel_class, double *total_ms, int64_t *launches);
-#define VG_KC_RHS 0
-#define VG_KC_CG_SPMV 1
-#define VG_KC_CG_UPDATE 2
-#define VG_KC_IONIC 3
+#define VG_KC_RHS 0      /* r_0 = K_rhs V */
+#define VG_KC_CG_ITER 1  /* fused CG iteration (update + SpMV), one per launch */
+#define VG_KC_CG_START 2 /* w_0 = A r_0 */
+#define VG_KC_IONIC 3    /* ionic update, Purkinje kernel included */
 #define VG_KC_APPLY 4
 #define VG_KC_COUNT 5
 

@@ -32,14 +32,14 @@ def _run_pair(mesh, model, steps, dt, stim_nodes, istim, stim_steps, gpu_kw, ora
     return gpu, ref
 
 
-def _lr_rest_states(n):
-    # The library's default LR state is the model's rest state; the oracle gets
-    # the same numbers through the states/V0 arguments.
-    return None
+# Grid of the CG launches: the library's choice, and more blocks than slices
+# (empty blocks store zero partials).
+GRIDS = [0, 16]
 
 
+@pytest.mark.parametrize("cg_blocks", GRIDS)
 @pytest.mark.parametrize("model", [vg.MODEL_LR, vg.MODEL_TT])
-def test_fp64_step_matches_oracle(gpu_device, model):
+def test_fp64_step_matches_oracle(gpu_device, model, cg_blocks):
     nx, ny, nz = 13, 11, 4
     mesh = meshgen.slab(nx, ny, nz, 0.025)
     dt = 0.02 if model == vg.MODEL_TT else 0.01
@@ -51,13 +51,14 @@ def test_fp64_step_matches_oracle(gpu_device, model):
     gpu0.close()
     steps = 400 if model == vg.MODEL_TT else 800
     gpu, ref = _run_pair(mesh, model, steps, dt, stim, 50.0 * xi, int(2.0 / dt),
-                         dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10),
+                         dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10,
+                              cg_blocks=cg_blocks),
                          dict(cg_tol=1e-10), V0=V0, states=states)
     Vg = gpu.getVoltage()
     err = np.abs(Vg - ref.V).max()
     at_g, at_o = gpu.activationTimes(), ref.act
     st = gpu.stats()
-    print(f"model {model}: max|dV| = {err:.3e} mV, V in [{ref.V.min():.1f},{ref.V.max():.1f}], "
+    print(f"model {model} grid {cg_blocks}: max|dV| = {err:.3e} mV, V in [{ref.V.min():.1f},{ref.V.max():.1f}], "
           f"CG its/solve GPU {st['cg_iterations'] / st['cg_solves']:.1f} "
           f"oracle {ref.cg_iters_total / (2 * steps):.1f}")
     assert ref.V.max() > 0.0  # a wave is passing through
@@ -68,23 +69,29 @@ def test_fp64_step_matches_oracle(gpu_device, model):
     assert np.abs(at_g[both] - at_o[both]).max() < 1e-6
 
 
-def test_mixed_rush_larsen_tt_activation(gpu_device):
+@pytest.mark.parametrize("cg_blocks", [0, 8])
+def test_mixed_rush_larsen_tt_activation(gpu_device, cg_blocks):
     """FP32 ionic/SpMV + FP64 dots with Rush-Larsen tT against the FP64
-    reference-integrator oracle: activation time within 0.25 ms."""
+    reference-integrator oracle: activation time within 0.25 ms.  With 8
+    blocks each wave takes several slices."""
     nx, ny, nz = 21, 21, 5
     mesh = meshgen.slab(nx, ny, nz, 0.025)
-    dt = 0.02
+    dt, steps = 0.02, 1000
     stim = meshgen.slab_corner_nodes(nx, ny, nz, 3)
-    gpu, ref = _run_pair(mesh, vg.MODEL_TT, 1000, dt, stim, 50.0 * 2000.0, 100,
+    gpu, ref = _run_pair(mesh, vg.MODEL_TT, steps, dt, stim, 50.0 * 2000.0, 100,
                          dict(precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN,
-                              cg_tol=1e-7),
+                              cg_tol=1e-7, cg_blocks=cg_blocks),
                          dict(cg_tol=1e-10))
     at_g, at_o = gpu.activationTimes(), ref.act
-    assert (at_o >= 0).all(), "the whole slab activates within 20 ms"
-    assert (at_g >= 0).all()
-    d = np.abs(at_g - at_o)
-    print(f"mixed tT RL: max |dAT| {d.max():.4f} ms, mean {d.mean():.4f} ms, "
-          f"last activation {at_o.max():.2f} ms")
+    # Nodes either side of the 20 ms window edge may fall either way: compare
+    # every node that one of the two activated before 19.5 ms.
+    t_cut = steps * dt - 0.5
+    early = ((at_o >= 0) & (at_o < t_cut)) | ((at_g >= 0) & (at_g < t_cut))
+    assert early.mean() > 0.3, "the wave crosses a good part of the slab"
+    assert (at_o[early] >= 0).all() and (at_g[early] >= 0).all()
+    d = np.abs(at_g[early] - at_o[early])
+    print(f"mixed tT RL: {early.sum()} nodes, max |dAT| {d.max():.4f} ms, mean {d.mean():.4f} ms, "
+          f"last compared activation {at_o[early].max():.2f} ms")
     assert d.max() <= 0.25
 
 
@@ -100,3 +107,21 @@ def test_ecg_matches_direct_quadrature(gpu_device):
     want = oracle.ecg_direct(mesh, V, el)
     print("ECG", got, want)
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-14 * np.abs(want).max())
+
+
+def test_runs_repeat_bit_for_bit(gpu_device):
+    """Fixed-order FP64 reductions: two runs from the same state give the same
+    bits (and the same CG iteration counts)."""
+    nx, ny, nz = 17, 15, 4
+    mesh = meshgen.slab(nx, ny, nz, 0.025)
+    stim = meshgen.slab_corner_nodes(nx, ny, nz, 3)
+    out = []
+    for _ in range(2):
+        gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, dt=0.02, D=D_SLAB,
+                               precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN)
+        gpu.setStimulusCurrent(stim, 50.0 * 2000.0)
+        gpu.Solve(60)
+        out.append((gpu.getVoltage(), gpu.stats()["cg_iterations"]))
+        gpu.close()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])

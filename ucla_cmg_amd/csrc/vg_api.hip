@@ -98,20 +98,26 @@ struct vg_handle {
   DevBuf off, width, col, Ahat, Krhs, sinv, diag;
   DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
   DevBuf r2, w2, s2;  // ping-pong partners of r, w, s for the fused CG
+  // Increment of the second half-step (x is the first's); with guess = 1 each
+  // solve starts from its half-step's previous increment.
+  DevBuf x2;
+  int guess = 0;
   // Rows [0, n_myo) run the body's cell model, rows [n_myo, n) the Purkinje
   // model (their states in pk_states, SoA with n_pk_alloc lanes per state).
   int64_t n_myo = 0, n_pk = 0, n_pk_alloc = 0;
   DevBuf pk_states;
-  bool fused = true;  // one launch per CG iteration (VG_CG_FUSED=0: two)
-  // Persistent cooperative solve (one launch per half-step, no host sync).
-  bool coop = false;
-  unsigned coop_blocks = 0;
-  DevBuf bar_count, bar_gen;
-  DevBuf cg, part, counter, newton;
-  DevBuf H, ecg_out, ecg_frames;
-  CgState *h_cg = nullptr;  // pinned
+  // The Purkinje ionic kernel runs beside the myocardium's on a second
+  // stream (forked and joined with events).
+  hipStream_t stream_pk = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // CG: grid of the SpMV-type launches (multiple of XCDS), partials, carried
+  // scalars and the result record (pinned copy for the host).
+  unsigned cg_blocks = 0, node_blocks = 0;
+  DevBuf part, carry, res, newton;
+  DevBuf H, ecg_part, ecg_out, ecg_frames;
+  CgResult *h_res = nullptr;  // pinned
   int nsets = 1;
-  int cg_guess = 8;
+  int cg_guess[2] = {8, 8};  // launches expected per solve, by half-step
   // activation
   bool at_enabled = false;
   double at_thr = -40.0;
@@ -140,7 +146,10 @@ struct vg_handle {
         (void)hipEventDestroy(pr.second);
       }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
-    if (h_cg) (void)hipHostFree(h_cg);
+    if (h_res) (void)hipHostFree(h_res);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream_pk) (void)hipStreamDestroy(stream_pk);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -203,148 +212,139 @@ void drain_profile(vg_handle *h) {
   }
 }
 
-Reduce red_of(vg_handle *h) { return Reduce{h->part.as<double>(), h->counter.as<unsigned>()}; }
+CgBufs cgbufs_of(vg_handle *h) {
+  double *part = h->part.as<double>();
+  return CgBufs{part, part + 3 * (size_t)h->cg_blocks, h->carry.as<CgCarry>(),
+                h->res.as<CgResult>()};
+}
 
-unsigned blocks_nodes(const vg_handle *h) { return (unsigned)(h->n_alloc / BLK); }
-unsigned blocks_slices(const vg_handle *h) { return (unsigned)((h->n_slices + WAVES - 1) / WAVES); }
+// Increment buffer of half-step `which` (0: before the ionic update).
+template <class T>
+T *xbuf(vg_handle *h, int which) {
+  return which ? h->x2.as<T>() : h->x.as<T>();
+}
 
 template <class T>
-int launch_rhs(vg_handle *h, const T *V) {
+void launch_rhs(vg_handle *h, const T *V, int which) {
   ProfScope ps(h, VG_KC_RHS);
-  hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
-                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
-                     V, h->r.as<T>(), h->diag.as<T>(), h->n_slices, h->cg.as<CgState>(),
-                     red_of(h), h->tol * h->tol, h->maxit);
-  return VG_OK;
+  if (h->guess)
+    hipLaunchKernelGGL((k_rhs<T, true>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
+                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
+                       h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
+                       h->n_slices, cgbufs_of(h));
+  else
+    hipLaunchKernelGGL((k_rhs<T, false>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
+                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
+                       h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
+                       h->n_slices, cgbufs_of(h));
 }
 
 template <class T>
-void launch_cg_pair(vg_handle *h) {
-  {
-    ProfScope ps(h, VG_KC_CG_SPMV);
-    hipLaunchKernelGGL(k_cg_spmv_dots<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
-                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(),
-                       h->Ahat.as<T>(), h->r.as<T>(), h->w.as<T>(), h->n_slices,
-                       h->cg.as<CgState>(), red_of(h));
-  }
-  {
-    ProfScope ps(h, VG_KC_CG_UPDATE);
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
-                       h->x.as<T>(), h->r.as<T>(), h->p.as<T>(), h->s.as<T>(), h->w.as<T>(),
-                       h->diag.as<T>(), h->n_alloc, h->cg.as<CgState>(), red_of(h));
-  }
-}
-
-template <class T>
-void launch_cg_init(vg_handle *h) {
-  ProfScope ps(h, VG_KC_CG_UPDATE);
-  hipLaunchKernelGGL(k_cg_spmv_dots<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+void launch_cg_start(vg_handle *h) {
+  ProfScope ps(h, VG_KC_CG_START);
+  hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
-                     h->r.as<T>(), h->w.as<T>(), h->n_slices, h->cg.as<CgState>(), red_of(h));
+                     h->r.as<T>(), h->w.as<T>(), h->diag.as<T>(), h->n_slices, cgbufs_of(h));
 }
 
 template <class T>
-void launch_cg_fused(vg_handle *h) {
-  ProfScope ps(h, VG_KC_CG_SPMV);
-  hipLaunchKernelGGL(k_cg_fused<T>, dim3(blocks_slices(h)), dim3(BLK), 0, h->stream,
+void launch_cg_iter(vg_handle *h, int j, int which) {
+  ProfScope ps(h, VG_KC_CG_ITER);
+  hipLaunchKernelGGL(k_cg_iter<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
                      h->r.as<T>(), h->r2.as<T>(), h->w.as<T>(), h->w2.as<T>(), h->s.as<T>(),
-                     h->s2.as<T>(), h->p.as<T>(), h->x.as<T>(), h->diag.as<T>(), h->n_slices,
-                     h->cg.as<CgState>(), red_of(h));
+                     h->s2.as<T>(), h->p.as<T>(), xbuf<T>(h, which), h->diag.as<T>(),
+                     h->n_slices, cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
 }
 
-// The whole half-step (right-hand side + CG to convergence) in one
-// cooperative launch; nothing comes back to the host.
+// One diffusion half-step: r_0 = K_rhs V, then CG to convergence.  The
+// expected number of iteration launches (from the previous solve of this
+// half-step) is queued, then the result record is read back; launches after
+// the solve has ended return at once, and a solve still running gets more.
 template <class T>
-int launch_cg_solve(vg_handle *h, const T *V) {
-  ProfScope ps(h, VG_KC_CG_SPMV);
-  const int64_t *off = h->off.as<int64_t>();
-  const int *width = h->width.as<int>(), *col = h->col.as<int>();
-  const T *Ahat = h->Ahat.as<T>(), *Krhs = h->Krhs.as<T>(), *diag = h->diag.as<T>();
-  T *rA = h->r.as<T>(), *rB = h->r2.as<T>(), *wA = h->w.as<T>(), *wB = h->w2.as<T>();
-  T *sA = h->s.as<T>(), *sB = h->s2.as<T>(), *p = h->p.as<T>(), *x = h->x.as<T>();
-  int64_t ns = h->n_slices;
-  CgState *cg = h->cg.as<CgState>();
-  Reduce red = red_of(h);
-  Barrier bar{h->bar_count.as<unsigned>(), h->bar_gen.as<unsigned>()};
-  double tol2 = h->tol * h->tol;
-  int maxit = h->maxit;
-  void *args[] = {&off, &width, &col, &Ahat, &Krhs, &V,   &rA,  &rB,   &wA,  &wB,    &sA,
-                  &sB,  &p,     &x,   &diag, &ns,   &cg,  &red, &bar, &tol2, &maxit};
-  HIPCHK(h, hipLaunchCooperativeKernel((const void *)k_cg_solve<T>, dim3(h->coop_blocks), dim3(BLK),
-                                       args, 0, h->stream));
-  return VG_OK;
-}
-
-// Runs CG to convergence: launches the expected number of iterations, then
-// reads the device state back and continues if needed.  Launches after
-// convergence return at once.
-template <class T>
-int cg_solve(vg_handle *h) {
-  int launched = 0;
-  int batch = h->cg_guess;
-  if (h->fused) launch_cg_init<T>(h);
+int cg_solve(vg_handle *h, const T *V, int which) {
+  launch_rhs<T>(h, V, which);
+  launch_cg_start<T>(h);
+  int j = 0;
+  int batch = h->cg_guess[which];
   for (;;) {
-    for (int k = 0; k < batch; k++) {
-      if (h->fused) launch_cg_fused<T>(h);
-      else launch_cg_pair<T>(h);
-    }
-    launched += batch;
-    HIPCHK(h, hipMemcpyAsync(h->h_cg, h->cg.p, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
+    for (int k = 0; k < batch && j <= h->maxit; k++) launch_cg_iter<T>(h, j++, which);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(h->h_res, h->res.p, sizeof(CgResult), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->h_cg->done) break;
-    if (launched > h->maxit + 2) break;
+    if (h->h_res->done || j > h->maxit) break;
     batch = 4;
   }
-  const CgState &c = *h->h_cg;
+  const CgResult &c = *h->h_res;
+  if (!c.done) return h->fail(VG_ERR_HIP, "CG launches ended without a result");
   h->cg_iters += c.iter;
   h->cg_solves += 1;
-  h->last_relres = c.rho0 > 0 ? std::sqrt(c.rho / c.rho0) : 0.0;
-  if (c.notconv || !c.done) h->cg_notconv += 1;
-  h->cg_guess = std::max(1, c.iter + 1);
-  if (c.notconv && !std::isfinite(c.rho)) return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
+  h->last_relres = c.rho_ref > 0 ? std::sqrt(c.rho / c.rho_ref) : 0.0;
+  if (c.notconv) h->cg_notconv += 1;
+  // The launch that finds convergence is the (iter+1)-th; one more spare.
+  h->cg_guess[which] = std::min(h->maxit + 1, c.iter + 2);
+  if (c.notconv && !(std::isfinite(c.rho) && std::isfinite(c.rho_ref)))
+    return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
   return VG_OK;
 }
 
 template <class T, int MODEL, int INTEG>
 void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   IonicArgs a{h->n_myo, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0};
-  hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,
-                     Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
+  hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3((unsigned)((h->n_myo + BLK - 1) / BLK)),
+                     dim3(BLK), 0, h->stream, Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
                      h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
                      h->scales.as<T>(), h->sub, a, h->newton.as<unsigned long long>(),
-                     h->cg.as<CgState>());
+                     h->res.as<CgResult>());
 }
 
+// Ionic update of every node.  The Purkinje rows (few, long per-node work)
+// run on the second stream beside the myocardium's kernel.
 template <class T>
-void launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
+int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   ProfScope ps(h, VG_KC_IONIC);
+  const bool fork = h->n_pk > 0 && h->n_myo > 0;
+  if (h->n_pk > 0) {
+    hipStream_t sp = h->stream;
+    if (fork) {
+      HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
+      HIPCHK(h, hipStreamWaitEvent(h->stream_pk, h->ev_fork, 0));
+      sp = h->stream_pk;
+    }
+    hipLaunchKernelGGL(k_ionic_pk<T>, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, sp, Vin,
+                       x, h->sinv.as<T>(), Vout, h->pk_states.as<T>(),
+                       h->stim_active ? h->istim.as<T>() : (const T *)nullptr, h->n_myo, h->n_pk,
+                       h->n_pk_alloc, h->sub, h->res.as<CgResult>());
+    if (fork) HIPCHK(h, hipEventRecord(h->ev_join, h->stream_pk));
+  }
   if (h->n_myo > 0) {
     if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else if (h->integ == VG_INTEG_REFERENCE)
       launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN>(h, Vin, x, Vout);
   }
-  if (h->n_pk > 0)
-    hipLaunchKernelGGL(k_ionic_pk<T>, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, h->stream,
-                       Vin, x, h->sinv.as<T>(), Vout, h->pk_states.as<T>(),
-                       h->stim_active ? h->istim.as<T>() : (const T *)nullptr, h->n_myo, h->n_pk,
-                       h->n_pk_alloc, h->sub, h->cg.as<CgState>());
+  if (fork) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  return VG_OK;
+}
+
+// Sums the ECG partials of the last k_apply / k_ecg launch into dst[ne].
+void launch_ecg_finish(vg_handle *h, double *dst) {
+  hipLaunchKernelGGL(k_ecg_finish, dim3(1), dim3(BLK), 0, h->stream, h->ecg_part.as<double>(),
+                     (int)h->node_blocks, h->ne, dst);
 }
 
 template <class T>
 void launch_apply(vg_handle *h, double t0, bool ecg_frame) {
   ProfScope ps(h, VG_KC_APPLY);
   const T thr = (T)h->at_thr;
-  double *ecg_dst = nullptr;
-  if (ecg_frame) ecg_dst = h->ecg_frames.as<double>() + (size_t)h->ecg_count * h->ne;
   const T *Hm = h->H.p ? h->H.as<T>() : nullptr;
+  double *part = h->ecg_part.p ? h->ecg_part.as<double>() : nullptr;
 #define VG_APPLY(AT, ECG)                                                                        \
-  hipLaunchKernelGGL((k_apply<T, AT, ECG>), dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream,      \
-                     h->Vb.as<T>(), h->x.as<T>(), h->sinv.as<T>(), h->Va.as<T>(), h->n,          \
-                     h->at.as<T>(), thr, t0, h->dt, Hm, h->ne, h->n_alloc, red_of(h), ecg_dst,   \
-                     h->cg.as<CgState>())
+  hipLaunchKernelGGL((k_apply<T, AT, ECG>), dim3(h->node_blocks), dim3(BLK), 0, h->stream,       \
+                     h->Vb.as<T>(), xbuf<T>(h, 1), h->sinv.as<T>(), h->Va.as<T>(), h->n,         \
+                     h->at.as<T>(), thr, t0, h->dt, Hm, h->ne, h->n_alloc, part,                 \
+                     h->res.as<CgResult>())
   if (h->at_enabled) {
     if (ecg_frame) VG_APPLY(true, true);
     else VG_APPLY(true, false);
@@ -353,6 +353,8 @@ void launch_apply(vg_handle *h, double t0, bool ecg_frame) {
     else VG_APPLY(false, false);
   }
 #undef VG_APPLY
+  if (ecg_frame)
+    launch_ecg_finish(h, h->ecg_frames.as<double>() + (size_t)h->ecg_count * h->ne);
 }
 
 int upload_stim(vg_handle *h) {
@@ -385,22 +387,12 @@ int step_t(vg_handle *h) {
   if (rc) return rc;
   const double t0 = h->t;
   // Step 1: half-step diffusion, C_m M + dT/4 K against C_m M - dT/4 K.
-  if (h->coop) {
-    if ((rc = launch_cg_solve<T>(h, h->Va.as<T>()))) return rc;
-  } else {
-    launch_rhs<T>(h, h->Va.as<T>());
-    if ((rc = cg_solve<T>(h))) return rc;
-  }
+  if ((rc = cg_solve<T>(h, h->Va.as<T>(), 0))) return rc;
   // Step 2: ionic update over dT in the reference's sub-steps, fused with
   // V = V_a + D^-1/2 x.
-  launch_ionic<T>(h, h->Va.as<T>(), h->x.as<T>(), h->Vb.as<T>());
+  if ((rc = launch_ionic<T>(h, h->Va.as<T>(), xbuf<T>(h, 0), h->Vb.as<T>()))) return rc;
   // Step 3: second half-step.
-  if (h->coop) {
-    if ((rc = launch_cg_solve<T>(h, h->Vb.as<T>()))) return rc;
-  } else {
-    launch_rhs<T>(h, h->Vb.as<T>());
-    if ((rc = cg_solve<T>(h))) return rc;
-  }
+  if ((rc = cg_solve<T>(h, h->Vb.as<T>(), 1))) return rc;
   h->steps += 1;
   h->t = h->steps * h->dt;
   const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
@@ -563,6 +555,10 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown precision");
   }
+  if (model->cg_x0 != VG_X0_ZERO && model->cg_x0 != VG_X0_PREVIOUS) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "unknown cg_x0");
+  }
   if (!(model->dt > 0) || !(model->dt_ionic > 0) || !(model->Cm > 0) || !(model->cg_tol > 0) ||
       model->cg_maxit < 1) {
     *out = h.release();
@@ -577,6 +573,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->Cm = model->Cm;
   h->tol = model->cg_tol;
   h->maxit = model->cg_maxit;
+  h->guess = model->cg_x0 == VG_X0_PREVIOUS ? 1 : 0;
   h->tsz = h->prec == VG_PREC_FP64 ? 8 : 4;
   vg_handle *hp = h.get();
   if (!substeps(h->dt, h->dt_ion, h->sub)) {
@@ -658,49 +655,42 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     if ((rc = upload_raw<float>(H_, H_->sinv, sinv_pad))) return rc;
     if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
   }
-  // Vectors.
-  // CG driver: the persistent cooperative solve when the device supports it
-  // and every block of it can be resident; VG_CG_MODE=fused|pair selects the
-  // host-driven launch-per-iteration variants (kept for measurement).
+  // Grids.  SpMV-type launches: one block per 4 slices up to 1024 blocks (4
+  // per CU; waves then take several slices each), a multiple of XCDS; the
+  // caller may set it.  Node-wise launches: one thread per node up to 1024
+  // blocks, grid-stride beyond.
   {
-    int coop_ok = 0, ncu = 0, per_cu = 0;
-    (void)hipDeviceGetAttribute(&coop_ok, hipDeviceAttributeCooperativeLaunch, H_->device);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, H_->device);
-    hipError_t oe = T == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                 &per_cu, (const void *)k_cg_solve<double>, BLK, 0)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                 &per_cu, (const void *)k_cg_solve<float>, BLK, 0);
-    const int64_t want = (H_->n_slices + WAVES - 1) / WAVES;
-    const int64_t resident = (int64_t)ncu * per_cu;
-    const char *mode = getenv("VG_CG_MODE");
-    H_->coop = coop_ok && oe == hipSuccess && resident > 0 && mode && std::string(mode) == "coop";
-    H_->coop_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, resident));
-    if (mode && std::string(mode) == "pair") {
-      H_->coop = false;
-      H_->fused = false;
-    }
+    int64_t g = model->cg_blocks > 0 ? model->cg_blocks
+                                     : std::min<int64_t>(1024, (H_->n_slices + WAVES - 1) / WAVES);
+    g = std::max<int64_t>(XCDS, ((g + XCDS - 1) / XCDS) * XCDS);
+    if (g > 65536) return H_->fail(VG_ERR_ARG, "cg_blocks above 65536");
+    H_->cg_blocks = (unsigned)g;
+    H_->node_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, na / BLK));
   }
-  HIPCHK(H_, H_->bar_count.alloc(sizeof(unsigned)));
-  HIPCHK(H_, hipMemset(H_->bar_count.p, 0, sizeof(unsigned)));
-  HIPCHK(H_, H_->bar_gen.alloc(sizeof(unsigned)));
-  HIPCHK(H_, hipMemset(H_->bar_gen.p, 0, sizeof(unsigned)));
+  // Vectors.
   for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at,
-                    &H_->r2, &H_->w2, &H_->s2}) {
+                    &H_->r2, &H_->w2, &H_->s2, &H_->x2}) {
     HIPCHK(H_, b->alloc(na * T));
     HIPCHK(H_, hipMemset(b->p, 0, na * T));
   }
   HIPCHK(H_, H_->states.alloc((size_t)ndev_of(H_->model) * na * T));
-  if (H_->n_pk > 0) HIPCHK(H_, H_->pk_states.alloc((size_t)PK_NSTATE * H_->n_pk_alloc * T));
-  // Reduction scratch: one FP64 partial per block and value.
-  const int64_t maxblocks = std::max<int64_t>(na / BLK, (H_->n_slices + WAVES - 1) / WAVES) + 1;
-  HIPCHK(H_, H_->part.alloc((size_t)maxblocks * MAX_ECG * 8));
-  HIPCHK(H_, H_->counter.alloc(sizeof(unsigned)));
-  HIPCHK(H_, hipMemset(H_->counter.p, 0, sizeof(unsigned)));
-  HIPCHK(H_, H_->cg.alloc(sizeof(CgState)));
-  HIPCHK(H_, hipMemset(H_->cg.p, 0, sizeof(CgState)));
+  if (H_->n_pk > 0) {
+    HIPCHK(H_, H_->pk_states.alloc((size_t)PK_NSTATE * H_->n_pk_alloc * T));
+    HIPCHK(H_, hipStreamCreateWithFlags(&H_->stream_pk, hipStreamNonBlocking));
+    HIPCHK(H_, hipEventCreateWithFlags(&H_->ev_fork, hipEventDisableTiming));
+    HIPCHK(H_, hipEventCreateWithFlags(&H_->ev_join, hipEventDisableTiming));
+  }
+  // CG scratch: two sets of 3 FP64 partials per block, the carried scalars
+  // and the result record.
+  HIPCHK(H_, H_->part.alloc((size_t)H_->cg_blocks * 6 * 8));
+  HIPCHK(H_, hipMemset(H_->part.p, 0, (size_t)H_->cg_blocks * 6 * 8));
+  HIPCHK(H_, H_->carry.alloc(2 * sizeof(CgCarry)));
+  HIPCHK(H_, hipMemset(H_->carry.p, 0, 2 * sizeof(CgCarry)));
+  HIPCHK(H_, H_->res.alloc(sizeof(CgResult)));
+  HIPCHK(H_, hipMemset(H_->res.p, 0, sizeof(CgResult)));
   HIPCHK(H_, H_->newton.alloc(sizeof(unsigned long long)));
   HIPCHK(H_, hipMemset(H_->newton.p, 0, sizeof(unsigned long long)));
-  HIPCHK(H_, hipHostMalloc((void **)&H_->h_cg, sizeof(CgState), hipHostMallocDefault));
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult), hipHostMallocDefault));
   // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi).
   {
     int nsets = model->n_param_sets > 0 ? model->n_param_sets : 1;
@@ -940,7 +930,8 @@ extern "C" int vg_ecg_setup(vg_handle *h, const vg_mesh *mesh, const double *el,
     for (int64_t r = 0; r < h->n; r++) Hp[(size_t)k * na + r] = Hm[(size_t)k * h->n + h->new2old[r]];
   int rc = h->tsz == 8 ? upload_raw<double>(h, h->H, Hp) : upload_raw<float>(h, h->H, Hp);
   if (rc) return rc;
-  HIPCHK(h, h->ecg_out.alloc(MAX_ECG * 8));
+  if (!h->ecg_out.p) HIPCHK(h, h->ecg_out.alloc(MAX_ECG * 8));
+  if (!h->ecg_part.p) HIPCHK(h, h->ecg_part.alloc((size_t)MAX_ECG * h->node_blocks * 8));
   h->ne = ne;
   return VG_OK;
 }
@@ -950,12 +941,14 @@ extern "C" {
 int vg_ecg(vg_handle *h, double *out) {
   if (!h || !out) return VG_ERR_ARG;
   if (h->ne == 0) return h->fail(VG_ERR_ARG, "ECG not set up");
+  double *part = h->ecg_part.as<double>();
   if (h->tsz == 8)
-    hipLaunchKernelGGL(k_ecg<double>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream, h->Va.as<double>(),
-                       h->n, h->H.as<double>(), h->ne, h->n_alloc, red_of(h), h->ecg_out.as<double>());
+    hipLaunchKernelGGL(k_ecg<double>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
+                       h->Va.as<double>(), h->n, h->H.as<double>(), h->ne, h->n_alloc, part);
   else
-    hipLaunchKernelGGL(k_ecg<float>, dim3(blocks_nodes(h)), dim3(BLK), 0, h->stream, h->Va.as<float>(),
-                       h->n, h->H.as<float>(), h->ne, h->n_alloc, red_of(h), h->ecg_out.as<double>());
+    hipLaunchKernelGGL(k_ecg<float>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
+                       h->Va.as<float>(), h->n, h->H.as<float>(), h->ne, h->n_alloc, part);
+  launch_ecg_finish(h, h->ecg_out.as<double>());
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->ecg_out.p, h->ne * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1007,18 +1000,15 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   unsigned long long nw = 0;
   HIPCHK(h, hipMemcpy(&nw, h->newton.p, sizeof(nw), hipMemcpyDeviceToHost));
-  CgState c;
-  HIPCHK(h, hipMemcpy(&c, h->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
-  // Host-driven solves are counted on the host, persistent ones on the device.
   o->steps = h->steps;
-  o->cg_iterations = h->cg_iters + (int64_t)c.iters_total;
-  o->cg_solves = h->cg_solves + (int64_t)c.solves_total;
-  o->cg_not_converged = h->cg_notconv + (int64_t)c.notconv_total;
+  o->cg_iterations = h->cg_iters;
+  o->cg_solves = h->cg_solves;
+  o->cg_not_converged = h->cg_notconv;
   o->newton_iterations = (int64_t)nw;
   o->n_nodes = h->n;
   o->nnz = h->nnz;
   o->sell_padded_nnz = h->padded;
-  o->last_cg_relres = h->coop ? (c.rho0 > 0 ? std::sqrt(c.rho / c.rho0) : 0.0) : h->last_relres;
+  o->last_cg_relres = h->last_relres;
   o->sim_time = h->t;
   return VG_OK;
 }

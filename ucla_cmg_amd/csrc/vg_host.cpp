@@ -274,6 +274,7 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
         w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]));
       }
     }
+    w = (w + SELL_GROUP - 1) / SELL_GROUP * SELL_GROUP;  // the kernels load in groups
     o.width[s] = w;
     o.offset[s + 1] = o.offset[s] + (int64_t)w * SELL_C;
   }

@@ -24,6 +24,7 @@ std::string validate_mesh(const vg_mesh &mesh);
 // SELL-64 layout: rows in slices of 64 lanes, one lane per row; slice s holds
 // width[s] entries per lane stored lane-fastest from offset[s].
 constexpr int SELL_C = 64;
+constexpr int SELL_GROUP = 4;  // slice widths are padded to a multiple of this
 struct HostSell {
   int64_t n = 0, n_slices = 0, padded = 0;
   std::vector<int64_t> offset;   // [n_slices + 1]

@@ -3,11 +3,17 @@
 // Layout in HBM (n_alloc = nodes rounded up to 256, in the solver's row order):
 //   SELL-64 operator: col[int32], A_hat[T], K_rhs[T] per padded entry; one lane
 //     per row, 64 rows per slice, lane-fastest so every load is coalesced.
-//   vectors V_a, V_b, r, w, p, s, x, sinv, diag: [n_alloc] T
+//   vectors V_a, V_b, r, w, s (two copies each), p, x, sinv, diag: [n_alloc] T
 //   ionic states: SoA [nstate][n_alloc] T (V excluded)
-// Reductions are FP64: wavefront butterflies, LDS across the 4 waves of a
-// block, then one FP64 partial per block and a fixed-order sum in the last
-// block to finish (bit-reproducible run to run).
+//
+// Reductions never cross blocks inside a launch.  Each block of a launch sums
+// its values in FP64 (wavefront butterfly, then LDS over the block's waves)
+// and stores one partial per value; every block of the NEXT launch sums those
+// partials itself, in one fixed order, so all blocks hold bit-identical totals
+// and take the same decisions, and results repeat bit for bit run to run.  On
+// a multi-XCD device a release/acquire handshake between blocks of one launch
+// costs a write-back and an invalidation of an XCD's whole L2 per block; the
+// launch boundary does that once.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,24 +25,37 @@ constexpr int BLK = 256;
 constexpr int WAVES = BLK / 64;
 constexpr int MAX_SUB = 32;
 constexpr int MAX_ECG = 16;
-
-struct CgState {
-  double alpha, beta, gamma_old, alpha_old;
-  double rho0, rho, tol2;
-  int iter, maxit, done, first, upd_first, notconv;
-  int par;  // which ping-pong buffer holds the current r, w, s (fused CG)
-  // Running totals kept by the persistent solver across solves.
-  unsigned long long iters_total, solves_total, notconv_total;
-};
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; grids
+// of the SpMV-type launches are a multiple of this.
+constexpr int XCDS = 8;
 
 struct SubSteps {
   int n;
   double dt[MAX_SUB];
 };
 
-struct Reduce {
-  double *part;       // [gridDim.x * NV]
-  unsigned *counter;  // zero between launches
+// CG scalars carried from one iteration launch to the next; launch j reads
+// carry[j & 1] and block 0 writes carry[(j + 1) & 1].
+struct CgCarry {
+  double rho_ref;  // ||b||_D^2: the stopping test is ||r||_D <= tol ||b||_D
+  double gamma;    // (r_j, r_j) of the launch that wrote it
+  double alpha;
+};
+
+// Outcome of one solve, written by block 0 of the launch that ends it.
+struct CgResult {
+  int iter;     // iterations done
+  int done;
+  int notconv;  // ended on maxit, breakdown or a non-finite residual
+  int stop;     // set when the solve has ended; later launches return at once
+  int x_valid;  // x holds the increment (0: the increment is zero)
+  double rho, rho_ref;
+};
+
+struct CgBufs {
+  double *part0, *part1;  // [3][grid] block partials of alternating launches
+  CgCarry *carry;         // [2]
+  CgResult *res;
 };
 
 __device__ inline double wave_sum(double v) {
@@ -45,13 +64,10 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sums NV values over the block, writes the block partial, and returns true
-// in every thread of the block that finishes last; in that block tot[] holds
-// the grid totals, summed in block order.
+// Block sum of NV values, stored as part[k * gridDim.x + blockIdx.x].
 template <int NV>
-__device__ inline bool grid_reduce(double (&v)[NV], Reduce red, double (&tot)[NV]) {
+__device__ inline void put_partials(const double (&v)[NV], double *__restrict__ part) {
   __shared__ double sm[WAVES][NV];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
@@ -59,479 +75,306 @@ __device__ inline bool grid_reduce(double (&v)[NV], Reduce red, double (&tot)[NV
     if (lane == 0) sm[wv][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < NV) {
+    double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0.0;
-      for (int w = 0; w < WAVES; w++) s += sm[w][k];
-      red.part[(size_t)blockIdx.x * NV + k] = s;
-    }
-    __threadfence();
-    const unsigned prev = atomicAdd(red.counter, 1u);
-    last = (prev == gridDim.x - 1);
+    for (int w = 0; w < WAVES; w++) s += sm[w][threadIdx.x];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // Fixed-order sum: thread t takes blocks t, t+BLK, ...; then a tree.
-  __shared__ double st[BLK];
+}
+
+// Totals of the partials that a previous launch of nblk blocks stored; the
+// same value in every thread of every block.
+template <int NV>
+__device__ inline void sum_partials(const double *__restrict__ part, int nblk, double (&tot)[NV]) {
+  __shared__ double sm[WAVES][NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += BLK)
-      s += __builtin_nontemporal_load(&red.part[(size_t)b * NV + k]);
-    st[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = BLK / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) st[threadIdx.x] += st[threadIdx.x + o];
-      __syncthreads();
-    }
-    tot[k] = st[0];
-    __syncthreads();
+    for (int b = threadIdx.x; b < nblk; b += BLK) s += part[(size_t)k * nblk + b];
+    s = wave_sum(s);
+    if (lane == 0) sm[wv][k] = s;
   }
-  if (threadIdx.x == 0) *red.counter = 0u;
-  return true;
-}
-
-__device__ inline int load_done(const CgState *cg) {
-  return __hip_atomic_load(&cg->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- diffusion right-hand side: r = K_rhs V, rho0 = ||r||_D^2 --------------
-template <class T>
-__global__ __launch_bounds__(BLK) void k_rhs(const int64_t *__restrict__ off,
-                                             const int *__restrict__ width,
-                                             const int *__restrict__ col,
-                                             const T *__restrict__ val,
-                                             const T *__restrict__ V, T *__restrict__ r,
-                                             const T *__restrict__ diag, int64_t n_slices,
-                                             CgState *cg, Reduce red, double tol2, int maxit) {
-  const int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  double v[1] = {0.0};
-  if (slice < n_slices) {
-    const int64_t o = off[slice];
-    const int w = width[slice];
-    T acc = T(0);
-    for (int j = 0; j < w; j++) {
-      const int64_t k = o + (int64_t)j * 64 + lane;
-      acc += val[k] * V[col[k]];
-    }
-    const int64_t row = slice * 64 + lane;
-    r[row] = acc;
-    v[0] = (double)diag[row] * (double)acc * (double)acc;
-  }
-  double tot[1];
-  if (grid_reduce<1>(v, red, tot) && threadIdx.x == 0) {
-    cg->rho0 = tot[0];
-    cg->rho = tot[0];
-    cg->tol2 = tol2;
-    cg->maxit = maxit;
-    cg->iter = 0;
-    cg->first = 1;
-    cg->upd_first = 1;
-    cg->notconv = 0;
-    cg->par = 0;
-    cg->alpha = cg->beta = cg->gamma_old = cg->alpha_old = 0.0;
-    cg->done = (tot[0] == 0.0) ? 1 : 0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < WAVES; w++) s += sm[w][k];
+    tot[k] = s;
   }
 }
 
-// ---- CG (Chronopoulos-Gear form, Jacobi scaling folded into A_hat) ---------
-// w = A_hat r, gamma = (r,r), delta = (r,w); the last block forms alpha, beta.
-template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_spmv_dots(const int64_t *__restrict__ off,
-                                                      const int *__restrict__ width,
-                                                      const int *__restrict__ col,
-                                                      const T *__restrict__ val,
-                                                      const T *__restrict__ r, T *__restrict__ w,
-                                                      int64_t n_slices, CgState *cg,
-                                                      Reduce red) {
-  if (load_done(cg)) return;
-  const int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  double v[2] = {0.0, 0.0};
-  if (slice < n_slices) {
-    const int64_t o = off[slice];
-    const int wd = width[slice];
-    T acc = T(0);
-    for (int j = 0; j < wd; j++) {
-      const int64_t k = o + (int64_t)j * 64 + lane;
-      acc += val[k] * r[col[k]];
-    }
-    const int64_t row = slice * 64 + lane;
-    w[row] = acc;
-    const double rr = (double)r[row];
-    v[0] = rr * rr;
-    v[1] = rr * (double)acc;
-  }
-  double tot[2];
-  if (grid_reduce<2>(v, red, tot) && threadIdx.x == 0) {
-    const double gamma = tot[0], delta = tot[1];
-    double alpha, beta;
-    if (cg->first) {
-      beta = 0.0;
-      alpha = gamma / delta;
-    } else {
-      beta = gamma / cg->gamma_old;
-      alpha = gamma / (delta - beta * gamma / cg->alpha_old);
-    }
-    if (!(alpha > 0.0) || !isfinite(alpha)) {  // breakdown: stop, report
-      cg->done = 1;
-      cg->notconv = 1;
-    }
-    cg->alpha = alpha;
-    cg->beta = beta;
-    cg->gamma_old = gamma;
-    cg->alpha_old = alpha;
-    cg->upd_first = cg->first;
-    cg->first = 0;
-  }
+// Slices [lo, hi) of this block.  The slices are cut into gridDim.x (a
+// multiple of XCDS) contiguous chunks, and the chunks of the blocks one XCD
+// runs are adjacent, so the rows an XCD gathers from mostly sit in its own L2.
+__device__ inline void block_slices(int64_t n_slices, int64_t &lo, int64_t &hi) {
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t c = (b % XCDS) * (G / XCDS) + b / XCDS;
+  lo = n_slices * c / G;
+  hi = n_slices * (c + 1) / G;
 }
 
-// p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s; rho = ||r||_D^2.
-template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_update(T *__restrict__ x, T *__restrict__ r,
-                                                   T *__restrict__ p, T *__restrict__ s,
-                                                   const T *__restrict__ w,
-                                                   const T *__restrict__ diag, int64_t n_alloc,
-                                                   CgState *cg, Reduce red) {
-  if (load_done(cg)) return;
-  const T alpha = (T)cg->alpha, beta = (T)cg->beta;
-  const bool first = cg->upd_first != 0;
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  double v[1] = {0.0};
-  if (i < n_alloc) {
-    T pi, si, xi;
-    const T ri = r[i], wi = w[i];
-    if (first) {
-      pi = ri;
-      si = wi;
-      xi = alpha * pi;
-    } else {
-      pi = ri + beta * p[i];
-      si = wi + beta * s[i];
-      xi = x[i] + alpha * pi;
-    }
-    const T rn = ri - alpha * si;
-    p[i] = pi;
-    s[i] = si;
-    x[i] = xi;
-    r[i] = rn;
-    v[0] = (double)diag[i] * (double)rn * (double)rn;
-  }
-  double tot[1];
-  if (grid_reduce<1>(v, red, tot) && threadIdx.x == 0) {
-    cg->iter += 1;
-    cg->rho = tot[0];
-    if (tot[0] <= cg->tol2 * cg->rho0) {
-      cg->done = 1;
-    } else if (cg->iter >= cg->maxit || !isfinite(tot[0])) {
-      cg->done = 1;
-      cg->notconv = 1;
-    }
-  }
-}
-
-// One whole CG iteration in one launch.  Given alpha_k, beta_k (from the
-// previous launch's last block) and the previous r, w, s (read-only here,
-// ping-pong buffers selected by cg->par), each row forms p, s, x, r of
-// iteration k+1 and w_{k+1} = A_hat r_{k+1}; the neighbours' r_{k+1} =
-// r - alpha (w + beta s) is rebuilt from their previous values, so no other
-// block's writes are needed.  The last block tests ||r_{k+1}||_D against
-// tol ||r_0||_D and forms alpha_{k+1}, beta_{k+1}.
-template <class T, bool FIRST>
-__device__ inline T cg_fused_row(const int64_t *__restrict__ off, const int *__restrict__ width,
-                                 const int *__restrict__ col, const T *__restrict__ val,
-                                 const T *__restrict__ r0, const T *__restrict__ w0,
-                                 const T *__restrict__ s0, int64_t slice, int lane, T alpha,
-                                 T beta) {
+// One lane's row of a SELL-64 slice: sum_j val[j] * gather(col[j]).  Slice
+// widths are padded to a multiple of 4, so entries go in groups of 4 with all
+// column and value loads of a group issued before its gathers: each wave keeps
+// a dozen independent loads in flight instead of one dependent chain.
+template <class T, class G>
+__device__ inline T sell_row(const int64_t *__restrict__ off, const int *__restrict__ width,
+                             const int *__restrict__ col, const T *__restrict__ val,
+                             int64_t slice, int lane, G gather) {
   const int64_t o = off[slice];
   const int wd = width[slice];
+  const int *__restrict__ cp = col + o + lane;
+  const T *__restrict__ vp = val + o + lane;
   T acc = T(0);
-  for (int j = 0; j < wd; j++) {
-    const int64_t k = o + (int64_t)j * 64 + lane;
-    const int c = col[k];
-    const T sc = FIRST ? w0[c] : w0[c] + beta * s0[c];
-    acc += val[k] * (r0[c] - alpha * sc);
+#pragma unroll 2
+  for (int j = 0; j < wd; j += 4) {
+    const int c0 = cp[(j + 0) * 64], c1 = cp[(j + 1) * 64];
+    const int c2 = cp[(j + 2) * 64], c3 = cp[(j + 3) * 64];
+    const T v0 = vp[(j + 0) * 64], v1 = vp[(j + 1) * 64];
+    const T v2 = vp[(j + 2) * 64], v3 = vp[(j + 3) * 64];
+    const T g0 = gather(c0), g1 = gather(c1), g2 = gather(c2), g3 = gather(c3);
+    acc += v0 * g0;
+    acc += v1 * g1;
+    acc += v2 * g2;
+    acc += v3 * g3;
   }
   return acc;
 }
 
+// Two operators on one pattern in one pass (columns loaded once):
+// acc_a = sum va[j] ga(col[j]), acc_b = sum vb[j] gb(col[j]).
+template <class T, class GA, class GB>
+__device__ inline void sell_row2(const int64_t *__restrict__ off, const int *__restrict__ width,
+                                 const int *__restrict__ col, const T *__restrict__ va,
+                                 const T *__restrict__ vb, int64_t slice, int lane, GA ga, GB gb,
+                                 T &acc_a, T &acc_b) {
+  const int64_t o = off[slice];
+  const int wd = width[slice];
+  const int *__restrict__ cp = col + o + lane;
+  const T *__restrict__ ap = va + o + lane;
+  const T *__restrict__ bp = vb + o + lane;
+  T sa = T(0), sb = T(0);
+  for (int j = 0; j < wd; j += 4) {
+    int c[4];
+    T a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      c[q] = cp[(j + q) * 64];
+      a[q] = ap[(j + q) * 64];
+      b[q] = bp[(j + q) * 64];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sa += a[q] * ga(c[q]);
+      sb += b[q] * gb(c[q]);
+    }
+  }
+  acc_a = sa;
+  acc_b = sb;
+}
+
+__device__ inline bool cg_stopped(const CgResult *res) {
+  return __hip_atomic_load(&res->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+__device__ inline void cg_set_stop(CgResult *res) {
+  __hip_atomic_store(&res->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- diffusion right-hand side ---------------------------------------------
+// b = K_rhs V and r_0 = b - A_hat x_0 (x_0 = 0 unless GUESS: then x holds the
+// increment of the previous solve of this half-step); partials of ||b||_D^2
+// and ||r_0||_D^2.  Opens a solve: clears the result record (no CG launch of
+// the previous solve runs concurrently: launches on one stream are ordered).
+template <class T, bool GUESS>
+__global__ __launch_bounds__(BLK) void k_rhs(const int64_t *__restrict__ off,
+                                             const int *__restrict__ width,
+                                             const int *__restrict__ col,
+                                             const T *__restrict__ Krhs,
+                                             const T *__restrict__ Ahat,
+                                             const T *__restrict__ V, const T *__restrict__ x,
+                                             T *__restrict__ r, const T *__restrict__ diag,
+                                             int64_t n_slices, CgBufs cb) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cb.res->iter = 0;
+    cb.res->done = 0;
+    cb.res->notconv = 0;
+    cb.res->x_valid = 0;
+    cb.res->rho = cb.res->rho_ref = 0.0;
+    __hip_atomic_store(&cb.res->stop, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  int64_t lo, hi;
+  block_slices(n_slices, lo, hi);
+  const int lane = threadIdx.x & 63;
+  double v[2] = {0.0, 0.0};
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+    const int64_t row = sl * 64 + lane;
+    const double d = (double)diag[row];
+    if constexpr (GUESS) {
+      T bk, ax;
+      sell_row2<T>(off, width, col, Krhs, Ahat, sl, lane, [&](int c) { return V[c]; },
+                   [&](int c) { return x[c]; }, bk, ax);
+      const T rr = bk - ax;
+      r[row] = rr;
+      v[0] += d * (double)bk * (double)bk;
+      v[1] += d * (double)rr * (double)rr;
+    } else {
+      const T bk = sell_row<T>(off, width, col, Krhs, sl, lane, [&](int c) { return V[c]; });
+      r[row] = bk;
+      const double e = d * (double)bk * (double)bk;
+      v[0] += e;
+      v[1] += e;
+    }
+  }
+  put_partials<2>(v, cb.part1);
+}
+
+// ---- CG (Chronopoulos-Gear form, Jacobi scaling folded into A_hat) ---------
+// Start: ||b||_D^2 from the right-hand side's partials (b = 0: the increment
+// is zero); w_0 = A_hat r_0 and partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.
 template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_fused(const int64_t *__restrict__ off,
+__global__ __launch_bounds__(BLK) void k_cg_start(const int64_t *__restrict__ off,
                                                   const int *__restrict__ width,
                                                   const int *__restrict__ col,
-                                                  const T *__restrict__ val, T *rA, T *rB, T *wA,
-                                                  T *wB, T *sA, T *sB, T *__restrict__ p,
-                                                  T *__restrict__ x, const T *__restrict__ diag,
-                                                  int64_t n_slices, CgState *cg, Reduce red) {
-  if (load_done(cg)) return;
-  const bool first = cg->upd_first != 0;
-  const int par = cg->par;
-  const T *r0 = par ? rB : rA, *w0 = par ? wB : wA, *s0 = par ? sB : sA;
-  T *r1 = par ? rA : rB, *w1 = par ? wA : wB, *s1 = par ? sA : sB;
-  const T alpha = (T)cg->alpha, beta = first ? T(0) : (T)cg->beta;
-  const int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+                                                  const T *__restrict__ val,
+                                                  const T *__restrict__ r, T *__restrict__ w,
+                                                  const T *__restrict__ diag, int64_t n_slices,
+                                                  CgBufs cb) {
+  double t0[2];
+  sum_partials<2>(cb.part1, gridDim.x, t0);
+  const double rho_ref = t0[0];
+  const bool b0 = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(rho_ref > 0.0) || !isfinite(rho_ref)) {
+    if (b0) {
+      cb.res->rho_ref = cb.res->rho = rho_ref;
+      cb.res->done = 1;
+      cb.res->notconv = rho_ref == 0.0 ? 0 : 1;
+      cg_set_stop(cb.res);
+    }
+    return;
+  }
+  int64_t lo, hi;
+  block_slices(n_slices, lo, hi);
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
-  if (slice < n_slices) {
-    const int64_t row = slice * 64 + lane;
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+    const T acc = sell_row<T>(off, width, col, val, sl, lane, [&](int c) { return r[c]; });
+    const int64_t row = sl * 64 + lane;
+    w[row] = acc;
+    const double rr = (double)r[row];
+    v[0] += rr * rr;
+    v[1] += rr * (double)acc;
+    v[2] += (double)diag[row] * rr * rr;
+  }
+  put_partials<3>(v, cb.part0);
+  if (b0) {
+    cb.carry[0] = CgCarry{rho_ref, 0.0, 0.0};
+    cb.res->rho_ref = rho_ref;
+    cb.res->rho = t0[1];
+  }
+}
+
+// Iteration j in one launch.  From the partials of launch j-1 every block
+// forms gamma_j = (r_j, r_j), delta_j = (r_j, w_j), rho_j = ||r_j||_D^2, tests
+// rho_j <= tol^2 ||b||_D^2 and forms alpha_j, beta_j; then each row updates
+//   p_j = r_j + beta_j p_{j-1},  s_j = w_j + beta_j s_{j-1},
+//   x_{j+1} = x_j + alpha_j p_j,  r_{j+1} = r_j - alpha_j s_j,
+// and w_{j+1} = A_hat r_{j+1}, rebuilding the neighbours' r_{j+1} from their
+// r_j, w_j, s_{j-1} (ping-pong buffers by parity of j), so no other block's
+// writes of this launch are read.
+template <class T, bool FIRST>
+__device__ inline T cg_iter_row(const int64_t *__restrict__ off, const int *__restrict__ width,
+                                const int *__restrict__ col, const T *__restrict__ val,
+                                const T *__restrict__ r0, const T *__restrict__ w0,
+                                const T *__restrict__ s0, int64_t slice, int lane, T alpha,
+                                T beta) {
+  return sell_row<T>(off, width, col, val, slice, lane, [&](int c) {
+    const T sc = FIRST ? w0[c] : w0[c] + beta * s0[c];
+    return r0[c] - alpha * sc;
+  });
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void k_cg_iter(const int64_t *__restrict__ off,
+                                                 const int *__restrict__ width,
+                                                 const int *__restrict__ col,
+                                                 const T *__restrict__ val, T *rA, T *rB, T *wA,
+                                                 T *wB, T *sA, T *sB, T *__restrict__ p,
+                                                 T *__restrict__ x, const T *__restrict__ diag,
+                                                 int64_t n_slices, CgBufs cb, int j, double tol2,
+                                                 int maxit, int guess) {
+  if (cg_stopped(cb.res)) return;
+  const int par = j & 1;
+  double t[3];
+  sum_partials<3>(par ? cb.part1 : cb.part0, gridDim.x, t);
+  const CgCarry c = cb.carry[par];
+  const double gamma = t[0], delta = t[1], rho = t[2];
+  int end = 0;  // 1: converged, 2: not converged
+  double alpha = 0.0, beta = 0.0;
+  if (rho <= tol2 * c.rho_ref) {
+    end = 1;
+  } else if (j >= maxit || !isfinite(rho)) {
+    end = 2;
+  } else {
+    if (j == 0) {
+      alpha = gamma / delta;
+    } else {
+      beta = gamma / c.gamma;
+      alpha = gamma / (delta - beta * gamma / c.alpha);
+    }
+    if (!(alpha > 0.0) || !isfinite(alpha)) end = 2;  // breakdown
+  }
+  if (end) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      cb.res->iter = j;
+      cb.res->done = 1;
+      cb.res->notconv = end == 2 ? 1 : 0;
+      cb.res->x_valid = (j > 0 || guess) ? 1 : 0;
+      cb.res->rho = rho;
+      cg_set_stop(cb.res);
+    }
+    return;
+  }
+  const T *r0 = par ? rB : rA, *w0 = par ? wB : wA, *s0 = par ? sB : sA;
+  T *r1 = par ? rA : rB, *w1 = par ? wA : wB, *s1 = par ? sA : sB;
+  const T a = (T)alpha, bt = (T)beta;
+  int64_t lo, hi;
+  block_slices(n_slices, lo, hi);
+  const int lane = threadIdx.x & 63;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+    const int64_t row = sl * 64 + lane;
     const T ri = r0[row], wi = w0[row];
     T pi, si, xi, acc;
-    if (first) {
+    if (j == 0) {
       pi = ri;
       si = wi;
-      xi = alpha * pi;
-      acc = cg_fused_row<T, true>(off, width, col, val, r0, w0, s0, slice, lane, alpha, beta);
+      xi = guess ? x[row] + a * pi : a * pi;
+      acc = cg_iter_row<T, true>(off, width, col, val, r0, w0, s0, sl, lane, a, bt);
     } else {
-      pi = ri + beta * p[row];
-      si = wi + beta * s0[row];
-      xi = x[row] + alpha * pi;
-      acc = cg_fused_row<T, false>(off, width, col, val, r0, w0, s0, slice, lane, alpha, beta);
+      pi = ri + bt * p[row];
+      si = wi + bt * s0[row];
+      xi = x[row] + a * pi;
+      acc = cg_iter_row<T, false>(off, width, col, val, r0, w0, s0, sl, lane, a, bt);
     }
-    const T rn = ri - alpha * si;
+    const T rn = ri - a * si;
     r1[row] = rn;
     w1[row] = acc;
     s1[row] = si;
     p[row] = pi;
     x[row] = xi;
     const double rd = (double)rn;
-    v[0] = rd * rd;
-    v[1] = rd * (double)acc;
-    v[2] = (double)diag[row] * rd * rd;
+    v[0] += rd * rd;
+    v[1] += rd * (double)acc;
+    v[2] += (double)diag[row] * rd * rd;
   }
-  double tot[3];
-  if (grid_reduce<3>(v, red, tot) && threadIdx.x == 0) {
-    cg->iter += 1;
-    cg->rho = tot[2];
-    cg->par = par ^ 1;
-    if (tot[2] <= cg->tol2 * cg->rho0) {
-      cg->done = 1;
-    } else if (cg->iter >= cg->maxit || !isfinite(tot[2])) {
-      cg->done = 1;
-      cg->notconv = 1;
-    } else {
-      const double gamma = tot[0], delta = tot[1];
-      const double beta_n = gamma / cg->gamma_old;
-      const double alpha_n = gamma / (delta - beta_n * gamma / cg->alpha_old);
-      if (!(alpha_n > 0.0) || !isfinite(alpha_n)) {
-        cg->done = 1;
-        cg->notconv = 1;
-      }
-      cg->alpha = alpha_n;
-      cg->beta = beta_n;
-      cg->gamma_old = gamma;
-      cg->alpha_old = alpha_n;
-      cg->upd_first = 0;
-    }
-  }
-}
-
-// ---- persistent half-step solve ---------------------------------------------
-// One cooperative launch does a whole diffusion half-step: r0 = K_rhs V,
-// w0 = A_hat r0, then CG iterations until ||r||_D <= tol ||r0||_D or maxit.
-// Every block loops over slices grid-stride; phases are separated by a grid
-// barrier whose last arriving block also finishes the FP64 reductions (in
-// block order) and updates the CG scalars before releasing the others, so
-// every block leaves the loop on the same iteration.
-struct Barrier {
-  unsigned *count;  // arrivals in the current phase (reset by the last block)
-  unsigned *gen;    // phase generation, bumped by the last block
-};
-
-// Block-sums NV values into red.part[block]; in the last block to arrive,
-// `finish(tot)` runs (thread 0) before the barrier opens.
-template <int NV, class F>
-__device__ inline void barrier_reduce(double (&v)[NV], Reduce red, Barrier bar, F finish) {
-  __shared__ double sm[WAVES][NV];
-  __shared__ bool last;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    const double s = wave_sum(v[k]);
-    if (lane == 0) sm[wv][k] = s;
-  }
-  __syncthreads();
-  unsigned g = 0;
-  if (threadIdx.x == 0) {
-    g = __hip_atomic_load(bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0.0;
-      for (int w = 0; w < WAVES; w++) s += sm[w][k];
-      red.part[(size_t)blockIdx.x * NV + k] = s;
-    }
-    __threadfence();
-    last = atomicAdd(bar.count, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    __shared__ double st[BLK];
-    double tot[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0.0;
-      for (unsigned b = threadIdx.x; b < gridDim.x; b += BLK)
-        s += __builtin_nontemporal_load(&red.part[(size_t)b * NV + k]);
-      st[threadIdx.x] = s;
-      __syncthreads();
-      for (int o = BLK / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) st[threadIdx.x] += st[threadIdx.x + o];
-        __syncthreads();
-      }
-      tot[k] = st[0];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      finish(tot);
-      __hip_atomic_store(bar.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      __hip_atomic_fetch_add(bar.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  } else if (threadIdx.x == 0) {
-    while (__hip_atomic_load(bar.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
-      __builtin_amdgcn_s_sleep(1);
-  }
-  __syncthreads();
-  __threadfence();
-}
-
-template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_solve(const int64_t *__restrict__ off,
-                                                  const int *__restrict__ width,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ Ahat,
-                                                  const T *__restrict__ Krhs,
-                                                  const T *__restrict__ V, T *rA, T *rB, T *wA,
-                                                  T *wB, T *sA, T *sB, T *__restrict__ p,
-                                                  T *__restrict__ x, const T *__restrict__ diag,
-                                                  int64_t n_slices, CgState *cg, Reduce red,
-                                                  Barrier bar, double tol2, int maxit) {
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * WAVES;
-  const int64_t s0 = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  // Phase 0: r0 = K_rhs V and ||r0||_D^2.
-  {
-    double v[1] = {0.0};
-    for (int64_t sl = s0; sl < n_slices; sl += stride) {
-      const int64_t o = off[sl];
-      const int wd = width[sl];
-      T acc = T(0);
-      for (int j = 0; j < wd; j++) {
-        const int64_t k = o + (int64_t)j * 64 + lane;
-        acc += Krhs[k] * V[col[k]];
-      }
-      const int64_t row = sl * 64 + lane;
-      rA[row] = acc;
-      v[0] += (double)diag[row] * (double)acc * (double)acc;
-    }
-    barrier_reduce<1>(v, red, bar, [&](const double *t) {
-      cg->solves_total += 1;
-      cg->rho0 = t[0];
-      cg->rho = t[0];
-      cg->tol2 = tol2;
-      cg->maxit = maxit;
-      cg->iter = 0;
-      cg->notconv = 0;
-      cg->par = 0;
-      cg->done = (t[0] == 0.0) ? 1 : 0;
-    });
-  }
-  if (__hip_atomic_load(&cg->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  // Phase 1: w0 = A_hat r0, gamma0 = (r0,r0), delta0 = (r0,w0).
-  {
-    double v[2] = {0.0, 0.0};
-    for (int64_t sl = s0; sl < n_slices; sl += stride) {
-      const int64_t o = off[sl];
-      const int wd = width[sl];
-      T acc = T(0);
-      for (int j = 0; j < wd; j++) {
-        const int64_t k = o + (int64_t)j * 64 + lane;
-        acc += Ahat[k] * rA[col[k]];
-      }
-      const int64_t row = sl * 64 + lane;
-      wA[row] = acc;
-      const double rr = (double)rA[row];
-      v[0] += rr * rr;
-      v[1] += rr * (double)acc;
-    }
-    barrier_reduce<2>(v, red, bar, [&](const double *t) {
-      const double alpha = t[0] / t[1];
-      cg->alpha = alpha;
-      cg->beta = 0.0;
-      cg->gamma_old = t[0];
-      cg->alpha_old = alpha;
-      cg->upd_first = 1;
-      if (!(alpha > 0.0) || !isfinite(alpha)) {
-        cg->done = 1;
-        cg->notconv = 1;
-        cg->notconv_total += 1;
-      }
-    });
-  }
-  // Iterations: the fused update + SpMV of k_cg_fused, one barrier each.
-  for (;;) {
-    if (__hip_atomic_load(&cg->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const bool first = cg->upd_first != 0;
-    const int par = cg->par;
-    const T *r0 = par ? rB : rA, *w0 = par ? wB : wA, *sp0 = par ? sB : sA;
-    T *r1 = par ? rA : rB, *w1 = par ? wA : wB, *s1 = par ? sA : sB;
-    const T alpha = (T)cg->alpha, beta = first ? T(0) : (T)cg->beta;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int64_t sl = s0; sl < n_slices; sl += stride) {
-      const int64_t row = sl * 64 + lane;
-      const T ri = r0[row], wi = w0[row];
-      T pi, si, xi, acc;
-      if (first) {
-        pi = ri;
-        si = wi;
-        xi = alpha * pi;
-        acc = cg_fused_row<T, true>(off, width, col, Ahat, r0, w0, sp0, sl, lane, alpha, beta);
-      } else {
-        pi = ri + beta * p[row];
-        si = wi + beta * sp0[row];
-        xi = x[row] + alpha * pi;
-        acc = cg_fused_row<T, false>(off, width, col, Ahat, r0, w0, sp0, sl, lane, alpha, beta);
-      }
-      const T rn = ri - alpha * si;
-      r1[row] = rn;
-      w1[row] = acc;
-      s1[row] = si;
-      p[row] = pi;
-      x[row] = xi;
-      const double rd = (double)rn;
-      v[0] += rd * rd;
-      v[1] += rd * (double)acc;
-      v[2] += (double)diag[row] * rd * rd;
-    }
-    barrier_reduce<3>(v, red, bar, [&](const double *t) {
-      cg->iter += 1;
-      cg->iters_total += 1;
-      cg->rho = t[2];
-      cg->par = par ^ 1;
-      cg->upd_first = 0;
-      if (t[2] <= cg->tol2 * cg->rho0) {
-        cg->done = 1;
-      } else if (cg->iter >= cg->maxit || !isfinite(t[2])) {
-        cg->done = 1;
-        cg->notconv = 1;
-        cg->notconv_total += 1;
-      } else {
-        const double beta_n = t[0] / cg->gamma_old;
-        const double alpha_n = t[0] / (t[1] - beta_n * t[0] / cg->alpha_old);
-        if (!(alpha_n > 0.0) || !isfinite(alpha_n)) {
-          cg->done = 1;
-          cg->notconv = 1;
-          cg->notconv_total += 1;
-        }
-        cg->alpha = alpha_n;
-        cg->beta = beta_n;
-        cg->gamma_old = t[0];
-        cg->alpha_old = alpha_n;
-      }
-    });
+  put_partials<3>(v, par ? cb.part0 : cb.part1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cb.carry[par ^ 1] = CgCarry{c.rho_ref, gamma, alpha};
+    cb.res->iter = j + 1;
   }
 }
 
@@ -551,11 +394,11 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, SubSteps sub,
                                                IonicArgs a, unsigned long long *newton_sum,
-                                               const CgState *cg) {
+                                               const CgResult *cg) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   int newton = 0;
-  // x holds the half-step increment only if the solve did an iteration.
-  const bool use_x = x && cg->iter > 0;
+  // x holds the half-step increment unless that increment is zero.
+  const bool use_x = x && cg->x_valid;
   if (i < a.n) {
     T V = Vin[i];
     if (use_x) V += sinv[i] * x[i];
@@ -614,12 +457,12 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
                                                   const T *__restrict__ sinv, T *__restrict__ Vout,
                                                   T *__restrict__ st, const T *__restrict__ istim,
                                                   int64_t row0, int64_t n, int64_t n_alloc,
-                                                  SubSteps sub, const CgState *cg) {
+                                                  SubSteps sub, const CgResult *cg) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const int64_t row = row0 + i;
   T V = Vin[row];
-  if (x && cg->iter > 0) V += sinv[row] * x[row];
+  if (x && cg->x_valid) V += sinv[row] * x[row];
   const T stim = istim ? istim[row] : T(0);
   T s[PK_NSTATE];
 #pragma unroll
@@ -634,19 +477,20 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
 }
 
 // ---- second half-step apply: V_a = V_b + D^-1/2 x, activation, ECG ---------
+// Grid-stride over nodes; with ECG each block stores MAX_ECG partials of
+// sum_n H[e][n] V_n, finished by k_ecg_finish.
 template <class T, bool AT, bool ECG>
 __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T *__restrict__ x,
                                                const T *__restrict__ sinv, T *__restrict__ Va,
                                                int64_t n, T *__restrict__ at, T thr, double t0,
                                                double dt, const T *__restrict__ H, int ne,
-                                               int64_t n_alloc, Reduce red,
-                                               double *__restrict__ ecg_out, const CgState *cg) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  const bool use_x = cg->iter > 0;
+                                               int64_t n_alloc, double *__restrict__ part,
+                                               const CgResult *cg) {
+  const bool use_x = cg->x_valid != 0;
   double v[MAX_ECG];
 #pragma unroll
   for (int e = 0; e < MAX_ECG; e++) v[e] = 0.0;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
     const T Vn = use_x ? Vb[i] + sinv[i] * x[i] : Vb[i];
     if constexpr (AT) {
       const T Vo = Va[i];
@@ -655,30 +499,37 @@ __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T
     }
     Va[i] = Vn;
     if constexpr (ECG) {
-      for (int e = 0; e < ne; e++) v[e] = (double)H[(int64_t)e * n_alloc + i] * (double)Vn;
+#pragma unroll
+      for (int e = 0; e < MAX_ECG; e++)
+        if (e < ne) v[e] += (double)H[(int64_t)e * n_alloc + i] * (double)Vn;
     }
   }
-  if constexpr (ECG) {
-    double tot[MAX_ECG];
-    if (grid_reduce<MAX_ECG>(v, red, tot) && threadIdx.x == 0)
-      for (int e = 0; e < ne; e++) ecg_out[e] = tot[e];
-  }
+  if constexpr (ECG) put_partials<MAX_ECG>(v, part);
 }
 
+// ECG of V alone (partials as k_apply's).
 template <class T>
 __global__ __launch_bounds__(BLK) void k_ecg(const T *__restrict__ V, int64_t n,
                                              const T *__restrict__ H, int ne, int64_t n_alloc,
-                                             Reduce red, double *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+                                             double *__restrict__ part) {
   double v[MAX_ECG];
 #pragma unroll
   for (int e = 0; e < MAX_ECG; e++) v[e] = 0.0;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
     const double Vi = (double)V[i];
-    for (int e = 0; e < ne; e++) v[e] = (double)H[(int64_t)e * n_alloc + i] * Vi;
+#pragma unroll
+    for (int e = 0; e < MAX_ECG; e++)
+      if (e < ne) v[e] += (double)H[(int64_t)e * n_alloc + i] * Vi;
   }
+  put_partials<MAX_ECG>(v, part);
+}
+
+// One block: out[e] = sum of the nblk partials of lead e.
+__global__ __launch_bounds__(BLK) void k_ecg_finish(const double *__restrict__ part, int nblk,
+                                                    int ne, double *__restrict__ out) {
   double tot[MAX_ECG];
-  if (grid_reduce<MAX_ECG>(v, red, tot) && threadIdx.x == 0)
+  sum_partials<MAX_ECG>(part, nblk, tot);
+  if (threadIdx.x == 0)
     for (int e = 0; e < ne; e++) out[e] = tot[e];
 }
 
@@ -707,7 +558,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
     // All 22 states are kept: the model advances its own V copy internally.
     T s[PK_NSTATE];
 #pragma unroll
-    for (int k = 1; k < PK_NSTATE; k++) s[k] = st[k * n_alloc + i];
+    for (int k = 0; k < PK_NSTATE; k++) s[k] = st[k * n_alloc + i];
     dvdt[i] = (double)pk_step<T, sizeof(T) == 4>(s, V, stim, dt);
 #pragma unroll
     for (int k = 0; k < PK_NSTATE; k++) st[k * n_alloc + i] = s[k];

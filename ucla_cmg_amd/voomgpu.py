@@ -35,8 +35,9 @@ QUIRK_PMJ_STICKY = 2
 QUIRK_BAR_1X_VOLUME = 4
 QUIRK_NEWTON_JT = 8
 TT_NSCALE = 5
-KC_RHS, KC_CG_SPMV, KC_CG_UPDATE, KC_IONIC, KC_APPLY = range(5)
-KC_NAMES = ["rhs", "cg_spmv", "cg_update", "ionic", "apply"]
+X0_ZERO, X0_PREVIOUS = 0, 1
+KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY = range(5)
+KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply"]
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = [
@@ -69,7 +70,8 @@ class VgModel(C.Structure):
         ("Cm", C.c_double), ("dt", C.c_double), ("dt_ionic", C.c_double),
         ("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("quirks", C.c_uint),
         ("n_param_sets", C.c_int), ("param_scales", C.c_void_p),
-        ("param_index", C.c_void_p), ("sell_sigma", C.c_int),
+        ("param_index", C.c_void_p), ("sell_sigma", C.c_int), ("cg_blocks", C.c_int),
+        ("cg_x0", C.c_int),
     ]
 
 
@@ -182,7 +184,7 @@ class _MeshView:
 def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERENCE,
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
-               param_index=None, sell_sigma=0):
+               param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -208,6 +210,8 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
         keep.append(pi)
         m.param_index = _p(pi)
     m.sell_sigma = sell_sigma
+    m.cg_blocks = cg_blocks
+    m.cg_x0 = cg_x0
     return m, keep
 
 
