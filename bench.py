@@ -129,6 +129,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cg-blocks", type=int, default=0,
                     help="workgroups per CG launch (0 = library default)")
+    ap.add_argument("--cg-x0", choices=["zero", "previous", "last"], default="zero",
+                    help="initial guess of each half-step solve (same stopping bound)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of this many MiB 20 times (PMC byte "
+                         "counter calibration) and exit")
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",
                     help="Purkinje tree with high/low PMJ resistance, or none")
     args = ap.parse_args()
@@ -141,6 +146,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 20}))
+        return
     t_setup = time.perf_counter()
     mesh, scales, pidx = lv_workload(args.scale, args.purkinje)
     if args.purkinje == "none":
@@ -149,7 +163,10 @@ def main():
         stim, amp = his_stimulus(mesh), 2.0 * 50.0 * 2500.0
     solver = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED,
                               integrator=vg.INTEG_RUSH_LARSEN, param_scales=scales,
-                              param_index=pidx, cg_blocks=args.cg_blocks, **MODEL_KW)
+                              param_index=pidx, cg_blocks=args.cg_blocks,
+                              cg_x0={"zero": vg.X0_ZERO, "previous": vg.X0_PREVIOUS,
+                                     "last": vg.X0_LAST}[args.cg_x0],
+                              **MODEL_KW)
     solver.enableActivation(-40.0)
     solver.setupECG(electrodes10())
     total_steps = args.warmup + args.steps
@@ -236,6 +253,7 @@ def main():
                         "dt 0.02 ms, CG tol 1e-7, AT + 10-lead ECG/ms",
             "nodes": n, "tets": mesh.n_elems, "nnz": nnz, "cg_iters_per_step": k_per_step,
             "cg_blocks": args.cg_blocks,
+            "cg_x0": args.cg_x0,
             "parallelism": "single GPU",
         },
         "roofline": {

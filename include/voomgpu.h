@@ -60,7 +60,8 @@ extern "C" {
 
 /* Ionic integrators. */
 #define VG_INTEG_REFERENCE 0   /* the reference's scheme: LR Rush-Larsen + FE;
-                                  tT backward-Euler gates + Newton (J^T solve) */
+                                  tT backward-Euler gates + Newton (J solve;
+                                  VG_QUIRK_NEWTON_JT for the reference's J^T) */
 #define VG_INTEG_RUSH_LARSEN 1 /* tT: Rush-Larsen gates + forward-Euler
                                   concentrations, one rate evaluation */
 
@@ -117,7 +118,7 @@ typedef struct {
   double Cm;          /* uF/cm^2 */
   double dt;          /* diffusion step (ms) */
   double dt_ionic;    /* ionic sub-step (ms) */
-  double cg_tol;      /* ||r||/||r0|| */
+  double cg_tol;      /* stop at ||r||_D <= cg_tol ||K_rhs V||_D (see VG_X0_*) */
   int cg_maxit;
   unsigned quirks;    /* VG_QUIRK_* */
   int n_param_sets;   /* tT: number of scaling sets (0 = all ones) */
@@ -136,6 +137,7 @@ typedef struct {
 #define VG_X0_ZERO 0     /* increment 0 (= the reference's x_0 = V^n): the
                             reference's Krylov iterates */
 #define VG_X0_PREVIOUS 1 /* the previous increment of the same half-step */
+#define VG_X0_LAST 2     /* the increment of the last solve (either half-step) */
 
 typedef struct {
   int64_t steps;
@@ -146,7 +148,7 @@ typedef struct {
   int64_t n_nodes;
   int64_t nnz;
   int64_t sell_padded_nnz;
-  double last_cg_relres;      /* ||r||/||r0|| of the last solve */
+  double last_cg_relres;      /* ||r||_D / ||K_rhs V||_D of the last solve */
   double sim_time;            /* ms */
 } vg_stats;
 

@@ -95,13 +95,15 @@ struct vg_handle {
   bool stim_dirty = false, stim_active = false;
   SubSteps sub{};
   // device
-  DevBuf off, width, col, Ahat, Krhs, sinv, diag;
+  DevBuf off, width, wide, col16, col32, Ahat, Krhs, sinv, diag;
   DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
   DevBuf r2, w2, s2;  // ping-pong partners of r, w, s for the fused CG
-  // Increment of the second half-step (x is the first's); with guess = 1 each
-  // solve starts from its half-step's previous increment.
+  // Increment of the second half-step (x is the first's) unless both
+  // half-steps share x (cg_x0 = VG_X0_LAST); with guess = 1 each solve starts
+  // from what its increment buffer holds.
   DevBuf x2;
   int guess = 0;
+  bool shared_x = false;
   // Rows [0, n_myo) run the body's cell model, rows [n_myo, n) the Purkinje
   // model (their states in pk_states, SoA with n_pk_alloc lanes per state).
   int64_t n_myo = 0, n_pk = 0, n_pk_alloc = 0;
@@ -212,6 +214,11 @@ void drain_profile(vg_handle *h) {
   }
 }
 
+Sell sell_of(vg_handle *h) {
+  return Sell{h->off.as<int64_t>(), h->width.as<int>(), h->wide.as<int64_t>(),
+              h->col16.as<int16_t>(), h->col32.as<int>()};
+}
+
 CgBufs cgbufs_of(vg_handle *h) {
   double *part = h->part.as<double>();
   return CgBufs{part, part + 3 * (size_t)h->cg_blocks, h->carry.as<CgCarry>(),
@@ -221,7 +228,7 @@ CgBufs cgbufs_of(vg_handle *h) {
 // Increment buffer of half-step `which` (0: before the ionic update).
 template <class T>
 T *xbuf(vg_handle *h, int which) {
-  return which ? h->x2.as<T>() : h->x.as<T>();
+  return (which && !h->shared_x) ? h->x2.as<T>() : h->x.as<T>();
 }
 
 template <class T>
@@ -229,12 +236,12 @@ void launch_rhs(vg_handle *h, const T *V, int which) {
   ProfScope ps(h, VG_KC_RHS);
   if (h->guess)
     hipLaunchKernelGGL((k_rhs<T, true>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
+                       sell_of(h), h->Krhs.as<T>(),
                        h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
                        h->n_slices, cgbufs_of(h));
   else
     hipLaunchKernelGGL((k_rhs<T, false>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                       h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Krhs.as<T>(),
+                       sell_of(h), h->Krhs.as<T>(),
                        h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
                        h->n_slices, cgbufs_of(h));
 }
@@ -243,7 +250,7 @@ template <class T>
 void launch_cg_start(vg_handle *h) {
   ProfScope ps(h, VG_KC_CG_START);
   hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
+                     sell_of(h), h->Ahat.as<T>(),
                      h->r.as<T>(), h->w.as<T>(), h->diag.as<T>(), h->n_slices, cgbufs_of(h));
 }
 
@@ -251,7 +258,7 @@ template <class T>
 void launch_cg_iter(vg_handle *h, int j, int which) {
   ProfScope ps(h, VG_KC_CG_ITER);
   hipLaunchKernelGGL(k_cg_iter<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                     h->off.as<int64_t>(), h->width.as<int>(), h->col.as<int>(), h->Ahat.as<T>(),
+                     sell_of(h), h->Ahat.as<T>(),
                      h->r.as<T>(), h->r2.as<T>(), h->w.as<T>(), h->w2.as<T>(), h->s.as<T>(),
                      h->s2.as<T>(), h->p.as<T>(), xbuf<T>(h, which), h->diag.as<T>(),
                      h->n_slices, cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
@@ -555,7 +562,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown precision");
   }
-  if (model->cg_x0 != VG_X0_ZERO && model->cg_x0 != VG_X0_PREVIOUS) {
+  if (model->cg_x0 != VG_X0_ZERO && model->cg_x0 != VG_X0_PREVIOUS && model->cg_x0 != VG_X0_LAST) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown cg_x0");
   }
@@ -573,7 +580,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->Cm = model->Cm;
   h->tol = model->cg_tol;
   h->maxit = model->cg_maxit;
-  h->guess = model->cg_x0 == VG_X0_PREVIOUS ? 1 : 0;
+  h->guess = model->cg_x0 != VG_X0_ZERO ? 1 : 0;
+  h->shared_x = model->cg_x0 == VG_X0_LAST;
   h->tsz = h->prec == VG_PREC_FP64 ? 8 : 4;
   vg_handle *hp = h.get();
   if (!substeps(h->dt, h->dt_ion, h->sub)) {
@@ -639,8 +647,13 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, hipMemcpy(H_->off.p, sell.offset.data(), sell.offset.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(H_, H_->width.alloc(sell.width.size() * 4));
   HIPCHK(H_, hipMemcpy(H_->width.p, sell.width.data(), sell.width.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(H_, H_->col.alloc(sell.col.size() * 4));
-  HIPCHK(H_, hipMemcpy(H_->col.p, sell.col.data(), sell.col.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->wide.alloc(sell.wide.size() * 8));
+  HIPCHK(H_, hipMemcpy(H_->wide.p, sell.wide.data(), sell.wide.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->col16.alloc(sell.col16.size() * 2));
+  HIPCHK(H_, hipMemcpy(H_->col16.p, sell.col16.data(), sell.col16.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->col32.alloc(sell.col32.size() * 4));
+  if (!sell.col32.empty())
+    HIPCHK(H_, hipMemcpy(H_->col32.p, sell.col32.data(), sell.col32.size() * 4, hipMemcpyHostToDevice));
   std::vector<double> sinv_pad(na, 0.0), diag_pad(na, 0.0);
   std::copy(sell.sinv.begin(), sell.sinv.begin() + std::min<int64_t>(na, sell.sinv.size()), sinv_pad.begin());
   std::copy(sell.diag.begin(), sell.diag.begin() + std::min<int64_t>(na, sell.diag.size()), diag_pad.begin());

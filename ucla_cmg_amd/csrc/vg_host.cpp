@@ -274,7 +274,6 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
         w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]));
       }
     }
-    w = (w + SELL_GROUP - 1) / SELL_GROUP * SELL_GROUP;  // the kernels load in groups
     o.width[s] = w;
     o.offset[s + 1] = o.offset[s] + (int64_t)w * SELL_C;
   }
@@ -313,6 +312,31 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
           o.col[slot] = (int32_t)std::min<int64_t>(r, n - 1);  // in-bounds pad, value 0
         }
       }
+    }
+  }
+  // Column encoding per slice.
+  o.col16.assign(o.padded, 0);
+  o.wide.assign(o.n_slices, -1);
+  o.col32.clear();
+  for (int64_t s = 0; s < o.n_slices; s++) {
+    bool narrow = true;
+    for (int j = 0; j < o.width[s] && narrow; j++)
+      for (int l = 0; l < SELL_C; l++) {
+        const int64_t d = (int64_t)o.col[o.offset[s] + (int64_t)j * SELL_C + l] - (s * SELL_C + l);
+        if (d < -32767 || d > 32767) {
+          narrow = false;
+          break;
+        }
+      }
+    const int64_t a = o.offset[s], b = o.offset[s + 1];
+    if (narrow) {
+      for (int64_t k = a; k < b; k++) {
+        const int64_t row = s * SELL_C + (k - a) % SELL_C;
+        o.col16[k] = (int16_t)(o.col[k] - row);
+      }
+    } else {
+      o.wide[s] = (int64_t)o.col32.size();
+      o.col32.insert(o.col32.end(), o.col.begin() + a, o.col.begin() + b);
     }
   }
   return "";

@@ -22,14 +22,19 @@ std::string assemble_csr(const vg_mesh &mesh, const vg_model &model, HostCsr &ou
 std::string validate_mesh(const vg_mesh &mesh);
 
 // SELL-64 layout: rows in slices of 64 lanes, one lane per row; slice s holds
-// width[s] entries per lane stored lane-fastest from offset[s].
+// width[s] (its longest row) entries per lane stored lane-fastest from
+// offset[s].  Columns are stored as 16-bit offsets from the row (col16) in
+// every slice whose columns all lie within +-32767 rows of their row, else as
+// int32 in col32 from wide[s] (wide[s] = -1 for a 16-bit slice).
 constexpr int SELL_C = 64;
-constexpr int SELL_GROUP = 4;  // slice widths are padded to a multiple of this
 struct HostSell {
   int64_t n = 0, n_slices = 0, padded = 0;
   std::vector<int64_t> offset;   // [n_slices + 1]
   std::vector<int32_t> width;    // [n_slices]
-  std::vector<int32_t> col;      // [padded]
+  std::vector<int32_t> col;      // [padded] absolute columns (host copy)
+  std::vector<int16_t> col16;    // [padded]
+  std::vector<int32_t> col32;    // wide slices only, same lane-fastest layout
+  std::vector<int64_t> wide;     // [n_slices]
   std::vector<double> A_hat;     // D^-1/2 (Cm M + dt/4 K) D^-1/2
   std::vector<double> K_rhs;     // -(dt/2) D^-1/2 K
   std::vector<double> sinv;      // [n_pad] D^-1/2

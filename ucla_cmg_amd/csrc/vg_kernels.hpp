@@ -116,64 +116,121 @@ __device__ inline void block_slices(int64_t n_slices, int64_t &lo, int64_t &hi) 
   hi = n_slices * (c + 1) / G;
 }
 
-// One lane's row of a SELL-64 slice: sum_j val[j] * gather(col[j]).  Slice
-// widths are padded to a multiple of 4, so entries go in groups of 4 with all
-// column and value loads of a group issued before its gathers: each wave keeps
-// a dozen independent loads in flight instead of one dependent chain.
-template <class T, class G>
-__device__ inline T sell_row(const int64_t *__restrict__ off, const int *__restrict__ width,
-                             const int *__restrict__ col, const T *__restrict__ val,
-                             int64_t slice, int lane, G gather) {
-  const int64_t o = off[slice];
-  const int wd = width[slice];
-  const int *__restrict__ cp = col + o + lane;
-  const T *__restrict__ vp = val + o + lane;
+// SELL-64 operator pattern (values are separate arrays on the same layout).
+// Slice s: width[s] entries per lane, lane-fastest from off[s]; columns as
+// 16-bit offsets from the row in col16 (same layout), or, for a slice with a
+// column further than 32767 rows away (wide[s] >= 0), as int32 in col32 from
+// wide[s].
+struct Sell {
+  const int64_t *off;
+  const int *width;
+  const int64_t *wide;
+  const int16_t *col16;
+  const int *col32;
+};
+
+// sum over j < wd of v(j) * gather(col(j)) in order j = 0, 1, ...; loads go in
+// groups of 4 (then the 0-3 left over), all column and value loads of a
+// group issued before its gathers, so each wave keeps a dozen independent
+// loads in flight instead of one dependent chain.
+template <class T, class CF, class VF, class G>
+__device__ inline T sell_dot(int wd, CF colat, VF valat, G gather) {
   T acc = T(0);
+  int j = 0;
 #pragma unroll 2
-  for (int j = 0; j < wd; j += 4) {
-    const int c0 = cp[(j + 0) * 64], c1 = cp[(j + 1) * 64];
-    const int c2 = cp[(j + 2) * 64], c3 = cp[(j + 3) * 64];
-    const T v0 = vp[(j + 0) * 64], v1 = vp[(j + 1) * 64];
-    const T v2 = vp[(j + 2) * 64], v3 = vp[(j + 3) * 64];
-    const T g0 = gather(c0), g1 = gather(c1), g2 = gather(c2), g3 = gather(c3);
-    acc += v0 * g0;
-    acc += v1 * g1;
-    acc += v2 * g2;
-    acc += v3 * g3;
+  for (; j + 4 <= wd; j += 4) {
+    int c[4];
+    T v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      c[q] = colat(j + q);
+      v[q] = valat(j + q);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc += v[q] * gather(c[q]);
+  }
+  const int k = wd - j;
+  if (k > 0) {
+    int c[3];
+    T v[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < k) {
+        c[q] = colat(j + q);
+        v[q] = valat(j + q);
+      }
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < k) acc += v[q] * gather(c[q]);
   }
   return acc;
+}
+
+// One lane's row of a slice: sum_j val[j] * gather(col[j]).
+template <class T, class G>
+__device__ inline T sell_row(const Sell &S, const T *__restrict__ val, int64_t slice, int lane,
+                             G gather) {
+  const int64_t o = S.off[slice] + lane;
+  const int wd = S.width[slice];
+  const int64_t wo = S.wide[slice];
+  const T *__restrict__ vp = val + o;
+  auto valat = [&](int j) { return vp[j * 64]; };
+  if (wo < 0) {
+    const int row = (int)(slice * 64 + lane);
+    const int16_t *__restrict__ cp = S.col16 + o;
+    return sell_dot<T>(wd, [&](int j) { return row + (int)cp[j * 64]; }, valat, gather);
+  }
+  const int *__restrict__ cp = S.col32 + wo + lane;
+  return sell_dot<T>(wd, [&](int j) { return cp[j * 64]; }, valat, gather);
 }
 
 // Two operators on one pattern in one pass (columns loaded once):
 // acc_a = sum va[j] ga(col[j]), acc_b = sum vb[j] gb(col[j]).
 template <class T, class GA, class GB>
-__device__ inline void sell_row2(const int64_t *__restrict__ off, const int *__restrict__ width,
-                                 const int *__restrict__ col, const T *__restrict__ va,
-                                 const T *__restrict__ vb, int64_t slice, int lane, GA ga, GB gb,
-                                 T &acc_a, T &acc_b) {
-  const int64_t o = off[slice];
-  const int wd = width[slice];
-  const int *__restrict__ cp = col + o + lane;
-  const T *__restrict__ ap = va + o + lane;
-  const T *__restrict__ bp = vb + o + lane;
-  T sa = T(0), sb = T(0);
-  for (int j = 0; j < wd; j += 4) {
-    int c[4];
-    T a[4], b[4];
+__device__ inline void sell_row2(const Sell &S, const T *__restrict__ va, const T *__restrict__ vb,
+                                 int64_t slice, int lane, GA ga, GB gb, T &acc_a, T &acc_b) {
+  const int64_t o = S.off[slice] + lane;
+  const int wd = S.width[slice];
+  const int64_t wo = S.wide[slice];
+  const T *__restrict__ ap = va + o;
+  const T *__restrict__ bp = vb + o;
+  // Both sums share the column loads: gather a pair, accumulate both.
+  struct P {
+    T a, b;
+  };
+  auto pairdot = [&](auto colat) {
+    T sa = T(0), sb = T(0);
+    int j = 0;
+    for (; j < wd; j += 4) {
+      int c[4];
+      T x[4], y[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      c[q] = cp[(j + q) * 64];
-      a[q] = ap[(j + q) * 64];
-      b[q] = bp[(j + q) * 64];
-    }
+      for (int q = 0; q < 4; q++)
+        if (j + q < wd) {
+          c[q] = colat(j + q);
+          x[q] = ap[(j + q) * 64];
+          y[q] = bp[(j + q) * 64];
+        }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sa += a[q] * ga(c[q]);
-      sb += b[q] * gb(c[q]);
+      for (int q = 0; q < 4; q++)
+        if (j + q < wd) {
+          sa += x[q] * ga(c[q]);
+          sb += y[q] * gb(c[q]);
+        }
     }
+    return P{sa, sb};
+  };
+  P r;
+  if (wo < 0) {
+    const int row = (int)(slice * 64 + lane);
+    const int16_t *__restrict__ cp = S.col16 + o;
+    r = pairdot([&](int j) { return row + (int)cp[j * 64]; });
+  } else {
+    const int *__restrict__ cp = S.col32 + wo + lane;
+    r = pairdot([&](int j) { return cp[j * 64]; });
   }
-  acc_a = sa;
-  acc_b = sb;
+  acc_a = r.a;
+  acc_b = r.b;
 }
 
 __device__ inline bool cg_stopped(const CgResult *res) {
@@ -190,10 +247,7 @@ __device__ inline void cg_set_stop(CgResult *res) {
 // and ||r_0||_D^2.  Opens a solve: clears the result record (no CG launch of
 // the previous solve runs concurrently: launches on one stream are ordered).
 template <class T, bool GUESS>
-__global__ __launch_bounds__(BLK) void k_rhs(const int64_t *__restrict__ off,
-                                             const int *__restrict__ width,
-                                             const int *__restrict__ col,
-                                             const T *__restrict__ Krhs,
+__global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
                                              const T *__restrict__ Ahat,
                                              const T *__restrict__ V, const T *__restrict__ x,
                                              T *__restrict__ r, const T *__restrict__ diag,
@@ -215,14 +269,14 @@ __global__ __launch_bounds__(BLK) void k_rhs(const int64_t *__restrict__ off,
     const double d = (double)diag[row];
     if constexpr (GUESS) {
       T bk, ax;
-      sell_row2<T>(off, width, col, Krhs, Ahat, sl, lane, [&](int c) { return V[c]; },
+      sell_row2<T>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c]; },
                    [&](int c) { return x[c]; }, bk, ax);
       const T rr = bk - ax;
       r[row] = rr;
       v[0] += d * (double)bk * (double)bk;
       v[1] += d * (double)rr * (double)rr;
     } else {
-      const T bk = sell_row<T>(off, width, col, Krhs, sl, lane, [&](int c) { return V[c]; });
+      const T bk = sell_row<T>(S, Krhs, sl, lane, [&](int c) { return V[c]; });
       r[row] = bk;
       const double e = d * (double)bk * (double)bk;
       v[0] += e;
@@ -236,10 +290,7 @@ __global__ __launch_bounds__(BLK) void k_rhs(const int64_t *__restrict__ off,
 // Start: ||b||_D^2 from the right-hand side's partials (b = 0: the increment
 // is zero); w_0 = A_hat r_0 and partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.
 template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_start(const int64_t *__restrict__ off,
-                                                  const int *__restrict__ width,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ val,
+__global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ val,
                                                   const T *__restrict__ r, T *__restrict__ w,
                                                   const T *__restrict__ diag, int64_t n_slices,
                                                   CgBufs cb) {
@@ -261,7 +312,7 @@ __global__ __launch_bounds__(BLK) void k_cg_start(const int64_t *__restrict__ of
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
-    const T acc = sell_row<T>(off, width, col, val, sl, lane, [&](int c) { return r[c]; });
+    const T acc = sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
     const int64_t row = sl * 64 + lane;
     w[row] = acc;
     const double rr = (double)r[row];
@@ -286,22 +337,18 @@ __global__ __launch_bounds__(BLK) void k_cg_start(const int64_t *__restrict__ of
 // r_j, w_j, s_{j-1} (ping-pong buffers by parity of j), so no other block's
 // writes of this launch are read.
 template <class T, bool FIRST>
-__device__ inline T cg_iter_row(const int64_t *__restrict__ off, const int *__restrict__ width,
-                                const int *__restrict__ col, const T *__restrict__ val,
+__device__ inline T cg_iter_row(const Sell &S, const T *__restrict__ val,
                                 const T *__restrict__ r0, const T *__restrict__ w0,
                                 const T *__restrict__ s0, int64_t slice, int lane, T alpha,
                                 T beta) {
-  return sell_row<T>(off, width, col, val, slice, lane, [&](int c) {
+  return sell_row<T>(S, val, slice, lane, [&](int c) {
     const T sc = FIRST ? w0[c] : w0[c] + beta * s0[c];
     return r0[c] - alpha * sc;
   });
 }
 
 template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_iter(const int64_t *__restrict__ off,
-                                                 const int *__restrict__ width,
-                                                 const int *__restrict__ col,
-                                                 const T *__restrict__ val, T *rA, T *rB, T *wA,
+__global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ val, T *rA, T *rB, T *wA,
                                                  T *wB, T *sA, T *sB, T *__restrict__ p,
                                                  T *__restrict__ x, const T *__restrict__ diag,
                                                  int64_t n_slices, CgBufs cb, int j, double tol2,
@@ -353,12 +400,12 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(const int64_t *__restrict__ off
       pi = ri;
       si = wi;
       xi = guess ? x[row] + a * pi : a * pi;
-      acc = cg_iter_row<T, true>(off, width, col, val, r0, w0, s0, sl, lane, a, bt);
+      acc = cg_iter_row<T, true>(S, val, r0, w0, s0, sl, lane, a, bt);
     } else {
       pi = ri + bt * p[row];
       si = wi + bt * s0[row];
       xi = x[row] + a * pi;
-      acc = cg_iter_row<T, false>(off, width, col, val, r0, w0, s0, sl, lane, a, bt);
+      acc = cg_iter_row<T, false>(S, val, r0, w0, s0, sl, lane, a, bt);
     }
     const T rn = ri - a * si;
     r1[row] = rn;

@@ -35,7 +35,7 @@ QUIRK_PMJ_STICKY = 2
 QUIRK_BAR_1X_VOLUME = 4
 QUIRK_NEWTON_JT = 8
 TT_NSCALE = 5
-X0_ZERO, X0_PREVIOUS = 0, 1
+X0_ZERO, X0_PREVIOUS, X0_LAST = 0, 1, 2
 KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY = range(5)
 KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply"]
 
