@@ -22,8 +22,11 @@ TT_NSTATE = 19
 TT_NCONST = 53
 TT_NALG = 70
 LR_NSTATE = 8
+MJ_NSTATE = 26
+MJ_NCONST = 35
 MODEL_LR = 0
 MODEL_TT = 2
+MODEL_MJ = 3
 
 
 def build() -> None:
@@ -83,6 +86,7 @@ class _Sim(C.Structure):
         ("cg_iters_total", C.c_int64), ("newton_iters_total", C.c_int64),
         ("nthreads", C.c_int), ("tt_newton_jt", C.c_int),
         ("pk_flag", C.c_void_p), ("pk_states", C.c_void_p),
+        ("mj_const", C.c_void_p), ("mj_int_abs", C.c_int),
     ]
 
 
@@ -287,16 +291,24 @@ class Simulation:
     def __init__(self, mesh, model, dt, dt_ion=None, Cm=1.0, cg_tol=1e-10, cg_maxit=1000,
                  D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, DpLeft=None, V0=None, states=None,
                  tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None,
-                 newton_jt=False):
+                 newton_jt=False, mj_const=None, mj_int_abs=False):
+        """Mahajan (model 3) takes each node's tissue type 0-8 from
+        ``param_idx`` (default 2) and its initial state from that type's row."""
         self.csr = csr if csr is not None else assemble(mesh, D=D, Dp=Dp, DpLeft=DpLeft, Cm=Cm,
                                                         pmj_sticky=pmj_sticky)
         n = self.csr.n
         self.n = n
         self.model = model
-        nst = LR_NSTATE if model == MODEL_LR else TT_NSTATE
+        nst = {MODEL_LR: LR_NSTATE, MODEL_TT: TT_NSTATE, MODEL_MJ: MJ_NSTATE}[model]
         if states is None:
-            one = lr_init() if model == MODEL_LR else tt_init()
-            states = np.tile(one, (n, 1))
+            if model == MODEL_MJ:
+                types = (np.full(n, 2) if param_idx is None
+                         else np.asarray(param_idx).astype(int))
+                table = np.array([mj_init(t) for t in range(9)])
+                states = table[np.where(types < 9, types, 2)]
+            else:
+                one = lr_init() if model == MODEL_LR else tt_init()
+                states = np.tile(one, (n, 1))
         self.states = np.ascontiguousarray(states, np.float64).reshape(n, nst).copy()
         if V0 is None:
             V0 = self.states[:, 7] if model == MODEL_LR else self.states[:, 0]
@@ -328,6 +340,9 @@ class Simulation:
         s.param_idx = _ptr(self.param_idx)
         s.nthreads = nthreads
         s.tt_newton_jt = int(bool(newton_jt))
+        self.mj_const = None if mj_const is None else np.ascontiguousarray(mj_const, np.float64)
+        s.mj_const = _ptr(self.mj_const)
+        s.mj_int_abs = int(bool(mj_int_abs))
         tissue = getattr(mesh, "tissue", None)
         self.pk_flag = None
         self.pk_states = None
@@ -420,3 +435,54 @@ def pk_compute_ion(const, state, Cm, dt, V, istim):
     L = lib()
     _declare_pk(L)
     return L.or_pk_compute_ion(np.ascontiguousarray(const, np.float64), state, Cm, dt, V, istim)
+
+
+# ---- Mahajan 2008 ------------------------------------------------------------
+def _declare_mj(L):
+    L.or_mj_constants.argtypes = [C.c_int, _dp]
+    L.or_mj_init.argtypes = [C.c_int, _dp]
+    L.or_mj_xi.restype = C.c_double
+    L.or_mj_rates.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp]
+    L.or_mj_compute_ion.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int]
+    L.or_mj_compute_ion.restype = C.c_double
+
+
+def mj_constants(tissue_type=2):
+    L = lib()
+    _declare_mj(L)
+    c = np.zeros(MJ_NCONST)
+    L.or_mj_constants(int(tissue_type), c)
+    return c
+
+
+def mj_init(tissue_type=2):
+    L = lib()
+    _declare_mj(L)
+    s = np.zeros(MJ_NSTATE)
+    L.or_mj_init(int(tissue_type), s)
+    return s
+
+
+def mj_xi():
+    L = lib()
+    _declare_mj(L)
+    return L.or_mj_xi()
+
+
+def mj_rates(const, state, h, istim_uA_per_uF=0.0):
+    """Every rate of Mahajan::ucla_rhsfun at gate step ``h`` (state unchanged)."""
+    L = lib()
+    _declare_mj(L)
+    r = np.zeros(MJ_NSTATE)
+    L.or_mj_rates(np.ascontiguousarray(const, np.float64),
+                  np.ascontiguousarray(state, np.float64), h, istim_uA_per_uF, r)
+    return r
+
+
+def mj_compute_ion(const, state, Cm, dt, V, istim, int_abs=False):
+    """Advances ``state`` in place; returns dV/dt (Mahajan::Compute_Ion)."""
+    L = lib()
+    _declare_mj(L)
+    return L.or_mj_compute_ion(np.ascontiguousarray(const, np.float64), state, Cm, dt, V, istim,
+                               int(bool(int_abs)))

@@ -30,6 +30,11 @@
  *                      parity unpinned beyond rest/upstroke markers.
  *   or_ecg_direct      brute-force numpy quadrature sum, uniform V -> 0,
  *                      dipole far-field scaling
+ *   or_mj_rates        Ca and Na conservation across the flux terms; the
+ *                      Markov I_CaL rows sum to -dpo/dt; gate relaxation
+ *                      closed form; near-stationarity at the reference's
+ *                      initial states; constants and initial states against
+ *                      tests/golden/ionic_tables.json
  */
 #ifndef VOOM_ORACLE_H
 #define VOOM_ORACLE_H
@@ -82,7 +87,7 @@ int or_pcg(int64_t n, const int64_t *rp, const int32_t *col, const double *A,
            const double *b, double *x, double tol, int maxit, double *relres);
 
 /* ---- ionic models ------------------------------------------------------ */
-enum { OR_LR = 0, OR_TT = 2 };
+enum { OR_LR = 0, OR_TT = 2, OR_MJ = 3 };
 
 #define OR_LR_NSTATE 8
 #define OR_TT_NSTATE 19
@@ -114,9 +119,19 @@ void or_pk_rates(const double *c, const double *state, double *alg, double *rate
 double or_pk_compute_ion(const double *c, double *s, double Cm, double dt, double V,
                          double istim);
 
+#define OR_MJ_NSTATE 26
+#define OR_MJ_NCONST 35
+void or_mj_constants(int type, double *c);
+void or_mj_init(int type, double *s);
+double or_mj_xi(void);
+void or_mj_rates(const double *c, const double *s, double h, double istim_uA_per_uF,
+                 double *rates);
+double or_mj_compute_ion(const double *c, double *s, double Cm, double dt, double V,
+                         double istim, int int_abs);
+
 /* ---- whole-step driver ------------------------------------------------- */
 typedef struct {
-  int model;            /* OR_LR or OR_TT */
+  int model;            /* OR_LR, OR_TT or OR_MJ */
   int64_t n;
   const or_csr *csr;
   double dt, dt_ion, Cm, cg_tol;
@@ -125,7 +140,7 @@ typedef struct {
   double *states;       /* [n*nstate] AoS like getInternalParameters */
   double *istim;        /* [n] uA/cm^3 */
   const double *tt_const; /* [n_param*53] or NULL -> reference constants */
-  const uint16_t *param_idx; /* [n] or NULL */
+  const uint16_t *param_idx; /* [n] or NULL; Mahajan: tissue type 0-8 */
   /* work */
   double *A, *B, *rhs, *r, *z, *p, *q, *diag;
   int64_t cg_iters_total;
@@ -134,6 +149,8 @@ typedef struct {
   int tt_newton_jt;     /* 1 = solve Newton steps with J^T like the reference */
   const int8_t *pk_flag; /* [n] 1 = Purkinje node (runs the Purkinje model), or NULL */
   double *pk_states;    /* [n*22] AoS, used where pk_flag is set */
+  const double *mj_const; /* [9*35] Mahajan constants by tissue type, or NULL */
+  int mj_int_abs;       /* 1 = adaptive threshold read through int abs() */
 } or_sim;
 
 int or_sim_init(or_sim *s);

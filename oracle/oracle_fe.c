@@ -404,6 +404,21 @@ int or_cn_step(or_sim *s) {
                                             dt, s->V[i], s->istim[i]);
         s->V[i] += dt * dv;
       }
+    } else if (s->model == OR_MJ) {
+      /* CardiacBody::New case 3 (CardiacBody.cc:1045-1067): constants and
+       * initial state by the node's tissue type. */
+      double mc[9 * OR_MJ_NCONST];
+      for (int t = 0; t < 9; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
+      const double *mt = s->mj_const ? s->mj_const : mc;
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int64_t i = 0; i < n; i++) {
+        if (pkf && pkf[i]) continue;
+        const int t = s->param_idx ? s->param_idx[i] : 2;
+        const double dv = or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 2),
+                                            s->states + OR_MJ_NSTATE * i, s->Cm, dt, s->V[i],
+                                            s->istim[i], s->mj_int_abs);
+        s->V[i] += dt * dv;
+      }
     } else {
       double tc[OR_TT_NCONST];
       or_tt_constants(tc);

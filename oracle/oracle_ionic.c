@@ -579,3 +579,309 @@ double or_pk_compute_ion(const double *C, double *S, double Cm, double dt, doubl
   }
   return rates[0];
 }
+
+/* ---- Mahajan 2008 (UCLA rabbit ventricle) ----------------------------------
+ * IonicModel/Mahajan.cc and SetUpMahajanParameters (SetUpIonicConstants.h:
+ * 159-251).  State order of getInternalParameters: V, m, h, j, xr, xs1, xs2,
+ * c_p, c_s, c_i, c_j, c_j', I_r, and the L-type Ca channel Markov states
+ * c1, c2, xi1ca, xi1ba, xi2ca, xi2ba, then Na_i, x_tof, y_tof, x_tos, y_tos
+ * and the troponin-bound Ca of myoplasm and submembrane. */
+enum {
+  MJ_V, MJ_M, MJ_H, MJ_J, MJ_XR, MJ_XS1, MJ_XS2, MJ_CP, MJ_CS, MJ_CI, MJ_CJ, MJ_CJP, MJ_IR,
+  MJ_C1, MJ_C2, MJ_XI1CA, MJ_XI1BA, MJ_XI2CA, MJ_XI2BA, MJ_NAI, MJ_XTOF, MJ_YTOF, MJ_XTOS,
+  MJ_YTOS, MJ_TROPI, MJ_TROPS
+};
+
+/* Per tissue type (epi/myo/endo x apex/center/base, SetUpIonicConstants.h:
+ * 175-186): G_tof and G_Ks. */
+static const double MJ_GTOF[9] = {0.11, 0.11, 0.11, 0.11, 0.11, 0.11, 0.11 * 0.85, 0.11 * 0.85,
+                                  0.11 * 0.85};
+static const double MJ_GKS[9] = {1.9 * 0.32,  1.4 * 0.32,  1.0 * 0.32, 0.74 * 0.32, 0.52 * 0.32,
+                                 0.35 * 0.32, 0.98 * 0.32, 0.7 * 0.32, 0.5 * 0.32};
+
+void or_mj_constants(int type, double *c) {
+  for (int k = 0; k < OR_MJ_NCONST; k++) c[k] = 0.0;
+  c[0] = 136.0;   /* Na_o (mM) */
+  c[1] = 140.0;   /* K_i */
+  c[2] = 5.4;     /* K_o */
+  c[3] = 1.8;     /* Ca_o */
+  c[4] = 182.0;   /* g_Ca */
+  c[5] = 1.0;     /* (unused I_Ca factor) */
+  c[6] = 0.04;    /* g_tos */
+  if (type >= 0 && type < 9) {
+    c[7] = MJ_GTOF[type];
+    c[10] = MJ_GKS[type];
+  } else { /* the reference's fallback: epi-base with G_Ks 0.32 */
+    c[7] = 0.11;
+    c[10] = 0.32;
+  }
+  c[8] = 0.84;    /* g_NaCa */
+  c[9] = 0.0125;  /* g_Kr */
+  c[11] = 0.3;    /* g_K1 */
+  c[12] = 1.5;    /* g_NaK */
+  c[13] = 0.4;    /* v_up */
+  c[14] = 4.0;    /* tau_s */
+  c[15] = 12.0;   /* g_Na */
+  c[16] = 30.0;   /* tau_r */
+  c[17] = 100.0;  /* tau_a */
+  c[18] = 11.3;   /* a_v */
+  c[19] = 90.0;   /* c* */
+  c[20] = 308.0;  /* T */
+  c[21] = -80.0;  /* (unused V_c) */
+  c[22] = 8.314;  /* R */
+  c[23] = 96.485; /* F */
+  c[24] = c[23] / (c[22] * c[20]);
+  c[25] = 153.0;  /* q_max */
+  c[26] = 50.0;   /* release threshold */
+  c[27] = c[28] = c[29] = c[30] = 1.0; /* J_leak, J_d, J_trpn, J_up multipliers */
+  c[31] = c[32] = c[33] = 0.0;         /* m, h, j leaks */
+  c[34] = 1.0;                         /* tau_j multiplier */
+}
+
+/* Initial states by tissue type (Mahajan.cc:27-158; out-of-range types take
+ * the reference's default row, equal to type 2). */
+static const double MJ_INIT[9][OR_MJ_NSTATE] = {
+    {-81.4359, 0.002789, 0.96679, 0.97859, 0.014994, 0.044487, 0.091992, 2.1933, 0.23277,
+     0.24868, 101.6418, 95.4278, 0.009495, 3.7e-005, 0.96094, 0.003718, 6.7e-005, 0.010571,
+     0.024659, 11.5909, 0.00533, 0.99047, 0.005333, 0.17488, 21.5955, 20.0206},
+    {-81.3846, 0.002813, 0.96642, 0.97829, 0.015109, 0.04988, 0.10153, 2.3513, 0.23616, 0.2529,
+     102.5325, 96.0961, 0.010394, 3.7e-005, 0.95815, 0.004448, 6.7e-005, 0.01261, 0.024683,
+     11.442, 0.005349, 0.99012, 0.005352, 0.17323, 21.8749, 20.2219},
+    {-81.3238, 0.002841, 0.96596, 0.97791, 0.015254, 0.056799, 0.11328, 2.5207, 0.23954,
+     0.25736, 103.4612, 96.8014, 0.011385, 3.7e-005, 0.95449, 0.005413, 6.7e-005, 0.015284,
+     0.024706, 11.2566, 0.00537, 0.98962, 0.005373, 0.17158, 22.1689, 20.4268},
+    {-81.2657, 0.002868, 0.96553, 0.97752, 0.015406, 0.064019, 0.12504, 2.6684, 0.24235,
+     0.26131, 104.2505, 97.4054, 0.01227, 3.7e-005, 0.95055, 0.006458, 6.8e-005, 0.018158,
+     0.024723, 11.0791, 0.005391, 0.98902, 0.005394, 0.17018, 22.4276, 20.6034},
+    {-81.1929, 0.002903, 0.96497, 0.97694, 0.015625, 0.073994, 0.14059, 2.8407, 0.2456,
+     0.26605, 105.1459, 98.097, 0.013328, 3.7e-005, 0.94486, 0.00798, 6.8e-005, 0.022313,
+     0.024742, 10.8644, 0.005417, 0.98806, 0.005421, 0.16859, 22.7359, 20.8156},
+    {-81.1078, 0.002944, 0.96431, 0.97609, 0.01595, 0.087401, 0.1603, 3.0189, 0.24905,
+     0.27153, 106.0478, 98.7911, 0.014455, 3.8e-005, 0.93663, 0.010192, 6.8e-005, 0.028302,
+     0.024766, 10.6216, 0.005448, 0.98648, 0.005452, 0.16682, 23.0895, 21.0607},
+    {-81.2626, 0.00287, 0.9655, 0.97756, 0.015387, 0.061634, 0.12278, 2.7185, 0.24183, 0.2588,
+     104.2029, 97.6338, 0.01267, 3.7e-005, 0.95198, 0.006047, 6.8e-005, 0.017057, 0.024804,
+     11.0783, 0.005392, 0.98941, 0.005395, 0.16991, 22.2542, 20.5438},
+    {-81.1873, 0.002905, 0.96493, 0.97701, 0.015593, 0.071141, 0.13799, 2.9031, 0.24508,
+     0.26334, 105.1225, 98.3761, 0.013821, 3.7e-005, 0.94672, 0.007447, 6.8e-005, 0.020885,
+     0.024838, 10.8496, 0.005419, 0.9886, 0.005422, 0.16858, 22.5504, 20.7522},
+    {-81.1085, 0.002943, 0.96432, 0.97631, 0.015852, 0.082505, 0.15518, 3.0765, 0.2482,
+     0.26807, 105.9653, 99.0544, 0.01493, 3.8e-005, 0.94008, 0.009226, 6.8e-005, 0.025709,
+     0.024873, 10.6197, 0.005448, 0.98743, 0.005451, 0.16728, 22.8568, 20.9674}};
+
+void or_mj_init(int type, double *s) {
+  const double *row = MJ_INIT[(type >= 0 && type < 9) ? type : 2];
+  for (int k = 0; k < OR_MJ_NSTATE; k++) s[k] = row[k];
+}
+
+double or_mj_xi(void) { return 1.55e-4 / 2.58e-8; } /* Mahajan.cc:164 */
+
+/* Exact exponential relaxation over h toward x_inf with time constant tau,
+ * written as a rate so that the caller's forward-Euler update lands on it
+ * (Mahajan.cc:271-273 and the other gates). */
+static double mj_relax(double xinf, double tau, double x, double h) {
+  return (xinf - (xinf - x) * exp(-h / tau) - x) / h;
+}
+
+/* Instantaneous buffering factor of calmodulin, SR, membrane and
+ * sarcolemmal sites (Mahajan.cc:540-555). */
+static double mj_buffer(double c) {
+  const double b_cal = 24.0 * 7.0 / ((7.0 + c) * (7.0 + c));
+  const double b_sr = 47.0 * 0.6 / ((0.6 + c) * (0.6 + c));
+  const double b_mem = 15.0 * 0.3 / ((0.3 + c) * (0.3 + c));
+  const double b_sar = 42.0 * 13.0 / ((13.0 + c) * (13.0 + c));
+  return 1.0 / (1.0 + b_cal + b_sr + b_mem + b_sar);
+}
+
+/* ucla_rhsfun (Mahajan.cc:168-230) with the comp_* currents (:235-603):
+ * every rate of S at gate step h; istim in uA/uF.  Returns nothing; dV/dt is
+ * R[0]. */
+void or_mj_rates(const double *c, const double *S, double h, double istim, double *R) {
+  const double V = S[MJ_V], frt = c[24];
+  const double nao = c[0], ki = c[1], ko = c[2], cao = c[3];
+  const double nai = S[MJ_NAI], cs = S[MJ_CS], ci = S[MJ_CI], cj = S[MJ_CJ];
+
+  /* I_K1, reversal from P_K : P_Na = 100 : 1 (:331-347) */
+  const double ek1 = log((100.0 * ko + nao) / (100.0 * ki + nai)) / frt;
+  const double a_k1 = 1.02 / (1.0 + exp(0.2385 * (V - ek1 - 59.215)));
+  const double b_k1 = (0.49124 * exp(0.08032 * (V - ek1 + 5.476)) +
+                       exp(0.061750 * (V - ek1 - 594.31))) /
+                      (1.0 + exp(-0.5143 * (V - ek1 + 4.753)));
+  const double i_k1 = c[11] * sqrt(ko / 5.4) * a_k1 / (a_k1 + b_k1) * (V - ek1);
+
+  /* I_to, slow and fast (:352-385) */
+  const double ek = log(ko / ki) / frt;
+  const double x_inf = 1.0 / (1.0 + exp(-(V + 3.0) / 15.0));
+  const double y_inf = 1.0 / (1.0 + exp((V + 33.5) / 10.0));
+  const double tau_xs = 9.0 / (1.0 + exp((V + 3.0) / 15.0)) + 0.5;
+  const double tau_ys = 3000.0 / (1.0 + exp((V + 60.0) / 10.0)) + 30.0;
+  const double i_tos = c[6] * S[MJ_XTOS] * (S[MJ_YTOS] + 0.5 * y_inf) * (V - ek);
+  R[MJ_XTOS] = mj_relax(x_inf, tau_xs, S[MJ_XTOS], h);
+  R[MJ_YTOS] = mj_relax(y_inf, tau_ys, S[MJ_YTOS], h);
+  const double tau_xf = 3.5 * exp(-(V / 30.0) * (V / 30.0)) + 1.5;
+  const double tau_yf = 20.0 / (1.0 + exp((V + 33.5) / 10.0)) + 20.0;
+  const double i_tof = c[7] * S[MJ_XTOF] * S[MJ_YTOF] * (V - ek);
+  R[MJ_XTOF] = mj_relax(x_inf, tau_xf, S[MJ_XTOF], h);
+  R[MJ_YTOF] = mj_relax(y_inf, tau_yf, S[MJ_YTOF], h);
+
+  /* I_NaK (:390-398) */
+  const double sigma = (exp(nao / 67.3) - 1.0) / 7.0;
+  const double f_nak = 1.0 / (1.0 + 0.1245 * exp(-0.1 * V * frt) + 0.0365 * sigma * exp(-V * frt));
+  const double i_nak = c[12] * f_nak / (1.0 + 12.0 / nai) * ko / (ko + 1.5);
+
+  /* Na/Ca exchange flux (uM/ms), concentrations in mM (:403-425) */
+  const double csm = cs / 1000.0;
+  const double nai3 = nai * nai * nai, nao3 = nao * nao * nao;
+  const double e_in = exp(0.35 * V * frt), e_out = exp((0.35 - 1.0) * V * frt);
+  const double zw3 = nai3 * cao * e_in - nao3 * csm * e_out;
+  const double zw4 = 1.0 + 0.2 * e_out;
+  const double r_loss = 0.3 / cs;
+  const double aloss = 1.0 / (1.0 + r_loss * r_loss * r_loss);
+  const double yz1 = 1.3 * nai3 + 87.5 * 87.5 * 87.5 * csm;
+  const double yz2 = 12.3 * 12.3 * 12.3 * cao * (1.0 + csm / 0.0036);
+  const double rn = nai / 12.3;
+  const double yz3 = 0.0036 * nao3 * (1.0 + rn * rn * rn);
+  const double yz4 = nai3 * cao + nao3 * csm;
+  const double j_naca = c[8] * aloss * zw3 / (zw4 * (yz1 + yz2 + yz3 + yz4));
+
+  /* Ca cycling (:180-200) */
+  const double j_d = c[28] * (cs - ci) / c[14];
+  R[MJ_TROPI] = c[29] * (0.0327 * ci * (70.0 - S[MJ_TROPI]) - 0.0196 * S[MJ_TROPI]);
+  R[MJ_TROPS] = 0.0327 * cs * (70.0 - S[MJ_TROPS]) - 0.0196 * S[MJ_TROPS];
+  const double j_up = c[30] * c[13] * ci * ci / (ci * ci + 0.25);
+  const double j_leak = c[27] * 2.069e-5 * cj * cj / (cj * cj + 2500.0) * (16.667 * cj - ci);
+
+  /* L-type Ca channel, 7-state Markov model (:451-516) */
+  const double po_inf = 1.0 / (1.0 + exp(-V / 8.0));
+  const double alpha = po_inf, beta = 1.0 - po_inf;
+  const double rc = 3.0 / S[MJ_CP];
+  const double fca = 1.0 / (1.0 + rc * rc * rc);
+  const double s1 = 0.0182688 * fca, s1t = 0.00195;
+  const double k1 = 0.024168 * fca, k2 = 1.03615e-4, k1t = 0.00413, k2t = 0.00224;
+  const double r1 = 0.30, r2 = 3.0;
+  const double s2 = s1 * (r1 / r2) * (k2 / k1);
+  const double s2t = s1t * (r1 / r2) * (k2t / k1t);
+  const double poi = 1.0 / (1.0 + exp(-(V + 40.0) / 3.0));
+  const double k3 = (1.0 - poi) / 3.0, k3t = k3;
+  const double Pr = 1.0 - 1.0 / (1.0 + exp(-(V + 40.0) / 4.0));
+  const double recov = 10.0 + 4954.0 * exp(V / 15.6);
+  const double rcp = S[MJ_CP] / 6.09365;
+  const double tau_ca0 = 78.0329 / (1.0 + rcp * rcp * rcp * rcp) + 0.1;
+  const double tau_ca = (recov - tau_ca0) * Pr + tau_ca0;
+  const double tau_ba = (recov - 450.0) * Pr + 450.0;
+  const double Ps = 1.0 / (1.0 + exp(-(V + 40.0) / 11.32));
+  const double k6 = fca * Ps / tau_ca, k5 = (1.0 - Ps) / tau_ca;
+  const double k6t = Ps / tau_ba, k5t = (1.0 - Ps) / tau_ba;
+  const double k4 = k3 * (alpha / beta) * (k1 / k2) * (k5 / k6);
+  const double k4t = k3t * (alpha / beta) * (k1t / k2t) * (k5t / k6t);
+  const double c1 = S[MJ_C1], c2 = S[MJ_C2];
+  const double i1c = S[MJ_XI1CA], i1b = S[MJ_XI1BA], i2c = S[MJ_XI2CA], i2b = S[MJ_XI2BA];
+  const double po = 1.0 - i1c - i2c - i1b - i2b - c1 - c2;
+  R[MJ_C2] = beta * c1 + k5 * i2c + k5t * i2b - (k6 + k6t + alpha) * c2;
+  R[MJ_C1] = alpha * c2 + k2 * i1c + k2t * i1b + r2 * po - (beta + r1 + k1t + k1) * c1;
+  R[MJ_XI1CA] = k1 * c1 + k4 * i2c + s1 * po - (k3 + k2 + s2) * i1c;
+  R[MJ_XI2CA] = k3 * i1c + k6 * c2 - (k5 + k4) * i2c;
+  R[MJ_XI1BA] = k1t * c1 + k4t * i2b + s1t * po - (k3t + k2t + s2t) * i1b;
+  R[MJ_XI2BA] = k3t * i1b + k6t * c2 - (k5t + k4t) * i2b;
+
+  /* Ca driving force (:430-447) */
+  const double za = 2.0 * V * frt;
+  const double f1 = 4.0 * 0.00054 * c[23] * c[23] / (c[22] * c[20]);
+  double rxa;
+  if (fabs(za) < 0.001) rxa = f1 * (csm * exp(za) - 0.341 * cao) / (2.0 * frt);
+  else rxa = V * f1 * (csm * exp(za) - 0.341 * cao) / (exp(za) - 1.0);
+  const double j_ca = c[4] * po * rxa;
+
+  R[MJ_CS] = mj_buffer(cs) * (50.0 * (S[MJ_IR] - j_d - j_ca + j_naca) - R[MJ_TROPS]);
+  R[MJ_CI] = mj_buffer(ci) * (j_d - j_up + j_leak - R[MJ_TROPI]);
+  R[MJ_CJ] = -S[MJ_IR] + j_up - j_leak;
+  R[MJ_CJP] = (cj - S[MJ_CJP]) / c[17];
+
+  /* SR release: load dependence (:560-578), sparks (:582-589), dyad (:594-603) */
+  double q = 0.0;
+  const double cjp = S[MJ_CJP];
+  if (cjp > c[26] && cjp < c[19]) q = cjp - c[26];
+  else if (cjp >= c[19]) q = c[18] * cjp + (c[19] - c[26]) - c[18] * c[19];
+  if (q >= c[25]) q = c[25];
+  const double Qr = cj * q / c[19];
+  const double ey = exp(-0.05 * (V + 30.0));
+  const double spark = 2.58079 * po * fabs(rxa) * ey / (1.0 + ey);
+  R[MJ_IR] = spark * Qr - S[MJ_IR] * (1.0 - c[16] * R[MJ_CJ] / cj) / c[16];
+  const double ex = exp(-0.3576 * (V + 30.0));
+  const double g_sr = 26841.8 * ex / (1.0 + ex);
+  R[MJ_CP] = po * Qr * fabs(rxa) * g_sr + po * 9000.0 * fabs(rxa) - (S[MJ_CP] - cs) / 0.5;
+
+  /* I_Na, Hund-Rudy/Livshitz gates (:235-287) */
+  const double ena = log(nao / nai) / frt;
+  double am = 3.2;
+  if (fabs(V + 47.13) > 0.001) am = 0.32 * (V + 47.13) / (1.0 - exp(-0.1 * (V + 47.13)));
+  const double bm = 0.08 * exp(-V / 11.0);
+  const double a = 1.0 - 1.0 / (1.0 + exp(-(V + 40.0) / 0.24));
+  const double ah = a * 0.135 * exp((80.0 + V) / (-6.8));
+  const double bh = (1.0 - a) / (0.13 * (1.0 + exp((V + 10.66) / (-11.1)))) +
+                    a * (3.56 * exp(0.079 * V) + 3.1e5 * exp(0.35 * V));
+  const double aj = a * (-1.2714e5 * exp(0.2444 * V) - 3.474e-5 * exp(-0.04391 * V)) *
+                    (V + 37.78) / (1.0 + exp(0.311 * (V + 79.23)));
+  const double bj = (1.0 - a) * (0.3 * exp(-2.535e-7 * V) / (1.0 + exp(-0.1 * (V + 32.0)))) +
+                    a * (0.1212 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14))));
+  const double tau_h = 1.0 / (ah + bh), tau_j = c[34] / (aj + bj), tau_m = 1.0 / (am + bm);
+  const double mg = (S[MJ_M] + c[31]) / (1.0 + c[31]);
+  const double i_na = c[15] * (S[MJ_H] + c[32]) / (1.0 + c[32]) * (S[MJ_J] + c[33]) / (1.0 + c[33]) *
+                      mg * mg * mg * (V - ena);
+  R[MJ_H] = mj_relax(ah * tau_h, tau_h, S[MJ_H], h);
+  R[MJ_J] = mj_relax(aj * tau_j, tau_j, S[MJ_J], h);
+  R[MJ_M] = mj_relax(am * tau_m, tau_m, S[MJ_M], h);
+
+  /* I_Kr (:292-313) */
+  double kr1 = 0.00138 / 0.123, kr2 = 0.00061 / 0.145;
+  if (fabs(V + 7.0) > 0.001) kr1 = 0.00138 * (V + 7.0) / (1.0 - exp(-0.123 * (V + 7.0)));
+  if (fabs(V + 10.0) > 0.001) kr2 = 0.00061 * (V + 10.0) / (exp(0.145 * (V + 10.0)) - 1.0);
+  const double xr_inf = 1.0 / (1.0 + exp(-(V + 50.0) / 7.5));
+  const double rg = 1.0 / (1.0 + exp((V + 33.0) / 22.4));
+  const double i_kr = c[9] * sqrt(ko / 5.4) * S[MJ_XR] * rg * (V - ek);
+  R[MJ_XR] = mj_relax(xr_inf, 1.0 / (kr1 + kr2), S[MJ_XR], h);
+
+  /* I_Ks with Ca dependence (:318-345) */
+  const double eks = log((ko + 0.018330 * nao) / (ki + 0.018330 * nai)) / frt;
+  const double xs_inf = 1.0 / (1.0 + exp(-(V - 1.50) / 16.70));
+  double tau_xs1;
+  if (fabs(V + 30.0) < 0.001 / 0.0687)
+    tau_xs1 = 1.0 / (0.0000719 / 0.148 + 0.000131 / 0.0687);
+  else
+    tau_xs1 = 1.0 / (0.0000719 * (V + 30.0) / (1.0 - exp(-0.148 * (V + 30.0))) +
+                     0.000131 * (V + 30.0) / (exp(0.0687 * (V + 30.0)) - 1.0));
+  const double rks = 0.5 / ci;
+  const double g_ksx = 1.0 + 0.8 / (1.0 + rks * rks * rks);
+  const double i_ks = c[10] * g_ksx * S[MJ_XS1] * S[MJ_XS2] * (V - eks);
+  R[MJ_XS1] = mj_relax(xs_inf, tau_xs1, S[MJ_XS1], h);
+  R[MJ_XS2] = mj_relax(xs_inf, 4.0 * tau_xs1, S[MJ_XS2], h);
+
+  /* Na_i and V (:205-229); wca converts uM/ms to uA/uF */
+  const double wca = 8.0313;
+  R[MJ_NAI] = -(1.0 / wca / 1000.0) * (i_na + 3.0 * i_nak + 3.0 * wca * j_naca);
+  R[MJ_V] = -(i_na + i_k1 + i_kr + i_ks + i_tos + i_tof + wca * j_naca + 2.0 * wca * j_ca + i_nak) +
+            istim;
+}
+
+/* Mahajan::Compute_Ion + UpdateStateVariables (Mahajan.cc:607-644): forward
+ * Euler over dt, or 10 sub-steps of dt/10 when |dV/dt| > 20 (int_abs: the
+ * threshold as abs() of an int reads it, |dV/dt| >= 21); returns the last
+ * dV/dt evaluated. */
+double or_mj_compute_ion(const double *c, double *S, double Cm, double dt, double V,
+                         double istim, int int_abs) {
+  double R[OR_MJ_NSTATE];
+  S[MJ_V] = V;
+  const double ist = istim / (Cm * or_mj_xi());
+  or_mj_rates(c, S, dt, ist, R);
+  const int adapt = int_abs ? (fabs(trunc(R[MJ_V])) > 20.0) : (fabs(R[MJ_V]) > 20.0);
+  if (adapt) {
+    const double h = dt / 10.0;
+    for (int q = 0; q < 10; q++) {
+      or_mj_rates(c, S, h, ist, R);
+      for (int k = 0; k < OR_MJ_NSTATE; k++) S[k] += R[k] * h;
+    }
+  } else {
+    for (int k = 0; k < OR_MJ_NSTATE; k++) S[k] += R[k] * dt;
+  }
+  return R[MJ_V];
+}
