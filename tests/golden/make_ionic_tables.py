@@ -3,9 +3,11 @@
 tests/golden/ionic_tables.json.
 
 Reads, as text, voomEP/src/IonicModel/{SetUpIonicConstants.h, Tusscher.h,
-LR.cc} under /root/reference and records only numbers: the ten Tusscher and
-Purkinje constant arrays, the ten Tusscher initial state (the active block,
-not the commented-out one) and the Luo-Rudy initial table and constants.
+LR.cc, Mahajan.cc} under /root/reference and records only numbers: the ten
+Tusscher and Purkinje constant arrays, the ten Tusscher initial state (the
+active block, not the commented-out one), the Luo-Rudy initial table and
+constants, and the Mahajan constants, per-tissue-type G_tof / G_Ks and
+per-type initial states.
 Nothing of the reference is executed.  Run from the repository root:
 
     python tests/golden/make_ionic_tables.py
@@ -48,6 +50,38 @@ def indexed_literals(body, var):
     return [vals.get(i) for i in range(n)]
 
 
+def product(expr):
+    """Value of a literal product such as 1.9*0.32 or 0.11*.85."""
+    v = 1.0
+    for f in expr.split("*"):
+        v *= float(f)
+    return v
+
+
+def mahajan_tables(const_h, mj_cc):
+    body = function_body(const_h, "void SetUpMahajanParameters")
+    consts = indexed_literals(body, "Constants")
+    per_type = {}
+    for m in re.finditer(r"case\s+(\d+)\s*:\s*\{([^}]*)\}", body):
+        vals = {}
+        for a in re.finditer(r"Constants\[\s*(\d+)\s*\]\s*=\s*([0-9.eE*+-]+)\s*;", m.group(2)):
+            vals[int(a.group(1))] = product(a.group(2))
+        per_type[int(m.group(1))] = [vals[7], vals[10]]
+    ctor = function_body(mj_cc, "Mahajan::Mahajan(Real *Constants, int pos)")
+    init = {}
+    for m in re.finditer(r"case\s*\(\s*(\d+)\s*\)\s*:\s*\{([^}]*)\}", ctor):
+        init[int(m.group(1))] = indexed_literals(m.group(2), "_State")
+    d = re.search(r"default\s*:\s*\{([^}]*)\}", ctor)
+    xi = re.search(r"_Xi\s*=\s*" + NUM + r"\s*/\s*" + NUM, mj_cc)
+    return {
+        "mahajan_constants_literal": consts,
+        "mahajan_gtof_gks_by_type": [per_type[t] for t in range(9)],
+        "mahajan_initial_states": [init[t] for t in range(9)],
+        "mahajan_initial_state_default": indexed_literals(d.group(1), "_State"),
+        "mahajan_xi": float(xi.group(1)) / float(xi.group(2)),
+    }
+
+
 def main():
     const_h = strip_comments(open(os.path.join(REF, "SetUpIonicConstants.h")).read())
     tt = indexed_literals(function_body(const_h, "void SetUpTusscherParameters"), "Constants")
@@ -62,6 +96,7 @@ def main():
         m = re.search(r"\b" + name + r"\s*=\s*" + NUM + r"\s*;", lr_body)
         lr_consts[name] = float(m.group(1))
     xi = re.search(r"_Xi\s*=\s*" + NUM, tth)
+    mj_cc = strip_comments(open(os.path.join(REF, "Mahajan.cc")).read())
     out = {
         "source": "voomEP/src/IonicModel (SetUpIonicConstants.h, Tusscher.h, LR.cc), numbers only",
         "tusscher_constants": tt,
@@ -71,6 +106,7 @@ def main():
         "luorudy_initial_state": lr_init,
         "luorudy_constants": lr_consts,
     }
+    out.update(mahajan_tables(const_h, mj_cc))
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     print(f"wrote {OUT}: {len(tt)} tT constants, {len(tt_init)} tT states, "
