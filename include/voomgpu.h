@@ -49,6 +49,9 @@ extern "C" {
  * (CardiacBody.cc:988-1093). */
 #define VG_MODEL_LR 0
 #define VG_MODEL_TT 2
+/* Mahajan 2008 rabbit ventricle: each node's constants and initial state by
+ * its tissue type 0-8 (vg_mesh.tissue; epi/myo/endo x apex/center/base). */
+#define VG_MODEL_MJ 3
 /* Corrias rabbit Purkinje (IonicModel/Purkinje.cc); in a tissue the model of
  * every Purkinje node (tissue 11/12), here also selectable for vg_cells_*. */
 #define VG_MODEL_PURKINJE 10
@@ -56,7 +59,8 @@ extern "C" {
 /* Arithmetic of the hot path. */
 #define VG_PREC_FP64 0  /* every array and operation in FP64 */
 #define VG_PREC_MIXED 1 /* FP32 states, matrices and vectors; FP64 CG dots,
-                           residual norms and ECG sums */
+                           residual norms and ECG sums (Mahajan: Ca-cycling
+                           and Na_i states accumulated in FP64) */
 
 /* Ionic integrators. */
 #define VG_INTEG_REFERENCE 0   /* the reference's scheme: LR Rush-Larsen + FE;
@@ -77,6 +81,9 @@ extern "C" {
                                      step with J^T like the reference
                                      (Tusscher.cc:172-181); leaves the physical
                                      range during an upstroke at dt = 0.02 */
+#define VG_QUIRK_MJ_INT_ABS 16u   /* Mahajan: read the sub-step test
+                                     abs(dV/dt) > 20 (Mahajan.cc:611) through
+                                     int abs(), i.e. |dV/dt| >= 21 */
 
 /* Per-node conductance scalings of the ten Tusscher model (vg_model). */
 #define VG_TT_SCALE_GNA 0

@@ -293,7 +293,9 @@ class Simulation:
                  tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None,
                  newton_jt=False, mj_const=None, mj_int_abs=False):
         """Mahajan (model 3) takes each node's tissue type 0-8 from
-        ``param_idx`` (default 2) and its initial state from that type's row."""
+        ``param_idx`` (default 0, Mesh.cc:186,375; types beyond 8 run the
+        reference's fallback, type 2) and its initial state from that type's
+        row."""
         self.csr = csr if csr is not None else assemble(mesh, D=D, Dp=Dp, DpLeft=DpLeft, Cm=Cm,
                                                         pmj_sticky=pmj_sticky)
         n = self.csr.n
@@ -302,7 +304,7 @@ class Simulation:
         nst = {MODEL_LR: LR_NSTATE, MODEL_TT: TT_NSTATE, MODEL_MJ: MJ_NSTATE}[model]
         if states is None:
             if model == MODEL_MJ:
-                types = (np.full(n, 2) if param_idx is None
+                types = (np.zeros(n, int) if param_idx is None
                          else np.asarray(param_idx).astype(int))
                 table = np.array([mj_init(t) for t in range(9)])
                 states = table[np.where(types < 9, types, 2)]

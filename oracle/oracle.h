@@ -140,7 +140,7 @@ typedef struct {
   double *states;       /* [n*nstate] AoS like getInternalParameters */
   double *istim;        /* [n] uA/cm^3 */
   const double *tt_const; /* [n_param*53] or NULL -> reference constants */
-  const uint16_t *param_idx; /* [n] or NULL; Mahajan: tissue type 0-8 */
+  const uint16_t *param_idx; /* [n] or NULL; Mahajan: tissue type 0-8 (NULL: 0) */
   /* work */
   double *A, *B, *rhs, *r, *z, *p, *q, *diag;
   int64_t cg_iters_total;

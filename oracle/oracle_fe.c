@@ -413,7 +413,7 @@ int or_cn_step(or_sim *s) {
 #pragma omp parallel for schedule(dynamic, 256)
       for (int64_t i = 0; i < n; i++) {
         if (pkf && pkf[i]) continue;
-        const int t = s->param_idx ? s->param_idx[i] : 2;
+        const int t = s->param_idx ? s->param_idx[i] : 0; /* Mesh.cc:186,375 */
         const double dv = or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 2),
                                             s->states + OR_MJ_NSTATE * i, s->Cm, dt, s->V[i],
                                             s->istim[i], s->mj_int_abs);

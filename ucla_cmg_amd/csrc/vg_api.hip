@@ -53,12 +53,26 @@ struct ProfileClass {
 // (kept on the device as the voltage vector instead, -1 if the model keeps its
 // own V copy) and the number of state rows stored on the device.
 int nstate_of(int model) {
-  return model == VG_MODEL_LR ? LR_NSTATE : model == VG_MODEL_PURKINJE ? PK_NSTATE : TT_NSTATE;
+  return model == VG_MODEL_LR ? LR_NSTATE
+         : model == VG_MODEL_PURKINJE ? PK_NSTATE
+         : model == VG_MODEL_MJ ? MJ_NSTATE
+                                 : TT_NSTATE;
 }
 int vslot_of(int model) { return model == VG_MODEL_LR ? 7 : model == VG_MODEL_PURKINJE ? -1 : 0; }
+// Mahajan tissue type of a mesh node: the type of the .cell data, 0 without
+// one (Mesh/Mesh.cc:186,375); types beyond 8 take the reference's fallback,
+// whose constants and initial state are those of type 2
+// (SetUpIonicConstants.h:219-223, Mahajan.cc:147-158).
+int mj_type(const int8_t *tissue, int64_t i) {
+  const int t = tissue ? tissue[i] : 0;
+  return (t >= 0 && t < 9) ? t : 2;
+}
 int ndev_of(int model) { return nstate_of(model) - (vslot_of(model) >= 0 ? 1 : 0); }
 const double *init_table(int model) {
-  return model == VG_MODEL_LR ? LR_INIT : model == VG_MODEL_PURKINJE ? PK_INIT : TT_INIT;
+  return model == VG_MODEL_LR      ? LR_INIT
+         : model == VG_MODEL_PURKINJE ? PK_INIT
+         : model == VG_MODEL_MJ       ? MJ_INIT[0]
+                                      : TT_INIT;
 }
 
 // Ionic sub-step sequence of EP_CrankNicholson::Solve (:114-143).
@@ -97,6 +111,7 @@ struct vg_handle {
   // device
   DevBuf off, width, wide, col16, col32, Ahat, Krhs, sinv, diag;
   DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
+  DevBuf acc;  // Mahajan: FP64 Ca-cycling and Na_i states [MJ_NACC][n_alloc]
   DevBuf r2, w2, s2;  // ping-pong partners of r, w, s for the fused CG
   // Increment of the second half-step (x is the first's) unless both
   // half-steps share x (cg_x0 = VG_X0_LAST); with guess = 1 each solve starts
@@ -158,9 +173,9 @@ struct vg_handle {
 
 struct vg_cells {
   int prec, model, integ;
-  int newton_jt = 0;
+  int newton_jt = 0, mj_int_abs = 0;
   int64_t n, n_alloc;
-  DevBuf states, scales, pidx, V, stim, dvdt;
+  DevBuf states, acc, scales, pidx, V, stim, dvdt;
   int nsets = 1;
 };
 
@@ -297,9 +312,11 @@ int cg_solve(vg_handle *h, const T *V, int which) {
 
 template <class T, int MODEL, int INTEG>
 void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
-  IonicArgs a{h->n_myo, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0};
+  IonicArgs a{h->n_myo, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0,
+              (h->quirks & VG_QUIRK_MJ_INT_ABS) ? 1 : 0};
   hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3((unsigned)((h->n_myo + BLK - 1) / BLK)),
                      dim3(BLK), 0, h->stream, Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
+                     h->acc.p ? h->acc.as<double>() : (double *)nullptr,
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
                      h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
                      h->scales.as<T>(), h->sub, a, h->newton.as<unsigned long long>(),
@@ -327,6 +344,8 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   }
   if (h->n_myo > 0) {
     if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+    else if (h->model == VG_MODEL_MJ)
+      launch_ionic_t<T, VG_MODEL_MJ, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else if (h->integ == VG_INTEG_REFERENCE)
       launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else launch_ionic_t<T, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN>(h, Vin, x, Vout);
@@ -493,7 +512,20 @@ int set_states_impl(vg_handle *h, const int64_t *offsets, const double *data) {
   if (h->n_pk > 0)
     for (int k = 0; k < PK_NSTATE; k++)
       for (int64_t r = h->n_pk; r < npa; r++) pk[(size_t)k * npa + r] = pk[(size_t)k * npa];
-  int rc = h->tsz == 8 ? put_soa<double>(h, h->states, soa) : put_soa<float>(h, h->states, soa);
+  int rc;
+  if (h->model == VG_MODEL_MJ) {
+    // Rows in state order 1..25 -> gate rows (T) and accumulator rows (FP64).
+    std::vector<double> g((size_t)MJ_NGATE * na), a((size_t)MJ_NACC * na);
+    for (int k = 0; k < MJ_NGATE; k++)
+      std::copy_n(soa.begin() + (size_t)(MJ_GATE[k] - 1) * na, na, g.begin() + (size_t)k * na);
+    for (int k = 0; k < MJ_NACC; k++)
+      std::copy_n(soa.begin() + (size_t)(MJ_ACC[k] - 1) * na, na, a.begin() + (size_t)k * na);
+    rc = h->tsz == 8 ? put_soa<double>(h, h->states, g) : put_soa<float>(h, h->states, g);
+    if (rc) return rc;
+    rc = put_soa<double>(h, h->acc, a);
+  } else {
+    rc = h->tsz == 8 ? put_soa<double>(h, h->states, soa) : put_soa<float>(h, h->states, soa);
+  }
   if (rc) return rc;
   if (h->n_pk > 0) rc = h->tsz == 8 ? put_soa<double>(h, h->pk_states, pk) : put_soa<float>(h, h->pk_states, pk);
   return rc;
@@ -503,7 +535,19 @@ int get_states_impl(vg_handle *h, const int64_t *offsets, double *data, const do
   const int nd = ndev_of(h->model), vs = vslot_of(h->model);
   const int64_t na = h->n_alloc, npa = h->n_pk_alloc;
   std::vector<double> soa((size_t)nd * na), pk((size_t)PK_NSTATE * (h->n_pk > 0 ? npa : 0));
-  int rc = h->tsz == 8 ? get_soa<double>(h, h->states, soa) : get_soa<float>(h, h->states, soa);
+  int rc;
+  if (h->model == VG_MODEL_MJ) {
+    std::vector<double> g((size_t)MJ_NGATE * na), a((size_t)MJ_NACC * na);
+    rc = h->tsz == 8 ? get_soa<double>(h, h->states, g) : get_soa<float>(h, h->states, g);
+    if (rc) return rc;
+    if ((rc = get_soa<double>(h, h->acc, a))) return rc;
+    for (int k = 0; k < MJ_NGATE; k++)
+      std::copy_n(g.begin() + (size_t)k * na, na, soa.begin() + (size_t)(MJ_GATE[k] - 1) * na);
+    for (int k = 0; k < MJ_NACC; k++)
+      std::copy_n(a.begin() + (size_t)k * na, na, soa.begin() + (size_t)(MJ_ACC[k] - 1) * na);
+  } else {
+    rc = h->tsz == 8 ? get_soa<double>(h, h->states, soa) : get_soa<float>(h, h->states, soa);
+  }
   if (rc) return rc;
   rc = h->tsz == 8 ? get_soa<double>(h, h->pk_states, pk) : get_soa<float>(h, h->pk_states, pk);
   if (rc) return rc;
@@ -517,6 +561,61 @@ int get_states_impl(vg_handle *h, const int64_t *offsets, double *data, const do
       for (int k = 0; k < PK_NSTATE; k++) st[k] = pk[(size_t)k * npa + (r - h->n_myo)];
     }
   }
+  return VG_OK;
+}
+
+// Device rows of independent cells, rows in state order without the V slot;
+// Mahajan keeps its gate rows in T and its accumulator rows in FP64.
+int cells_put_rows(vg_cells *c, const std::vector<double> &soa) {
+  const int64_t na = c->n_alloc;
+  std::vector<double> g, a;
+  const std::vector<double> *rows = &soa;
+  if (c->model == VG_MODEL_MJ) {
+    g.resize((size_t)MJ_NGATE * na);
+    a.resize((size_t)MJ_NACC * na);
+    for (int k = 0; k < MJ_NGATE; k++)
+      std::copy_n(soa.begin() + (size_t)(MJ_GATE[k] - 1) * na, na, g.begin() + (size_t)k * na);
+    for (int k = 0; k < MJ_NACC; k++)
+      std::copy_n(soa.begin() + (size_t)(MJ_ACC[k] - 1) * na, na, a.begin() + (size_t)k * na);
+    if (hipMemcpy(c->acc.p, a.data(), a.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return VG_ERR_HIP;
+    rows = &g;
+  }
+  if (c->prec == VG_PREC_FP64) {
+    if (hipMemcpy(c->states.p, rows->data(), rows->size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(rows->begin(), rows->end());
+    if (hipMemcpy(c->states.p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return VG_ERR_HIP;
+  }
+  return VG_OK;
+}
+
+int cells_get_rows(vg_cells *c, std::vector<double> &soa) {
+  const int64_t na = c->n_alloc;
+  const size_t nrow = c->model == VG_MODEL_MJ ? (size_t)MJ_NGATE * na : soa.size();
+  std::vector<double> rows(nrow);
+  if (c->prec == VG_PREC_FP64) {
+    if (hipMemcpy(rows.data(), c->states.p, nrow * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      return VG_ERR_HIP;
+  } else {
+    std::vector<float> f(nrow);
+    if (hipMemcpy(f.data(), c->states.p, nrow * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return VG_ERR_HIP;
+    std::copy(f.begin(), f.end(), rows.begin());
+  }
+  if (c->model != VG_MODEL_MJ) {
+    soa = rows;
+    return VG_OK;
+  }
+  std::vector<double> a((size_t)MJ_NACC * na);
+  if (hipMemcpy(a.data(), c->acc.p, a.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return VG_ERR_HIP;
+  for (int k = 0; k < MJ_NGATE; k++)
+    std::copy_n(rows.begin() + (size_t)k * na, na, soa.begin() + (size_t)(MJ_GATE[k] - 1) * na);
+  for (int k = 0; k < MJ_NACC; k++)
+    std::copy_n(a.begin() + (size_t)k * na, na, soa.begin() + (size_t)(MJ_ACC[k] - 1) * na);
   return VG_OK;
 }
 
@@ -554,9 +653,15 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   *out = nullptr;
   std::unique_ptr<vg_handle> h(new vg_handle());
   if (!mesh || !model) return VG_ERR_ARG;
-  if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_TT) {
+  if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_TT &&
+      model->cell_model != VG_MODEL_MJ) {
     *out = h.release();
     return (*out)->fail(VG_ERR_UNSUPPORTED, "cell model not supported");
+  }
+  if (model->cell_model == VG_MODEL_MJ && model->n_param_sets > 0) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "Mahajan takes its constants from the tissue types, not "
+                                    "from parameter sets");
   }
   if (model->precision != VG_PREC_FP64 && model->precision != VG_PREC_MIXED) {
     *out = h.release();
@@ -686,7 +791,12 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     HIPCHK(H_, b->alloc(na * T));
     HIPCHK(H_, hipMemset(b->p, 0, na * T));
   }
-  HIPCHK(H_, H_->states.alloc((size_t)ndev_of(H_->model) * na * T));
+  if (H_->model == VG_MODEL_MJ) {
+    HIPCHK(H_, H_->states.alloc((size_t)MJ_NGATE * na * T));
+    HIPCHK(H_, H_->acc.alloc((size_t)MJ_NACC * na * 8));
+  } else {
+    HIPCHK(H_, H_->states.alloc((size_t)ndev_of(H_->model) * na * T));
+  }
   if (H_->n_pk > 0) {
     HIPCHK(H_, H_->pk_states.alloc((size_t)PK_NSTATE * H_->n_pk_alloc * T));
     HIPCHK(H_, hipStreamCreateWithFlags(&H_->stream_pk, hipStreamNonBlocking));
@@ -704,8 +814,26 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, H_->newton.alloc(sizeof(unsigned long long)));
   HIPCHK(H_, hipMemset(H_->newton.p, 0, sizeof(unsigned long long)));
   HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult), hipHostMallocDefault));
-  // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi).
-  {
+  // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi;
+  // Mahajan: one set per tissue type holding G_tof and G_Ks).
+  if (H_->model == VG_MODEL_MJ) {
+    std::vector<double> sc((size_t)9 * VG_TT_NSCALE, 0.0);
+    for (int t = 0; t < 9; t++) {
+      sc[(size_t)t * VG_TT_NSCALE + 0] = MJ_GTOF[t];
+      sc[(size_t)t * VG_TT_NSCALE + 1] = MJ_GKS[t];
+    }
+    H_->nsets = 9;
+    if (T == 8) {
+      if ((rc = upload_raw<double>(H_, H_->scales, sc))) return rc;
+    } else {
+      if ((rc = upload_raw<float>(H_, H_->scales, sc))) return rc;
+    }
+    std::vector<uint16_t> pi(na, 0);
+    for (int64_t r = 0; r < H_->n; r++) pi[r] = (uint16_t)mj_type(mesh->tissue, H_->new2old[r]);
+    for (int64_t r = H_->n; r < na; r++) pi[r] = pi[0];
+    HIPCHK(H_, H_->pidx.alloc(na * 2));
+    HIPCHK(H_, hipMemcpy(H_->pidx.p, pi.data(), na * 2, hipMemcpyHostToDevice));
+  } else {
     int nsets = model->n_param_sets > 0 ? model->n_param_sets : 1;
     std::vector<double> sc((size_t)nsets * VG_TT_NSCALE, 1.0);
     if (model->n_param_sets > 0) {
@@ -746,6 +874,9 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
         one[7] = LR_REST_V;
         v0 = LR_REST_V;
       }
+    } else if (H_->model == VG_MODEL_MJ) {
+      for (int k = 0; k < ns; k++) one[k] = MJ_INIT[0][k];
+      v0 = MJ_INIT[0][0];
     } else {
       for (int k = 0; k < ns; k++) one[k] = TT_INIT[k];
       v0 = (H_->quirks & VG_QUIRK_V0_ZERO) ? 0.0 : TT_INIT[0];
@@ -757,6 +888,12 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
       if (H_->purkinje[i]) {
         std::copy(PK_INIT, PK_INIT + PK_NSTATE, st.begin() + off[i]);
         V0[i] = PK_INIT[0];
+      } else if (H_->model == VG_MODEL_MJ) {
+        // Mahajan starts from its type's table with V = its state 0 (the
+        // driver's getInitialVoltage, CardiacBody.h:299-306).
+        const double *row = MJ_INIT[mj_type(mesh->tissue, i)];
+        std::copy(row, row + MJ_NSTATE, st.begin() + off[i]);
+        V0[i] = row[0];
       } else {
         std::copy(one.begin(), one.end(), st.begin() + off[i]);
         V0[i] = v0;
@@ -1029,7 +1166,8 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
 // ---- independent cells -------------------------------------------------------
 int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   if (!out || n < 1) return VG_ERR_ARG;
-  if (model != VG_MODEL_LR && model != VG_MODEL_TT && model != VG_MODEL_PURKINJE)
+  if (model != VG_MODEL_LR && model != VG_MODEL_TT && model != VG_MODEL_PURKINJE &&
+      model != VG_MODEL_MJ)
     return VG_ERR_UNSUPPORTED;
   if (prec != VG_PREC_FP64 && prec != VG_PREC_MIXED) return VG_ERR_ARG;
   std::unique_ptr<vg_cells> c(new vg_cells());
@@ -1040,11 +1178,22 @@ int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   c->n_alloc = ((n + BLK - 1) / BLK) * BLK;
   const size_t T = prec == VG_PREC_FP64 ? 8 : 4;
   const int ns = nstate_of(model);
-  if (c->states.alloc((size_t)ndev_of(model) * c->n_alloc * T) != hipSuccess) return VG_ERR_HIP;
+  if (model == VG_MODEL_MJ) {
+    if (c->states.alloc((size_t)MJ_NGATE * c->n_alloc * T) != hipSuccess) return VG_ERR_HIP;
+    if (c->acc.alloc((size_t)MJ_NACC * c->n_alloc * 8) != hipSuccess) return VG_ERR_HIP;
+  } else if (c->states.alloc((size_t)ndev_of(model) * c->n_alloc * T) != hipSuccess) {
+    return VG_ERR_HIP;
+  }
   if (c->V.alloc(n * 8) != hipSuccess || c->stim.alloc(n * 8) != hipSuccess ||
       c->dvdt.alloc(n * 8) != hipSuccess)
     return VG_ERR_HIP;
-  std::vector<double> one(VG_TT_NSCALE, 1.0);
+  // Mahajan: one scaling set per tissue type carrying G_tof, G_Ks.
+  std::vector<double> one(model == VG_MODEL_MJ ? 9 * VG_TT_NSCALE : VG_TT_NSCALE, 1.0);
+  if (model == VG_MODEL_MJ)
+    for (int t = 0; t < 9; t++) {
+      one[(size_t)t * VG_TT_NSCALE + 0] = MJ_GTOF[t];
+      one[(size_t)t * VG_TT_NSCALE + 1] = MJ_GKS[t];
+    }
   if (T == 8) {
     if (c->scales.alloc(one.size() * 8) != hipSuccess) return VG_ERR_HIP;
     if (hipMemcpy(c->scales.p, one.data(), one.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
@@ -1066,11 +1215,21 @@ void vg_cells_destroy(vg_cells *c) { delete c; }
 int vg_cells_set_quirks(vg_cells *c, unsigned quirks) {
   if (!c) return VG_ERR_ARG;
   c->newton_jt = (quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0;
+  c->mj_int_abs = (quirks & VG_QUIRK_MJ_INT_ABS) ? 1 : 0;
   return VG_OK;
 }
 
 int vg_cells_set_scales(vg_cells *c, int nsets, const double *sc, const uint16_t *pidx) {
-  if (!c || nsets < 1 || !sc) return VG_ERR_ARG;
+  if (!c) return VG_ERR_ARG;
+  if (c->model == VG_MODEL_MJ) {  // pidx = tissue types; no scaling sets
+    if (nsets != 0 || sc || !pidx) return VG_ERR_ARG;
+    std::vector<uint16_t> t(pidx, pidx + c->n);
+    for (auto &v : t) v = (uint16_t)(v < 9 ? v : 2);
+    if (!c->pidx.p && c->pidx.alloc(c->n * 2) != hipSuccess) return VG_ERR_HIP;
+    if (hipMemcpy(c->pidx.p, t.data(), c->n * 2, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+    return VG_OK;
+  }
+  if (nsets < 1 || !sc) return VG_ERR_ARG;
   const size_t T = c->prec == VG_PREC_FP64 ? 8 : 4;
   std::vector<double> v(sc, sc + (size_t)nsets * VG_TT_NSCALE);
   if (c->scales.p) {
@@ -1106,16 +1265,19 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, con
   const double *sp = istim ? c->stim.as<double>() : nullptr;
 #define VG_CELLS(T, M, I)                                                                      \
   hipLaunchKernelGGL((k_cells<T, M, I>), dim3(nb), dim3(BLK), 0, 0, c->V.as<double>(), sp,      \
-                     c->dvdt.as<double>(), c->states.as<T>(), pi, c->scales.as<T>(), n,         \
-                     c->n_alloc, Cm, dt, 500, c->newton_jt)
+                     c->dvdt.as<double>(), c->states.as<T>(),                                   \
+                     c->acc.p ? c->acc.as<double>() : (double *)nullptr, pi, c->scales.as<T>(), \
+                     n, c->n_alloc, Cm, dt, 500, c->newton_jt, c->mj_int_abs)
   if (c->prec == VG_PREC_FP64) {
     if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(double, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_MJ) VG_CELLS(double, VG_MODEL_MJ, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(double, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(double, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
   } else {
     if (c->model == VG_MODEL_LR) VG_CELLS(float, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(float, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_MJ) VG_CELLS(float, VG_MODEL_MJ, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(float, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(float, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
   }
@@ -1133,13 +1295,7 @@ int vg_cells_get_states(vg_cells *c, int *nvar, double *st) {
   const int vslot = vslot_of(c->model);
   const int64_t n = c->n, na = c->n_alloc;
   std::vector<double> soa((size_t)ndev_of(c->model) * na);
-  if (c->prec == VG_PREC_FP64) {
-    if (hipMemcpy(soa.data(), c->states.p, soa.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
-  } else {
-    std::vector<float> f(soa.size());
-    if (hipMemcpy(f.data(), c->states.p, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
-    std::copy(f.begin(), f.end(), soa.begin());
-  }
+  if (int rc = cells_get_rows(c, soa)) return rc;
   std::vector<double> V(n);
   if (hipMemcpy(V.data(), c->V.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return VG_ERR_HIP;
   for (int64_t i = 0; i < n; i++) {
@@ -1175,12 +1331,7 @@ int vg_cells_set_states(vg_cells *c, int nvar, const double *st) {
   }
   for (int k = 0; k < nd; k++)
     for (int64_t i = n; i < na; i++) soa[(size_t)k * na + i] = soa[(size_t)k * na];
-  if (c->prec == VG_PREC_FP64) {
-    if (hipMemcpy(c->states.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
-  } else {
-    std::vector<float> f(soa.begin(), soa.end());
-    if (hipMemcpy(c->states.p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
-  }
+  if (int rc = cells_put_rows(c, soa)) return rc;
   if (hipMemcpy(c->V.p, V.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
   return VG_OK;
 }

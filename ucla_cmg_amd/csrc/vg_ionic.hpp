@@ -572,4 +572,273 @@ __device__ inline T pk_step(T S[PK_NSTATE], T V, T istim, T dt) {
   return R[0];
 }
 
+// ======================= Mahajan 2008 (UCLA rabbit ventricle) ================
+// IonicModel/Mahajan.cc, constants SetUpIonicConstants.h:159-251.  State
+// order of getInternalParameters: 0 V, 1 m, 2 h, 3 j, 4 xr, 5 xs1, 6 xs2,
+// 7 c_p, 8 c_s, 9 c_i, 10 c_j, 11 c_j', 12 I_r, 13 c1, 14 c2, 15 xi1ca,
+// 16 xi1ba, 17 xi2ca, 18 xi2ba, 19 Na_i, 20 x_tof, 21 y_tof, 22 x_tos,
+// 23 y_tos, 24 troponin-Ca (myoplasm), 25 troponin-Ca (submembrane).
+// The Ca-cycling and Na_i states (MJ_ACC) are accumulated in FP64 in every
+// precision; the gates and channel states (MJ_GATE) in T.
+constexpr int MJ_NSTATE = 26;
+constexpr int MJ_NGATE = 16;
+constexpr int MJ_NACC = 9;
+constexpr int MJ_GATE[MJ_NGATE] = {1, 2, 3, 4, 5, 6, 13, 14, 15, 16, 17, 18, 20, 21, 22, 23};
+constexpr int MJ_ACC[MJ_NACC] = {7, 8, 9, 10, 11, 12, 19, 24, 25};
+constexpr double MJ_XI = 1.55e-4 / 2.58e-8;  // Mahajan.cc:164
+// G_tof and G_Ks by tissue type (epi/myo/endo x apex/center/base).
+constexpr double MJ_GTOF[9] = {0.11, 0.11, 0.11, 0.11, 0.11, 0.11, 0.11 * 0.85, 0.11 * 0.85,
+                               0.11 * 0.85};
+constexpr double MJ_GKS[9] = {1.9 * 0.32,  1.4 * 0.32,  1.0 * 0.32, 0.74 * 0.32, 0.52 * 0.32,
+                              0.35 * 0.32, 0.98 * 0.32, 0.7 * 0.32, 0.5 * 0.32};
+// Initial states by tissue type (Mahajan.cc:27-158).
+constexpr double MJ_INIT[9][MJ_NSTATE] = {
+    {-81.4359, 0.002789, 0.96679, 0.97859, 0.014994, 0.044487, 0.091992, 2.1933, 0.23277,
+     0.24868, 101.6418, 95.4278, 0.009495, 3.7e-005, 0.96094, 0.003718, 6.7e-005, 0.010571,
+     0.024659, 11.5909, 0.00533, 0.99047, 0.005333, 0.17488, 21.5955, 20.0206},
+    {-81.3846, 0.002813, 0.96642, 0.97829, 0.015109, 0.04988, 0.10153, 2.3513, 0.23616, 0.2529,
+     102.5325, 96.0961, 0.010394, 3.7e-005, 0.95815, 0.004448, 6.7e-005, 0.01261, 0.024683,
+     11.442, 0.005349, 0.99012, 0.005352, 0.17323, 21.8749, 20.2219},
+    {-81.3238, 0.002841, 0.96596, 0.97791, 0.015254, 0.056799, 0.11328, 2.5207, 0.23954,
+     0.25736, 103.4612, 96.8014, 0.011385, 3.7e-005, 0.95449, 0.005413, 6.7e-005, 0.015284,
+     0.024706, 11.2566, 0.00537, 0.98962, 0.005373, 0.17158, 22.1689, 20.4268},
+    {-81.2657, 0.002868, 0.96553, 0.97752, 0.015406, 0.064019, 0.12504, 2.6684, 0.24235,
+     0.26131, 104.2505, 97.4054, 0.01227, 3.7e-005, 0.95055, 0.006458, 6.8e-005, 0.018158,
+     0.024723, 11.0791, 0.005391, 0.98902, 0.005394, 0.17018, 22.4276, 20.6034},
+    {-81.1929, 0.002903, 0.96497, 0.97694, 0.015625, 0.073994, 0.14059, 2.8407, 0.2456,
+     0.26605, 105.1459, 98.097, 0.013328, 3.7e-005, 0.94486, 0.00798, 6.8e-005, 0.022313,
+     0.024742, 10.8644, 0.005417, 0.98806, 0.005421, 0.16859, 22.7359, 20.8156},
+    {-81.1078, 0.002944, 0.96431, 0.97609, 0.01595, 0.087401, 0.1603, 3.0189, 0.24905,
+     0.27153, 106.0478, 98.7911, 0.014455, 3.8e-005, 0.93663, 0.010192, 6.8e-005, 0.028302,
+     0.024766, 10.6216, 0.005448, 0.98648, 0.005452, 0.16682, 23.0895, 21.0607},
+    {-81.2626, 0.00287, 0.9655, 0.97756, 0.015387, 0.061634, 0.12278, 2.7185, 0.24183, 0.2588,
+     104.2029, 97.6338, 0.01267, 3.7e-005, 0.95198, 0.006047, 6.8e-005, 0.017057, 0.024804,
+     11.0783, 0.005392, 0.98941, 0.005395, 0.16991, 22.2542, 20.5438},
+    {-81.1873, 0.002905, 0.96493, 0.97701, 0.015593, 0.071141, 0.13799, 2.9031, 0.24508,
+     0.26334, 105.1225, 98.3761, 0.013821, 3.7e-005, 0.94672, 0.007447, 6.8e-005, 0.020885,
+     0.024838, 10.8496, 0.005419, 0.9886, 0.005422, 0.16858, 22.5504, 20.7522},
+    {-81.1085, 0.002943, 0.96432, 0.97631, 0.015852, 0.082505, 0.15518, 3.0765, 0.2482,
+     0.26807, 105.9653, 99.0544, 0.01493, 3.8e-005, 0.94008, 0.009226, 6.8e-005, 0.025709,
+     0.024873, 10.6197, 0.005448, 0.98743, 0.005451, 0.16728, 22.8568, 20.9674}};
+
+// Exponential relaxation over h toward x_inf (time constant tau) as a rate.
+template <class T, bool FAST>
+__device__ inline T mj_relax(T xinf, T tau, T x, T h) {
+  return (xinf - (xinf - x) * Mth<T, FAST>::ex(-h / tau) - x) / h;
+}
+
+// 1 / (1 + sum of the instantaneous buffers' derivatives) at concentration c.
+template <class T>
+__device__ inline T mj_buffer(T c) {
+  const T a = T(7.0) + c, b = T(0.6) + c, m = T(0.3) + c, s = T(13.0) + c;
+  return T(1) / (T(1) + T(24.0 * 7.0) / (a * a) + T(47.0 * 0.6) / (b * b) +
+                 T(15.0 * 0.3) / (m * m) + T(42.0 * 13.0) / (s * s));
+}
+
+// Mahajan::ucla_rhsfun with the comp_* currents (Mahajan.cc:168-603): every
+// rate at gate step h, stimulus istim in uA/uF, tissue-type G_tof and G_Ks.
+template <class T, bool FAST>
+__device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gks,
+                                T R[MJ_NSTATE]) {
+  using M = Mth<T, FAST>;
+  const T V = S[0];
+  const T frt = T(96.485 / (8.314 * 308.0));
+  const T nao = T(136.0), ki = T(140.0), ko = T(5.4), cao = T(1.8);
+  const T nai = S[19], cs = S[8], ci = S[9], cj = S[10];
+  const T sq_ko = T(1.0);  // sqrt(K_o / 5.4) with K_o = 5.4
+
+  // I_K1 with a P_K : P_Na = 100 : 1 reversal
+  const T ek1 = M::lg((T(100.0) * ko + nao) / (T(100.0) * ki + nai)) / frt;
+  const T a_k1 = T(1.02) / (T(1) + M::ex(T(0.2385) * (V - ek1 - T(59.215))));
+  const T b_k1 = (T(0.49124) * M::ex(T(0.08032) * (V - ek1 + T(5.476))) +
+                  M::ex(T(0.061750) * (V - ek1 - T(594.31)))) /
+                 (T(1) + M::ex(T(-0.5143) * (V - ek1 + T(4.753))));
+  const T i_k1 = T(0.3) * sq_ko * a_k1 / (a_k1 + b_k1) * (V - ek1);
+
+  // I_to slow and fast
+  const T ek = M::lg(ko / ki) / frt;
+  const T e3 = M::ex((V + T(3.0)) / T(15.0));
+  const T x_inf = T(1) / (T(1) + T(1) / e3);
+  const T y_inf = T(1) / (T(1) + M::ex((V + T(33.5)) / T(10.0)));
+  const T tau_xs = T(9.0) / (T(1) + e3) + T(0.5);
+  const T tau_ys = T(3000.0) / (T(1) + M::ex((V + T(60.0)) / T(10.0))) + T(30.0);
+  const T i_tos = T(0.04) * S[22] * (S[23] + T(0.5) * y_inf) * (V - ek);
+  R[22] = mj_relax<T, FAST>(x_inf, tau_xs, S[22], h);
+  R[23] = mj_relax<T, FAST>(y_inf, tau_ys, S[23], h);
+  const T v30 = V / T(30.0);
+  const T tau_xf = T(3.5) * M::ex(-v30 * v30) + T(1.5);
+  const T tau_yf = T(20.0) * y_inf + T(20.0);
+  const T i_tof = gtof * S[20] * S[21] * (V - ek);
+  R[20] = mj_relax<T, FAST>(x_inf, tau_xf, S[20], h);
+  R[21] = mj_relax<T, FAST>(y_inf, tau_yf, S[21], h);
+
+  // I_NaK
+  const T sigma = T((exp(136.0 / 67.3) - 1.0) / 7.0);  // Na_o = 136 mM
+  const T evf = M::ex(-V * frt);
+  const T f_nak = T(1) / (T(1) + T(0.1245) * M::ex(T(-0.1) * V * frt) + T(0.0365) * sigma * evf);
+  const T i_nak = T(1.5) * f_nak / (T(1) + T(12.0) / nai) * ko / (ko + T(1.5));
+
+  // Na/Ca exchange flux (concentrations in mM)
+  const T csm = cs / T(1000.0);
+  const T nai3 = nai * nai * nai, nao3 = nao * nao * nao;
+  const T e_in = M::ex(T(0.35) * V * frt), e_out = M::ex(T(0.35 - 1.0) * V * frt);
+  const T zw3 = nai3 * cao * e_in - nao3 * csm * e_out;
+  const T zw4 = T(1) + T(0.2) * e_out;
+  const T rl = T(0.3) / cs;
+  const T aloss = T(1) / (T(1) + rl * rl * rl);
+  const T rn = nai / T(12.3);
+  const T zw8 = T(1.3) * nai3 + T(87.5 * 87.5 * 87.5) * csm +
+                T(12.3 * 12.3 * 12.3) * cao * (T(1) + csm / T(0.0036)) +
+                T(0.0036) * nao3 * (T(1) + rn * rn * rn) + nai3 * cao + nao3 * csm;
+  const T j_naca = T(0.84) * aloss * zw3 / (zw4 * zw8);
+
+  // Ca cycling
+  const T j_d = (cs - ci) / T(4.0);
+  R[24] = T(0.0327) * ci * (T(70.0) - S[24]) - T(0.0196) * S[24];
+  R[25] = T(0.0327) * cs * (T(70.0) - S[25]) - T(0.0196) * S[25];
+  const T j_up = T(0.4) * ci * ci / (ci * ci + T(0.25));
+  const T j_leak = T(2.069e-5) * cj * cj / (cj * cj + T(2500.0)) * (T(16.667) * cj - ci);
+
+  // L-type Ca channel Markov chain
+  const T po_inf = T(1) / (T(1) + M::ex(-V / T(8.0)));
+  const T alpha = po_inf, beta = T(1) - po_inf;
+  const T rc = T(3.0) / S[7];
+  const T fca = T(1) / (T(1) + rc * rc * rc);
+  const T s1 = T(0.0182688) * fca, s1t = T(0.00195);
+  const T k1 = T(0.024168) * fca, k2 = T(1.03615e-4), k1t = T(0.00413), k2t = T(0.00224);
+  const T r1 = T(0.30), r2 = T(3.0);
+  const T s2 = s1 * (r1 / r2) * (k2 / k1);
+  const T s2t = T(0.00195 * (0.30 / 3.0) * (0.00224 / 0.00413));
+  const T poi = T(1) / (T(1) + M::ex(-(V + T(40.0)) / T(3.0)));
+  const T k3 = (T(1) - poi) / T(3.0), k3t = k3;
+  const T Pr = T(1) - T(1) / (T(1) + M::ex(-(V + T(40.0)) / T(4.0)));
+  const T recov = T(10.0) + T(4954.0) * M::ex(V / T(15.6));
+  const T rcp = S[7] / T(6.09365);
+  const T rcp2 = rcp * rcp;
+  const T tau_ca0 = T(78.0329) / (T(1) + rcp2 * rcp2) + T(0.1);
+  const T tau_ca = (recov - tau_ca0) * Pr + tau_ca0;
+  const T tau_ba = (recov - T(450.0)) * Pr + T(450.0);
+  const T Ps = T(1) / (T(1) + M::ex(-(V + T(40.0)) / T(11.32)));
+  const T k6 = fca * Ps / tau_ca, k5 = (T(1) - Ps) / tau_ca;
+  const T k6t = Ps / tau_ba, k5t = (T(1) - Ps) / tau_ba;
+  const T k4 = k3 * (alpha / beta) * (k1 / k2) * (k5 / k6);
+  const T k4t = k3t * (alpha / beta) * (k1t / k2t) * (k5t / k6t);
+  const T c1 = S[13], c2 = S[14], i1c = S[15], i1b = S[16], i2c = S[17], i2b = S[18];
+  const T po = T(1) - i1c - i2c - i1b - i2b - c1 - c2;
+  R[14] = beta * c1 + k5 * i2c + k5t * i2b - (k6 + k6t + alpha) * c2;
+  R[13] = alpha * c2 + k2 * i1c + k2t * i1b + r2 * po - (beta + r1 + k1t + k1) * c1;
+  R[15] = k1 * c1 + k4 * i2c + s1 * po - (k3 + k2 + s2) * i1c;
+  R[17] = k3 * i1c + k6 * c2 - (k5 + k4) * i2c;
+  R[16] = k1t * c1 + k4t * i2b + s1t * po - (k3t + k2t + s2t) * i1b;
+  R[18] = k3t * i1b + k6t * c2 - (k5t + k4t) * i2b;
+
+  // Ca driving force
+  const T za = T(2.0) * V * frt;
+  const T f1 = T(4.0 * 0.00054 * 96.485 * 96.485 / (8.314 * 308.0));
+  const T eza = M::ex(za);
+  T rxa;
+  if (fabs(za) < T(0.001)) rxa = f1 * (csm * eza - T(0.341) * cao) / (T(2.0) * frt);
+  else rxa = V * f1 * (csm * eza - T(0.341) * cao) / (eza - T(1));
+  const T j_ca = T(182.0) * po * rxa;
+
+  R[8] = mj_buffer(cs) * (T(50.0) * (S[12] - j_d - j_ca + j_naca) - R[25]);
+  R[9] = mj_buffer(ci) * (j_d - j_up + j_leak - R[24]);
+  R[10] = -S[12] + j_up - j_leak;
+  R[11] = (cj - S[11]) / T(100.0);
+
+  // SR release: load dependence, sparks, dyad
+  T q = T(0);
+  const T cjp = S[11];
+  if (cjp > T(50.0) && cjp < T(90.0)) q = cjp - T(50.0);
+  else if (cjp >= T(90.0)) q = T(11.3) * cjp + T((90.0 - 50.0) - 11.3 * 90.0);
+  if (q >= T(153.0)) q = T(153.0);
+  const T Qr = cj * q / T(90.0);
+  const T arx = fabs(rxa);
+  const T ey = M::ex(T(-0.05) * (V + T(30.0)));
+  const T spark = T(2.58079) * po * arx * ey / (T(1) + ey);
+  R[12] = spark * Qr - S[12] * (T(1) - T(30.0) * R[10] / cj) / T(30.0);
+  const T ex = M::ex(T(-0.3576) * (V + T(30.0)));
+  const T g_sr = T(26841.8) * ex / (T(1) + ex);
+  R[7] = po * Qr * arx * g_sr + po * T(9000.0) * arx - (S[7] - cs) / T(0.5);
+
+  // I_Na
+  const T ena = M::lg(nao / nai) / frt;
+  T am = T(3.2);
+  if (fabs(V + T(47.13)) > T(0.001))
+    am = T(0.32) * (V + T(47.13)) / (T(1) - M::ex(T(-0.1) * (V + T(47.13))));
+  const T bm = T(0.08) * M::ex(-V / T(11.0));
+  const T a = T(1) - T(1) / (T(1) + M::ex(-(V + T(40.0)) / T(0.24)));
+  const T ah = a * T(0.135) * M::ex((T(80.0) + V) / T(-6.8));
+  const T bh = (T(1) - a) / (T(0.13) * (T(1) + M::ex((V + T(10.66)) / T(-11.1)))) +
+               a * (T(3.56) * M::ex(T(0.079) * V) + T(3.1e5) * M::ex(T(0.35) * V));
+  const T aj = a * (T(-1.2714e5) * M::ex(T(0.2444) * V) - T(3.474e-5) * M::ex(T(-0.04391) * V)) *
+               (V + T(37.78)) / (T(1) + M::ex(T(0.311) * (V + T(79.23))));
+  const T bj = (T(1) - a) * (T(0.3) * M::ex(T(-2.535e-7) * V) / (T(1) + M::ex(T(-0.1) * (V + T(32.0))))) +
+               a * (T(0.1212) * M::ex(T(-0.01052) * V) / (T(1) + M::ex(T(-0.1378) * (V + T(40.14)))));
+  const T tau_h = T(1) / (ah + bh), tau_j = T(1) / (aj + bj), tau_m = T(1) / (am + bm);
+  const T m = S[1];
+  const T i_na = T(12.0) * S[2] * S[3] * m * m * m * (V - ena);
+  R[2] = mj_relax<T, FAST>(ah * tau_h, tau_h, S[2], h);
+  R[3] = mj_relax<T, FAST>(aj * tau_j, tau_j, S[3], h);
+  R[1] = mj_relax<T, FAST>(am * tau_m, tau_m, S[1], h);
+
+  // I_Kr
+  T kr1 = T(0.00138 / 0.123), kr2 = T(0.00061 / 0.145);
+  if (fabs(V + T(7.0)) > T(0.001)) kr1 = T(0.00138) * (V + T(7.0)) / (T(1) - M::ex(T(-0.123) * (V + T(7.0))));
+  if (fabs(V + T(10.0)) > T(0.001)) kr2 = T(0.00061) * (V + T(10.0)) / (M::ex(T(0.145) * (V + T(10.0))) - T(1));
+  const T xr_inf = T(1) / (T(1) + M::ex(-(V + T(50.0)) / T(7.5)));
+  const T rg = T(1) / (T(1) + M::ex((V + T(33.0)) / T(22.4)));
+  const T i_kr = T(0.0125) * sq_ko * S[4] * rg * (V - ek);
+  R[4] = mj_relax<T, FAST>(xr_inf, T(1) / (kr1 + kr2), S[4], h);
+
+  // I_Ks with Ca dependence
+  const T eks = M::lg((ko + T(0.018330) * nao) / (ki + T(0.018330) * nai)) / frt;
+  const T xs_inf = T(1) / (T(1) + M::ex(-(V - T(1.50)) / T(16.70)));
+  T tau_xs1;
+  if (fabs(V + T(30.0)) < T(0.001 / 0.0687))
+    tau_xs1 = T(1.0 / (0.0000719 / 0.148 + 0.000131 / 0.0687));
+  else
+    tau_xs1 = T(1) / (T(0.0000719) * (V + T(30.0)) / (T(1) - M::ex(T(-0.148) * (V + T(30.0)))) +
+                      T(0.000131) * (V + T(30.0)) / (M::ex(T(0.0687) * (V + T(30.0))) - T(1)));
+  const T rks = T(0.5) / ci;
+  const T g_ksx = T(1) + T(0.8) / (T(1) + rks * rks * rks);
+  const T i_ks = gks * g_ksx * S[5] * S[6] * (V - eks);
+  R[5] = mj_relax<T, FAST>(xs_inf, tau_xs1, S[5], h);
+  R[6] = mj_relax<T, FAST>(xs_inf, T(4.0) * tau_xs1, S[6], h);
+
+  // Na_i and V
+  const T wca = T(8.0313);
+  R[19] = -(T(1.0 / 8.0313 / 1000.0)) * (i_na + T(3.0) * i_nak + T(3.0) * wca * j_naca);
+  R[0] = -(i_na + i_k1 + i_kr + i_ks + i_tos + i_tof + wca * j_naca + T(2.0) * wca * j_ca + i_nak) +
+         istim;
+}
+
+// Mahajan::Compute_Ion + UpdateStateVariables (Mahajan.cc:607-644) for one
+// node: S holds all 26 states in T (S[0] is set to V), acc the FP64 copies of
+// the MJ_ACC states, which the updates accumulate.  Forward Euler over dt, or
+// 10 sub-steps of dt/10 when |dV/dt| > 20 (int_abs: when |trunc(dV/dt)| > 20);
+// returns the last dV/dt evaluated.
+template <class T, bool FAST>
+__device__ inline T mj_step(T S[MJ_NSTATE], double acc[MJ_NACC], T V, T istim, T dt, T gtof,
+                            T gks, bool int_abs) {
+  T R[MJ_NSTATE];
+  S[0] = V;
+  mj_rates<T, FAST>(S, dt, istim, gtof, gks, R);
+  const bool adapt = int_abs ? fabs(trunc(R[0])) > T(20) : fabs(R[0]) > T(20);
+  const int nsub = adapt ? 10 : 1;
+  const T h = adapt ? dt / T(10) : dt;
+  for (int q = 0; q < nsub; q++) {
+    if (adapt) mj_rates<T, FAST>(S, h, istim, gtof, gks, R);
+    S[0] += R[0] * h;
+#pragma unroll
+    for (int k = 0; k < MJ_NGATE; k++) S[MJ_GATE[k]] += R[MJ_GATE[k]] * h;
+#pragma unroll
+    for (int k = 0; k < MJ_NACC; k++) {
+      acc[k] += (double)R[MJ_ACC[k]] * (double)h;
+      S[MJ_ACC[k]] = (T)acc[k];
+    }
+  }
+  return R[0];
+}
+
 }  // namespace vg

@@ -137,7 +137,6 @@ template <class T, class CF, class VF, class G>
 __device__ inline T sell_dot(int wd, CF colat, VF valat, G gather) {
   T acc = T(0);
   int j = 0;
-#pragma unroll 2
   for (; j + 4 <= wd; j += 4) {
     int c[4];
     T v[4];
@@ -432,12 +431,14 @@ struct IonicArgs {
   double Cm;
   int max_newton;
   int newton_jt;
+  int mj_int_abs;
 };
 
 template <class T, int MODEL, int INTEG>
 __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const T *__restrict__ x,
                                                const T *__restrict__ sinv, T *__restrict__ Vout,
-                                               T *__restrict__ st, const T *__restrict__ istim,
+                                               T *__restrict__ st, double *__restrict__ acc,
+                                               const T *__restrict__ istim,
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, SubSteps sub,
                                                IonicArgs a, unsigned long long *newton_sum,
@@ -464,6 +465,27 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       }
 #pragma unroll
       for (int k = 0; k < LR_NSTATE - 1; k++) st[k * a.n_alloc + i] = s[k];
+    } else if constexpr (MODEL == VG_MODEL_MJ) {
+      // Gates and channel states in T (rows of st), Ca-cycling and Na_i in
+      // FP64 (rows of acc); sc[0], sc[1] = G_tof, G_Ks of the node's type.
+      T s[MJ_NSTATE];
+      double ac[MJ_NACC];
+#pragma unroll
+      for (int k = 0; k < MJ_NGATE; k++) s[MJ_GATE[k]] = st[k * a.n_alloc + i];
+#pragma unroll
+      for (int k = 0; k < MJ_NACC; k++) {
+        ac[k] = acc[k * a.n_alloc + i];
+        s[MJ_ACC[k]] = (T)ac[k];
+      }
+      const T it = stim / (Cm * T(MJ_XI));
+      for (int q = 0; q < sub.n; q++) {
+        const T dt = (T)sub.dt[q];
+        V += dt * mj_step<T, sizeof(T) == 4>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
+      }
+#pragma unroll
+      for (int k = 0; k < MJ_NGATE; k++) st[k * a.n_alloc + i] = s[MJ_GATE[k]];
+#pragma unroll
+      for (int k = 0; k < MJ_NACC; k++) acc[k * a.n_alloc + i] = ac[k];
     } else {
       T s[TT_NSTATE];
 #pragma unroll
@@ -585,10 +607,11 @@ template <class T, int MODEL, int INTEG>
 __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
                                                const double *__restrict__ stim_d,
                                                double *__restrict__ dvdt, T *__restrict__ st,
+                                               double *__restrict__ acc,
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, int64_t n,
                                                int64_t n_alloc, double Cm_d, double dt_d,
-                                               int max_newton, int newton_jt) {
+                                               int max_newton, int newton_jt, int mj_int_abs) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const T V = (T)Vd[i], stim = stim_d ? (T)stim_d[i] : T(0), Cm = (T)Cm_d, dt = (T)dt_d;
@@ -601,6 +624,22 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
     dvdt[i] = (double)lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3]);
 #pragma unroll
     for (int k = 0; k < LR_NSTATE - 1; k++) st[k * n_alloc + i] = s[k];
+  } else if constexpr (MODEL == VG_MODEL_MJ) {
+    T s[MJ_NSTATE];
+    double ac[MJ_NACC];
+#pragma unroll
+    for (int k = 0; k < MJ_NGATE; k++) s[MJ_GATE[k]] = st[k * n_alloc + i];
+#pragma unroll
+    for (int k = 0; k < MJ_NACC; k++) {
+      ac[k] = acc[k * n_alloc + i];
+      s[MJ_ACC[k]] = (T)ac[k];
+    }
+    dvdt[i] = (double)mj_step<T, sizeof(T) == 4>(s, ac, V, stim / (Cm * T(MJ_XI)), dt, sc[0],
+                                                 sc[1], mj_int_abs != 0);
+#pragma unroll
+    for (int k = 0; k < MJ_NGATE; k++) st[k * n_alloc + i] = s[MJ_GATE[k]];
+#pragma unroll
+    for (int k = 0; k < MJ_NACC; k++) acc[k * n_alloc + i] = ac[k];
   } else if constexpr (MODEL == VG_MODEL_PURKINJE) {
     // All 22 states are kept: the model advances its own V copy internally.
     T s[PK_NSTATE];

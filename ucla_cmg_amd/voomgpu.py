@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libvoomgpu.so")
 VG_OK = 0
 MODEL_LR = 0
 MODEL_TT = 2
+MODEL_MJ = 3
 MODEL_PURKINJE = 10
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -34,6 +35,7 @@ QUIRK_V0_ZERO = 1
 QUIRK_PMJ_STICKY = 2
 QUIRK_BAR_1X_VOLUME = 4
 QUIRK_NEWTON_JT = 8
+QUIRK_MJ_INT_ABS = 16
 TT_NSCALE = 5
 X0_ZERO, X0_PREVIOUS, X0_LAST = 0, 1, 2
 KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY = range(5)
@@ -87,11 +89,13 @@ class VgStats(C.Structure):
 _lib = None
 
 
-def load_library(path: str = LIB_PATH):
-    """Loads libvoomgpu.so; raises if it has not been built."""
+def load_library(path: str = None):
+    """Loads libvoomgpu.so (or the build named by $VOOMGPU_LIB); raises if it
+    has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("VOOMGPU_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
     L = C.CDLL(path)
@@ -417,6 +421,13 @@ class IonicCells:
         sc = np.ascontiguousarray(scales, np.float64).reshape(-1, TT_NSCALE)
         pi = None if index is None else np.ascontiguousarray(index, np.uint16)
         rc = self.L.vg_cells_set_scales(self.h, sc.shape[0], _p(sc), _p(pi))
+        if rc != VG_OK:
+            raise VoomError(f"vg_cells_set_scales failed ({rc})")
+
+    def set_tissue_types(self, types):
+        """Mahajan: tissue type 0-8 of each cell (others run type 2)."""
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(types), (self.n,)), np.uint16)
+        rc = self.L.vg_cells_set_scales(self.h, 0, None, _p(t))
         if rc != VG_OK:
             raise VoomError(f"vg_cells_set_scales failed ({rc})")
 
