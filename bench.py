@@ -148,6 +148,8 @@ def main():
                     help="workgroups per CG launch (0 = library default)")
     ap.add_argument("--cg-x0", choices=["zero", "previous", "last"], default="last",
                     help="initial guess of each half-step solve (same stopping bound)")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="replay each step as a captured hipGraph (no host syncs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB 20 times (PMC byte "
                          "counter calibration) and exit")
@@ -183,6 +185,7 @@ def main():
                               param_index=pidx, cg_blocks=args.cg_blocks,
                               cg_x0={"zero": vg.X0_ZERO, "previous": vg.X0_PREVIOUS,
                                      "last": vg.X0_LAST}[args.cg_x0],
+                              graph=args.graph,
                               **MODEL_KW)
     solver.enableActivation(-40.0)
     solver.setupECG(electrodes10())
@@ -272,6 +275,7 @@ def main():
             "nodes": n, "tets": mesh.n_elems, "nnz": nnz, "cg_iters_per_step": k_per_step,
             "cg_blocks": args.cg_blocks,
             "cg_x0": args.cg_x0,
+            "graph": args.graph,
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -136,6 +136,18 @@ typedef struct {
   int cg_blocks;      /* workgroups of each SpMV/CG launch, rounded up to a
                          multiple of 8 (0 = the library's choice) */
   int cg_x0;          /* VG_X0_*: initial guess of each half-step solve */
+  /* Batching: n_instances > 1 independent bodies in one handle (no shared
+   * elements, no Purkinje nodes); instance[n_nodes] gives each node's body.
+   * Every body runs its own CG to its own tolerance; launches are shared. */
+  int n_instances;
+  const int32_t *instance;
+  /* 1: after a few host-driven steps, capture the whole step (both solves
+   * with a fixed number of CG launches each, learned from those steps with a
+   * margin; launches past convergence return at once) in a hipGraph and
+   * replay it with no host synchronisation.  A solve that would need more
+   * launches is counted in vg_stats.cg_not_converged and the step is
+   * re-captured with more. */
+  int graph;
 } vg_model;
 
 /* Initial guess of the CG solves.  Either way a solve stops when

@@ -97,7 +97,10 @@ struct vg_handle {
   unsigned quirks = 0;
   double dt = 0.01, dt_ion = 0.01, Cm = 1.0, tol = 1e-7;
   int maxit = 1000;
-  int64_t n = 0, n_alloc = 0, n_slices = 0, nnz = 0, padded = 0;
+  // n: mesh nodes; rows: device rows (n plus the padding rows that align
+  // each batched instance to a slice); n_alloc: rows rounded up to BLK.
+  int64_t n = 0, rows = 0, n_alloc = 0, n_slices = 0, nnz = 0, padded = 0;
+  int n_inst = 1;
   size_t tsz = 8;  // sizeof(T)
   hipStream_t stream = nullptr;
   int device = 0;
@@ -131,10 +134,22 @@ struct vg_handle {
   // scalars and the result record (pinned copy for the host).
   unsigned cg_blocks = 0, node_blocks = 0;
   DevBuf part, carry, res, newton;
+  DevBuf blk_inst, inst_blk, inst_slice, slice_inst;  // instance tables (CgBufs)
   DevBuf H, ecg_part, ecg_out, ecg_frames;
   CgResult *h_res = nullptr;  // pinned
   int nsets = 1;
   int cg_guess[2] = {8, 8};  // launches expected per solve, by half-step
+  int seen_max[2] = {0, 0};  // most iterations seen per half-step
+  // Step counter on the device (the apply kernel's time) and the solve
+  // totals of captured steps.
+  DevBuf step_ctr, totals;
+  CgTotals *h_tot = nullptr;  // pinned copy
+  // Captured step (graph mode).
+  int graph = 0, warm = 0;
+  hipGraphExec_t gexec = nullptr;
+  int glaunch[2] = {0, 0};
+  bool g_stim = false, g_at = false;
+  unsigned long long g_notconv_seen = 0;
   // activation
   bool at_enabled = false;
   double at_thr = -40.0;
@@ -164,6 +179,8 @@ struct vg_handle {
       }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_res) (void)hipHostFree(h_res);
+    if (h_tot) (void)hipHostFree(h_tot);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream_pk) (void)hipStreamDestroy(stream_pk);
@@ -236,8 +253,19 @@ Sell sell_of(vg_handle *h) {
 
 CgBufs cgbufs_of(vg_handle *h) {
   double *part = h->part.as<double>();
-  return CgBufs{part, part + 3 * (size_t)h->cg_blocks, h->carry.as<CgCarry>(),
-                h->res.as<CgResult>()};
+  return CgBufs{part,
+                part + 3 * (size_t)h->cg_blocks,
+                h->carry.as<CgCarry>(),
+                h->res.as<CgResult>(),
+                h->blk_inst.as<int>(),
+                h->inst_blk.as<int>(),
+                h->inst_slice.as<int64_t>(),
+                h->n_inst};
+}
+
+// Slice -> instance table for the node-wise kernels (none with one instance).
+const int *slice_inst_of(vg_handle *h) {
+  return h->n_inst > 1 ? h->slice_inst.as<int>() : nullptr;
 }
 
 // Increment buffer of half-step `which` (0: before the ionic update).
@@ -253,12 +281,12 @@ void launch_rhs(vg_handle *h, const T *V, int which) {
     hipLaunchKernelGGL((k_rhs<T, true>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                        sell_of(h), h->Krhs.as<T>(),
                        h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
-                       h->n_slices, cgbufs_of(h));
+                       cgbufs_of(h));
   else
     hipLaunchKernelGGL((k_rhs<T, false>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                        sell_of(h), h->Krhs.as<T>(),
                        h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
-                       h->n_slices, cgbufs_of(h));
+                       cgbufs_of(h));
 }
 
 template <class T>
@@ -266,7 +294,7 @@ void launch_cg_start(vg_handle *h) {
   ProfScope ps(h, VG_KC_CG_START);
   hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
-                     h->r.as<T>(), h->w.as<T>(), h->diag.as<T>(), h->n_slices, cgbufs_of(h));
+                     h->r.as<T>(), h->w.as<T>(), h->diag.as<T>(), cgbufs_of(h));
 }
 
 template <class T>
@@ -276,7 +304,30 @@ void launch_cg_iter(vg_handle *h, int j, int which) {
                      sell_of(h), h->Ahat.as<T>(),
                      h->r.as<T>(), h->r2.as<T>(), h->w.as<T>(), h->w2.as<T>(), h->s.as<T>(),
                      h->s2.as<T>(), h->p.as<T>(), xbuf<T>(h, which), h->diag.as<T>(),
-                     h->n_slices, cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
+                     cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
+}
+
+// Folds the result records of a finished solve (one per instance) into the
+// statistics and the expected launch count.
+int cg_account(vg_handle *h, int which) {
+  int max_iter = 0;
+  bool nonfinite = false;
+  h->last_relres = 0.0;
+  for (int k = 0; k < h->n_inst; k++) {
+    const CgResult &c = h->h_res[k];
+    if (!c.done) return h->fail(VG_ERR_HIP, "CG launches ended without a result");
+    h->cg_iters += c.iter;
+    h->cg_solves += 1;
+    h->last_relres = std::max(h->last_relres, c.rho_ref > 0 ? std::sqrt(c.rho / c.rho_ref) : 0.0);
+    if (c.notconv) h->cg_notconv += 1;
+    if (c.notconv && !(std::isfinite(c.rho) && std::isfinite(c.rho_ref))) nonfinite = true;
+    max_iter = std::max(max_iter, c.iter);
+  }
+  h->seen_max[which] = std::max(h->seen_max[which], max_iter);
+  // The launch that finds convergence is the (iter+1)-th; one more spare.
+  h->cg_guess[which] = std::min(h->maxit + 1, max_iter + 2);
+  if (nonfinite) return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
+  return VG_OK;
 }
 
 // One diffusion half-step: r_0 = K_rhs V, then CG to convergence.  The
@@ -292,22 +343,15 @@ int cg_solve(vg_handle *h, const T *V, int which) {
   for (;;) {
     for (int k = 0; k < batch && j <= h->maxit; k++) launch_cg_iter<T>(h, j++, which);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemcpyAsync(h->h_res, h->res.p, sizeof(CgResult), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_res, h->res.p, sizeof(CgResult) * h->n_inst,
+                             hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->h_res->done || j > h->maxit) break;
+    bool all = true;
+    for (int k = 0; k < h->n_inst; k++) all = all && h->h_res[k].done;
+    if (all || j > h->maxit) break;
     batch = 4;
   }
-  const CgResult &c = *h->h_res;
-  if (!c.done) return h->fail(VG_ERR_HIP, "CG launches ended without a result");
-  h->cg_iters += c.iter;
-  h->cg_solves += 1;
-  h->last_relres = c.rho_ref > 0 ? std::sqrt(c.rho / c.rho_ref) : 0.0;
-  if (c.notconv) h->cg_notconv += 1;
-  // The launch that finds convergence is the (iter+1)-th; one more spare.
-  h->cg_guess[which] = std::min(h->maxit + 1, c.iter + 2);
-  if (c.notconv && !(std::isfinite(c.rho) && std::isfinite(c.rho_ref)))
-    return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
-  return VG_OK;
+  return cg_account(h, which);
 }
 
 template <class T, int MODEL, int INTEG>
@@ -320,7 +364,7 @@ void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
                      h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
                      h->scales.as<T>(), h->sub, a, h->newton.as<unsigned long long>(),
-                     h->res.as<CgResult>());
+                     h->res.as<CgResult>(), slice_inst_of(h));
 }
 
 // Ionic update of every node.  The Purkinje rows (few, long per-node work)
@@ -339,7 +383,7 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
     hipLaunchKernelGGL(k_ionic_pk<T>, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, sp, Vin,
                        x, h->sinv.as<T>(), Vout, h->pk_states.as<T>(),
                        h->stim_active ? h->istim.as<T>() : (const T *)nullptr, h->n_myo, h->n_pk,
-                       h->n_pk_alloc, h->sub, h->res.as<CgResult>());
+                       h->n_pk_alloc, h->sub, h->res.as<CgResult>(), slice_inst_of(h));
     if (fork) HIPCHK(h, hipEventRecord(h->ev_join, h->stream_pk));
   }
   if (h->n_myo > 0) {
@@ -354,6 +398,14 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   return VG_OK;
 }
 
+// Folds half-step `which`'s results into the device totals and/or advances
+// the device step counter (one thread).
+void launch_fold(vg_handle *h, int which, bool fold, bool tick) {
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, h->stream, h->res.as<CgResult>(), h->n_inst,
+                     h->totals.as<CgTotals>(), which, fold ? 1 : 0, h->step_ctr.as<int64_t>(),
+                     tick ? 1 : 0);
+}
+
 // Sums the ECG partials of the last k_apply / k_ecg launch into dst[ne].
 void launch_ecg_finish(vg_handle *h, double *dst) {
   hipLaunchKernelGGL(k_ecg_finish, dim3(1), dim3(BLK), 0, h->stream, h->ecg_part.as<double>(),
@@ -361,16 +413,17 @@ void launch_ecg_finish(vg_handle *h, double *dst) {
 }
 
 template <class T>
-void launch_apply(vg_handle *h, double t0, bool ecg_frame) {
+void launch_apply(vg_handle *h, bool ecg_frame) {
   ProfScope ps(h, VG_KC_APPLY);
   const T thr = (T)h->at_thr;
   const T *Hm = h->H.p ? h->H.as<T>() : nullptr;
   double *part = h->ecg_part.p ? h->ecg_part.as<double>() : nullptr;
 #define VG_APPLY(AT, ECG)                                                                        \
   hipLaunchKernelGGL((k_apply<T, AT, ECG>), dim3(h->node_blocks), dim3(BLK), 0, h->stream,       \
-                     h->Vb.as<T>(), xbuf<T>(h, 1), h->sinv.as<T>(), h->Va.as<T>(), h->n,         \
-                     h->at.as<T>(), thr, t0, h->dt, Hm, h->ne, h->n_alloc, part,                 \
-                     h->res.as<CgResult>())
+                     h->Vb.as<T>(), xbuf<T>(h, 1), h->sinv.as<T>(), h->Va.as<T>(), h->rows,      \
+                     h->at.as<T>(), thr, h->step_ctr.as<int64_t>(), h->dt, Hm, h->ne,           \
+                     h->n_alloc, part,                                                            \
+                     h->res.as<CgResult>(), slice_inst_of(h))
   if (h->at_enabled) {
     if (ecg_frame) VG_APPLY(true, true);
     else VG_APPLY(true, false);
@@ -391,15 +444,17 @@ int upload_stim(vg_handle *h) {
       h->stim_active = true;
       break;
     }
-  const int64_t n = h->n;
+  const int64_t rows = h->rows;
   if (h->tsz == 8) {
     std::vector<double> buf(h->n_alloc, 0.0);
-    for (int64_t r = 0; r < n; r++) buf[r] = h->istim_host[h->new2old[r]];
+    for (int64_t r = 0; r < rows; r++)
+      if (h->new2old[r] >= 0) buf[r] = h->istim_host[h->new2old[r]];
     HIPCHK(h, hipMemcpyAsync(h->istim.p, buf.data(), buf.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
   } else {
     std::vector<float> buf(h->n_alloc, 0.0f);
-    for (int64_t r = 0; r < n; r++) buf[r] = (float)h->istim_host[h->new2old[r]];
+    for (int64_t r = 0; r < rows; r++)
+      if (h->new2old[r] >= 0) buf[r] = (float)h->istim_host[h->new2old[r]];
     HIPCHK(h, hipMemcpyAsync(h->istim.p, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
   }
@@ -411,7 +466,6 @@ template <class T>
 int step_t(vg_handle *h) {
   int rc = upload_stim(h);
   if (rc) return rc;
-  const double t0 = h->t;
   // Step 1: half-step diffusion, C_m M + dT/4 K against C_m M - dT/4 K.
   if ((rc = cg_solve<T>(h, h->Va.as<T>(), 0))) return rc;
   // Step 2: ionic update over dT in the reference's sub-steps, fused with
@@ -423,15 +477,91 @@ int step_t(vg_handle *h) {
   h->t = h->steps * h->dt;
   const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
                      h->ecg_count < h->ecg_max;
-  launch_apply<T>(h, t0, frame);
+  launch_apply<T>(h, frame);
+  launch_fold(h, 1, false, true);
   if (frame) h->ecg_count += 1;
+  return VG_OK;
+}
+
+// Captures one whole step with glaunch[w] CG launches per half-step.
+template <class T>
+int capture_step(vg_handle *h) {
+  if (h->gexec) {
+    HIPCHK(h, hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
+  for (int w = 0; w < 2; w++)
+    h->glaunch[w] = std::min(h->maxit + 1, h->seen_max[w] + std::max(4, h->seen_max[w] / 4) + 1);
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const T *V[2] = {h->Va.as<T>(), h->Vb.as<T>()};
+  for (int w = 0; w < 2; w++) {
+    launch_rhs<T>(h, V[w], w);
+    launch_cg_start<T>(h);
+    for (int j = 0; j < h->glaunch[w]; j++) launch_cg_iter<T>(h, j, w);
+    launch_fold(h, w, true, false);
+    if (w == 0) {
+      const int rc = launch_ionic<T>(h, h->Va.as<T>(), xbuf<T>(h, 0), h->Vb.as<T>());
+      if (rc) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+    }
+  }
+  launch_apply<T>(h, false);
+  launch_fold(h, 1, false, true);
+  hipGraph_t g = nullptr;
+  HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+  const hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(h, e);
+  h->g_stim = h->stim_active;
+  h->g_at = h->at_enabled;
+  return VG_OK;
+}
+
+// Graph mode: the first steps run host-driven (learning the launch counts),
+// then the captured step is replayed.  ECG frames are taken after a replay
+// from the new V.  Every 32 steps the device totals are read: a solve that
+// ran out of captured launches raises the counts and re-captures.
+template <class T>
+int step_graph(vg_handle *h) {
+  int rc = upload_stim(h);
+  if (rc) return rc;
+  if (h->warm < 3) {
+    h->warm++;
+    return step_t<T>(h);
+  }
+  if (!h->gexec || h->g_stim != h->stim_active || h->g_at != h->at_enabled)
+    if ((rc = capture_step<T>(h))) return rc;
+  HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  h->steps += 1;
+  h->t = h->steps * h->dt;
+  if (h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 && h->ecg_count < h->ecg_max) {
+    hipLaunchKernelGGL(k_ecg<T>, dim3(h->node_blocks), dim3(BLK), 0, h->stream, h->Va.as<T>(),
+                       h->rows, h->H.as<T>(), h->ne, h->n_alloc, h->ecg_part.as<double>());
+    launch_ecg_finish(h, h->ecg_frames.as<double>() + (size_t)h->ecg_count * h->ne);
+    h->ecg_count += 1;
+  }
+  if (h->steps % 32 == 0) {
+    HIPCHK(h, hipMemcpyAsync(h->h_tot, h->totals.p, sizeof(CgTotals), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->h_tot->nonfinite) return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
+    if (h->h_tot->notconv > h->g_notconv_seen) {
+      h->g_notconv_seen = h->h_tot->notconv;
+      for (int w = 0; w < 2; w++) h->seen_max[w] = std::max(h->seen_max[w], h->glaunch[w]) * 3 / 2;
+      if ((rc = capture_step<T>(h))) return rc;
+    }
+  }
   return VG_OK;
 }
 
 template <class T>
 int upload_vec(vg_handle *h, void *dst, const double *src_orig, double fill) {
   std::vector<T> buf(h->n_alloc, (T)fill);
-  for (int64_t r = 0; r < h->n; r++) buf[r] = (T)src_orig[h->new2old[r]];
+  for (int64_t r = 0; r < h->rows; r++)
+    if (h->new2old[r] >= 0) buf[r] = (T)src_orig[h->new2old[r]];
   HIPCHK(h, hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return VG_OK;
@@ -442,7 +572,8 @@ int download_vec(vg_handle *h, const void *src, double *dst_orig) {
   std::vector<T> buf(h->n_alloc);
   HIPCHK(h, hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(T), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (int64_t r = 0; r < h->n; r++) dst_orig[h->new2old[r]] = (double)buf[r];
+  for (int64_t r = 0; r < h->rows; r++)
+    if (h->new2old[r] >= 0) dst_orig[h->new2old[r]] = (double)buf[r];
   return VG_OK;
 }
 
@@ -492,8 +623,11 @@ int set_states_impl(vg_handle *h, const int64_t *offsets, const double *data) {
   const int nd = ndev_of(h->model), vs = vslot_of(h->model);
   const int64_t na = h->n_alloc, npa = h->n_pk_alloc;
   std::vector<double> soa((size_t)nd * na, 0.0), pk((size_t)PK_NSTATE * npa, 0.0);
-  for (int64_t r = 0; r < h->n; r++) {
+  int64_t first_real = -1;
+  for (int64_t r = 0; r < h->rows; r++) {
     const int64_t o = h->new2old[r];
+    if (o < 0) continue;
+    if (first_real < 0) first_real = r;
     const double *st = data + offsets[o];
     if (r < h->n_myo) {
       int k2 = 0;
@@ -505,7 +639,12 @@ int set_states_impl(vg_handle *h, const int64_t *offsets, const double *data) {
       for (int k = 0; k < PK_NSTATE; k++) pk[(size_t)k * npa + (r - h->n_myo)] = st[k];
     }
   }
-  // Padding lanes copy a real node's states so every lane computes finite values.
+  // Padding rows and lanes copy a real node's states so every lane computes
+  // finite values.
+  if (h->n_myo > 0 && first_real >= 0)
+    for (int64_t r = 0; r < h->n_myo; r++)
+      if (h->new2old[r] < 0)
+        for (int k = 0; k < nd; k++) soa[(size_t)k * na + r] = soa[(size_t)k * na + first_real];
   if (h->n_myo > 0)
     for (int k = 0; k < nd; k++)
       for (int64_t r = h->n_myo; r < na; r++) soa[(size_t)k * na + r] = soa[(size_t)k * na];
@@ -551,8 +690,9 @@ int get_states_impl(vg_handle *h, const int64_t *offsets, double *data, const do
   if (rc) return rc;
   rc = h->tsz == 8 ? get_soa<double>(h, h->pk_states, pk) : get_soa<float>(h, h->pk_states, pk);
   if (rc) return rc;
-  for (int64_t r = 0; r < h->n; r++) {
+  for (int64_t r = 0; r < h->rows; r++) {
     const int64_t o = h->new2old[r];
+    if (o < 0) continue;
     double *st = data + offsets[o];
     if (r < h->n_myo) {
       int k2 = 0;
@@ -699,18 +839,50 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, e);
   }
-  // Row order: myocardium first, then the Purkinje segment, each in mesh order.
+  // Row order: myocardium first, then the Purkinje segment, each in mesh
+  // order.  Batched instances: instance after instance, each padded with
+  // node-less rows to a whole number of slices.
   std::vector<int32_t> base;
+  std::vector<int64_t> inst_row0(1, 0);
   base.reserve(csr.n);
   h->purkinje.assign(csr.n, 0);
   if (mesh->tissue)
     for (int64_t i = 0; i < csr.n; i++) h->purkinje[i] = (mesh->tissue[i] == 11 || mesh->tissue[i] == 12);
-  for (int64_t i = 0; i < csr.n; i++)
-    if (!h->purkinje[i]) base.push_back((int32_t)i);
-  h->n_myo = (int64_t)base.size();
-  for (int64_t i = 0; i < csr.n; i++)
-    if (h->purkinje[i]) base.push_back((int32_t)i);
-  h->n_pk = csr.n - h->n_myo;
+  if (model->n_instances > 1) {
+    const int K = model->n_instances;
+    std::string bad;
+    if (!model->instance) bad = "instance is NULL";
+    for (int64_t i = 0; i < csr.n && bad.empty(); i++) {
+      if (model->instance[i] < 0 || model->instance[i] >= K) bad = "instance id out of range";
+      else if (h->purkinje[i]) bad = "batched instances cannot hold Purkinje nodes";
+    }
+    for (int64_t e = 0; e < mesh->n_elems && bad.empty(); e++) {
+      const int32_t *c = mesh->conn + 4 * e;
+      for (int q = 1; q < mesh->elem_nn[e]; q++)
+        if (model->instance[c[q]] != model->instance[c[0]]) bad = "an element spans two instances";
+    }
+    if (!bad.empty()) {
+      *out = h.release();
+      return (*out)->fail(VG_ERR_ARG, bad);
+    }
+    std::vector<std::vector<int32_t>> members(K);
+    for (int64_t i = 0; i < csr.n; i++) members[model->instance[i]].push_back((int32_t)i);
+    for (int k = 0; k < K; k++) {
+      base.insert(base.end(), members[k].begin(), members[k].end());
+      while (base.size() % SELL_C) base.push_back(-1);
+      inst_row0.push_back((int64_t)base.size());
+    }
+    h->n_inst = K;
+    h->n_myo = (int64_t)base.size();
+  } else {
+    for (int64_t i = 0; i < csr.n; i++)
+      if (!h->purkinje[i]) base.push_back((int32_t)i);
+    h->n_myo = (int64_t)base.size();
+    for (int64_t i = 0; i < csr.n; i++)
+      if (h->purkinje[i]) base.push_back((int32_t)i);
+    inst_row0.push_back((int64_t)base.size());
+  }
+  h->n_pk = (int64_t)base.size() - h->n_myo;
   h->n_pk_alloc = ((h->n_pk + BLK - 1) / BLK) * BLK;
   HostSell sell;
   e = build_sell(csr, h->dt, h->Cm, model->sell_sigma, base, h->n_myo, sell);
@@ -719,10 +891,11 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     return (*out)->fail(VG_ERR_ARG, e);
   }
   h->n = csr.n;
+  h->rows = sell.rows;
   h->nnz = (int64_t)csr.col.size();
   h->n_slices = sell.n_slices;
   h->padded = sell.padded;
-  h->n_alloc = ((h->n + BLK - 1) / BLK) * BLK;
+  h->n_alloc = ((h->rows + BLK - 1) / BLK) * BLK;
   h->new2old = sell.new2old;
   h->old2new = sell.old2new;
   h->CL = csr.CL;
@@ -782,8 +955,36 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
                                      : std::min<int64_t>(1024, (H_->n_slices + WAVES - 1) / WAVES);
     g = std::max<int64_t>(XCDS, ((g + XCDS - 1) / XCDS) * XCDS);
     if (g > 65536) return H_->fail(VG_ERR_ARG, "cg_blocks above 65536");
-    H_->cg_blocks = (unsigned)g;
+    // Blocks per instance in proportion to its slices (at least one, at most
+    // one per slice); instance k owns blocks [inst_blk[k], inst_blk[k+1]).
+    const int K = H_->n_inst;
+    std::vector<int> inst_blk(K + 1, 0), blk_inst;
+    std::vector<int64_t> inst_slice(K + 1);
+    for (int k = 0; k <= K; k++) inst_slice[k] = inst_row0[k] / SELL_C;
+    if (K == 1) inst_slice[1] = H_->n_slices;
+    for (int k = 0; k < K; k++) {
+      const int64_t ns = inst_slice[k + 1] - inst_slice[k];
+      int64_t gk = K == 1 ? g : (g * ns + H_->n_slices / 2) / H_->n_slices;
+      gk = std::max<int64_t>(1, K == 1 ? gk : std::min<int64_t>(gk, ns));
+      inst_blk[k + 1] = inst_blk[k] + (int)gk;
+      blk_inst.insert(blk_inst.end(), gk, k);
+    }
+    if (inst_blk[K] > 65536) return H_->fail(VG_ERR_ARG, "too many CG blocks");
+    H_->cg_blocks = (unsigned)inst_blk[K];
     H_->node_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, na / BLK));
+    HIPCHK(H_, H_->blk_inst.alloc(blk_inst.size() * 4));
+    HIPCHK(H_, hipMemcpy(H_->blk_inst.p, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(H_, H_->inst_blk.alloc(inst_blk.size() * 4));
+    HIPCHK(H_, hipMemcpy(H_->inst_blk.p, inst_blk.data(), inst_blk.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(H_, H_->inst_slice.alloc(inst_slice.size() * 8));
+    HIPCHK(H_, hipMemcpy(H_->inst_slice.p, inst_slice.data(), inst_slice.size() * 8, hipMemcpyHostToDevice));
+    if (K > 1) {
+      std::vector<int> si(H_->n_slices);
+      for (int k = 0; k < K; k++)
+        for (int64_t q = inst_slice[k]; q < inst_slice[k + 1]; q++) si[q] = k;
+      HIPCHK(H_, H_->slice_inst.alloc(si.size() * 4));
+      HIPCHK(H_, hipMemcpy(H_->slice_inst.p, si.data(), si.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   // Vectors.
   for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at,
@@ -807,13 +1008,19 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   // and the result record.
   HIPCHK(H_, H_->part.alloc((size_t)H_->cg_blocks * 6 * 8));
   HIPCHK(H_, hipMemset(H_->part.p, 0, (size_t)H_->cg_blocks * 6 * 8));
-  HIPCHK(H_, H_->carry.alloc(2 * sizeof(CgCarry)));
-  HIPCHK(H_, hipMemset(H_->carry.p, 0, 2 * sizeof(CgCarry)));
-  HIPCHK(H_, H_->res.alloc(sizeof(CgResult)));
-  HIPCHK(H_, hipMemset(H_->res.p, 0, sizeof(CgResult)));
+  HIPCHK(H_, H_->carry.alloc(2 * sizeof(CgCarry) * H_->n_inst));
+  HIPCHK(H_, hipMemset(H_->carry.p, 0, 2 * sizeof(CgCarry) * H_->n_inst));
+  HIPCHK(H_, H_->res.alloc(sizeof(CgResult) * H_->n_inst));
+  HIPCHK(H_, hipMemset(H_->res.p, 0, sizeof(CgResult) * H_->n_inst));
   HIPCHK(H_, H_->newton.alloc(sizeof(unsigned long long)));
   HIPCHK(H_, hipMemset(H_->newton.p, 0, sizeof(unsigned long long)));
-  HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult), hipHostMallocDefault));
+  HIPCHK(H_, H_->step_ctr.alloc(sizeof(int64_t)));
+  HIPCHK(H_, hipMemset(H_->step_ctr.p, 0, sizeof(int64_t)));
+  HIPCHK(H_, H_->totals.alloc(sizeof(CgTotals)));
+  HIPCHK(H_, hipMemset(H_->totals.p, 0, sizeof(CgTotals)));
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_tot, sizeof(CgTotals), hipHostMallocDefault));
+  H_->graph = model->graph ? 1 : 0;
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult) * H_->n_inst, hipHostMallocDefault));
   // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi;
   // Mahajan: one set per tissue type holding G_tof and G_Ks).
   if (H_->model == VG_MODEL_MJ) {
@@ -829,8 +1036,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
       if ((rc = upload_raw<float>(H_, H_->scales, sc))) return rc;
     }
     std::vector<uint16_t> pi(na, 0);
-    for (int64_t r = 0; r < H_->n; r++) pi[r] = (uint16_t)mj_type(mesh->tissue, H_->new2old[r]);
-    for (int64_t r = H_->n; r < na; r++) pi[r] = pi[0];
+    for (int64_t r = 0; r < H_->rows; r++)
+      pi[r] = H_->new2old[r] >= 0 ? (uint16_t)mj_type(mesh->tissue, H_->new2old[r]) : 0;
     HIPCHK(H_, H_->pidx.alloc(na * 2));
     HIPCHK(H_, hipMemcpy(H_->pidx.p, pi.data(), na * 2, hipMemcpyHostToDevice));
   } else {
@@ -848,7 +1055,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     }
     if (model->param_index) {
       std::vector<uint16_t> pi(na, 0);
-      for (int64_t r = 0; r < H_->n; r++) {
+      for (int64_t r = 0; r < H_->rows; r++) {
+        if (H_->new2old[r] < 0) continue;
         const uint16_t v = model->param_index[H_->new2old[r]];
         if (v >= nsets) return H_->fail(VG_ERR_ARG, "param_index out of range");
         pi[r] = v;
@@ -972,8 +1180,11 @@ int vg_clear_stimulus(vg_handle *h) {
 
 int vg_step(vg_handle *h, int nsteps) {
   if (!h || nsteps < 0) return VG_ERR_ARG;
+  const bool graph = h->graph && !h->profiling;
   for (int k = 0; k < nsteps; k++) {
-    const int rc = h->tsz == 8 ? step_t<double>(h) : step_t<float>(h);
+    int rc;
+    if (graph) rc = h->tsz == 8 ? step_graph<double>(h) : step_graph<float>(h);
+    else rc = h->tsz == 8 ? step_t<double>(h) : step_t<float>(h);
     if (rc) return rc;
   }
   const hipError_t e = hipGetLastError();
@@ -1077,7 +1288,8 @@ extern "C" int vg_ecg_setup(vg_handle *h, const vg_mesh *mesh, const double *el,
   const int64_t na = h->n_alloc;
   std::vector<double> Hp((size_t)ne * na, 0.0);
   for (int k = 0; k < ne; k++)
-    for (int64_t r = 0; r < h->n; r++) Hp[(size_t)k * na + r] = Hm[(size_t)k * h->n + h->new2old[r]];
+    for (int64_t r = 0; r < h->rows; r++)
+      if (h->new2old[r] >= 0) Hp[(size_t)k * na + r] = Hm[(size_t)k * h->n + h->new2old[r]];
   int rc = h->tsz == 8 ? upload_raw<double>(h, h->H, Hp) : upload_raw<float>(h, h->H, Hp);
   if (rc) return rc;
   if (!h->ecg_out.p) HIPCHK(h, h->ecg_out.alloc(MAX_ECG * 8));
@@ -1094,10 +1306,10 @@ int vg_ecg(vg_handle *h, double *out) {
   double *part = h->ecg_part.as<double>();
   if (h->tsz == 8)
     hipLaunchKernelGGL(k_ecg<double>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
-                       h->Va.as<double>(), h->n, h->H.as<double>(), h->ne, h->n_alloc, part);
+                       h->Va.as<double>(), h->rows, h->H.as<double>(), h->ne, h->n_alloc, part);
   else
     hipLaunchKernelGGL(k_ecg<float>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
-                       h->Va.as<float>(), h->n, h->H.as<float>(), h->ne, h->n_alloc, part);
+                       h->Va.as<float>(), h->rows, h->H.as<float>(), h->ne, h->n_alloc, part);
   launch_ecg_finish(h, h->ecg_out.as<double>());
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->ecg_out.p, h->ne * 8, hipMemcpyDeviceToHost, h->stream));
@@ -1150,10 +1362,13 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   unsigned long long nw = 0;
   HIPCHK(h, hipMemcpy(&nw, h->newton.p, sizeof(nw), hipMemcpyDeviceToHost));
+  CgTotals tot;
+  HIPCHK(h, hipMemcpy(&tot, h->totals.p, sizeof(CgTotals), hipMemcpyDeviceToHost));
+  // Host-driven solves are counted on the host, captured ones on the device.
   o->steps = h->steps;
-  o->cg_iterations = h->cg_iters;
-  o->cg_solves = h->cg_solves;
-  o->cg_not_converged = h->cg_notconv;
+  o->cg_iterations = h->cg_iters + (int64_t)tot.iters;
+  o->cg_solves = h->cg_solves + (int64_t)tot.solves;
+  o->cg_not_converged = h->cg_notconv + (int64_t)tot.notconv;
   o->newton_iterations = (int64_t)nw;
   o->n_nodes = h->n;
   o->nnz = h->nnz;

@@ -240,8 +240,10 @@ std::string assemble_csr(const vg_mesh &m, const vg_model &mod, HostCsr &out) {
 std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
                        const std::vector<int32_t> &base, int64_t n_first, HostSell &o) {
   const int64_t n = c.n;
+  const int64_t rows = (int64_t)base.size();
   o.n = n;
-  o.n_slices = (n + SELL_C - 1) / SELL_C;
+  o.rows = rows;
+  o.n_slices = (rows + SELL_C - 1) / SELL_C;
   const int64_t n_pad = o.n_slices * SELL_C;
   std::vector<double> d(n, 0.0);
   for (int64_t i = 0; i < n; i++) {
@@ -249,7 +251,19 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
       if (c.col[k] == i) d[i] = dt / 4.0 * c.K[k] + Cm * c.M[k];
     if (!(d[i] > 0.0)) return "non-positive diagonal of the CN operator at node " + std::to_string(i);
   }
-  if ((int64_t)base.size() != n || n_first < 0 || n_first > n) return "bad row order";
+  if (rows < n || n_first < 0 || n_first > rows) return "bad row order";
+  {
+    std::vector<char> seen(n, 0);
+    int64_t real = 0;
+    for (int32_t b : base) {
+      if (b < 0) continue;
+      if (b >= n || seen[b]) return "bad row order";
+      seen[b] = 1;
+      real++;
+    }
+    if (real != n) return "bad row order";
+    if (rows > n && sigma > 1) return "row-length sorting is not available with instance padding";
+  }
   o.new2old = base;
   if (sigma > 1) {
     const int64_t seg[3] = {0, n_first, n};
@@ -262,14 +276,15 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
       }
   }
   o.old2new.assign(n, 0);
-  for (int64_t i = 0; i < n; i++) o.old2new[o.new2old[i]] = (int32_t)i;
+  for (int64_t i = 0; i < rows; i++)
+    if (o.new2old[i] >= 0) o.old2new[o.new2old[i]] = (int32_t)i;
   o.width.assign(o.n_slices, 0);
   o.offset.assign(o.n_slices + 1, 0);
   for (int64_t s = 0; s < o.n_slices; s++) {
     int w = 0;
     for (int l = 0; l < SELL_C; l++) {
       const int64_t r = s * SELL_C + l;
-      if (r < n) {
+      if (r < rows && o.new2old[r] >= 0) {
         const int32_t old = o.new2old[r];
         w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]));
       }
@@ -283,16 +298,18 @@ std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
   o.K_rhs.assign(o.padded, 0.0);
   o.sinv.assign(n_pad, 0.0);
   o.diag.assign(n_pad, 0.0);
-  for (int64_t r = 0; r < n; r++) {
-    o.sinv[r] = 1.0 / std::sqrt(d[o.new2old[r]]);
-    o.diag[r] = d[o.new2old[r]];
+  // Padding rows (no node) carry D = 1 and no entries: they stay zero.
+  for (int64_t r = 0; r < rows; r++) {
+    const int32_t old = o.new2old[r];
+    o.sinv[r] = old >= 0 ? 1.0 / std::sqrt(d[old]) : 1.0;
+    o.diag[r] = old >= 0 ? d[old] : 1.0;
   }
   std::vector<std::pair<int32_t, int64_t>> row;
   for (int64_t s = 0; s < o.n_slices; s++) {
     for (int l = 0; l < SELL_C; l++) {
       const int64_t r = s * SELL_C + l;
       row.clear();
-      if (r < n) {
+      if (r < rows && o.new2old[r] >= 0) {
         const int32_t old = o.new2old[r];
         for (int64_t k = c.row_ptr[old]; k < c.row_ptr[old + 1]; k++)
           row.emplace_back(o.old2new[c.col[k]], k);

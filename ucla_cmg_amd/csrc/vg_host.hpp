@@ -28,7 +28,7 @@ std::string validate_mesh(const vg_mesh &mesh);
 // int32 in col32 from wide[s] (wide[s] = -1 for a 16-bit slice).
 constexpr int SELL_C = 64;
 struct HostSell {
-  int64_t n = 0, n_slices = 0, padded = 0;
+  int64_t n = 0, rows = 0, n_slices = 0, padded = 0;  // rows: n plus padding rows
   std::vector<int64_t> offset;   // [n_slices + 1]
   std::vector<int32_t> width;    // [n_slices]
   std::vector<int32_t> col;      // [padded] absolute columns (host copy)
@@ -43,9 +43,10 @@ struct HostSell {
 };
 
 // Builds the scaled CN operators in SELL-64 form.  `base` lists the mesh
-// nodes in the solver's row order (rows [0, n_first) and [n_first, n) form two
-// segments); rows are reordered by descending length inside windows of
-// `sigma` rows of one segment when sigma > 1.
+// nodes in the solver's row order, -1 marking a padding row (no node, no
+// entries); rows [0, n_first) and [n_first, rows) form two segments, and rows
+// are reordered by descending length inside windows of `sigma` rows of one
+// segment when sigma > 1 (not with padding rows).
 std::string build_sell(const HostCsr &c, double dt, double Cm, int sigma,
                        const std::vector<int32_t> &base, int64_t n_first, HostSell &out);
 

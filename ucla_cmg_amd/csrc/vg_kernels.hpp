@@ -52,10 +52,19 @@ struct CgResult {
   double rho, rho_ref;
 };
 
+// A launch may solve several independent systems (instances: separate
+// bodies batched into one handle, each slice-aligned).  The blocks of
+// instance k are [inst_blk[k], inst_blk[k+1]) and its slices
+// [inst_slice[k], inst_slice[k+1]); every block reduces only its instance's
+// partials, so each instance runs its own CG to its own convergence.
 struct CgBufs {
-  double *part0, *part1;  // [3][grid] block partials of alternating launches
-  CgCarry *carry;         // [2]
-  CgResult *res;
+  double *part0, *part1;      // [3][grid] block partials of alternating launches
+  CgCarry *carry;             // [2][n_inst]
+  CgResult *res;              // [n_inst]
+  const int *blk_inst;        // [grid]
+  const int *inst_blk;        // [n_inst + 1]
+  const int64_t *inst_slice;  // [n_inst + 1]
+  int n_inst;
 };
 
 __device__ inline double wave_sum(double v) {
@@ -83,16 +92,17 @@ __device__ inline void put_partials(const double (&v)[NV], double *__restrict__ 
   }
 }
 
-// Totals of the partials that a previous launch of nblk blocks stored; the
-// same value in every thread of every block.
+// Totals over blocks [b0, b1) of the partials that a previous launch of nblk
+// blocks stored; the same value in every thread of every block that asks.
 template <int NV>
-__device__ inline void sum_partials(const double *__restrict__ part, int nblk, double (&tot)[NV]) {
+__device__ inline void sum_partials(const double *__restrict__ part, int nblk, int b0, int b1,
+                                    double (&tot)[NV]) {
   __shared__ double sm[WAVES][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += BLK) s += part[(size_t)k * nblk + b];
+    for (int b = b0 + threadIdx.x; b < b1; b += BLK) s += part[(size_t)k * nblk + b];
     s = wave_sum(s);
     if (lane == 0) sm[wv][k] = s;
   }
@@ -106,14 +116,20 @@ __device__ inline void sum_partials(const double *__restrict__ part, int nblk, d
   }
 }
 
-// Slices [lo, hi) of this block.  The slices are cut into gridDim.x (a
-// multiple of XCDS) contiguous chunks, and the chunks of the blocks one XCD
+// Instance of this block, its block range [b0, b1) and its slices [lo, hi):
+// the instance's slices cut into b1 - b0 contiguous chunks.  With a single
+// instance over a multiple of XCDS blocks the chunks of the blocks one XCD
 // runs are adjacent, so the rows an XCD gathers from mostly sit in its own L2.
-__device__ inline void block_slices(int64_t n_slices, int64_t &lo, int64_t &hi) {
-  const int64_t G = gridDim.x, b = blockIdx.x;
-  const int64_t c = (b % XCDS) * (G / XCDS) + b / XCDS;
-  lo = n_slices * c / G;
-  hi = n_slices * (c + 1) / G;
+__device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int &b0, int &b1) {
+  const int b = blockIdx.x, k = cb.blk_inst[b];
+  b0 = cb.inst_blk[k];
+  b1 = cb.inst_blk[k + 1];
+  const int64_t g = b1 - b0, s0 = cb.inst_slice[k], ns = cb.inst_slice[k + 1] - s0;
+  int64_t l = b - b0;
+  if (cb.n_inst == 1 && g % XCDS == 0) l = (l % XCDS) * (g / XCDS) + l / XCDS;
+  lo = s0 + ns * l / g;
+  hi = s0 + ns * (l + 1) / g;
+  return k;
 }
 
 // SELL-64 operator pattern (values are separate arrays on the same layout).
@@ -250,17 +266,19 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
                                              const T *__restrict__ Ahat,
                                              const T *__restrict__ V, const T *__restrict__ x,
                                              T *__restrict__ r, const T *__restrict__ diag,
-                                             int64_t n_slices, CgBufs cb) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    cb.res->iter = 0;
-    cb.res->done = 0;
-    cb.res->notconv = 0;
-    cb.res->x_valid = 0;
-    cb.res->rho = cb.res->rho_ref = 0.0;
-    __hip_atomic_store(&cb.res->stop, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+                                             CgBufs cb) {
   int64_t lo, hi;
-  block_slices(n_slices, lo, hi);
+  int b0, b1;
+  const int inst = block_work(cb, lo, hi, b0, b1);
+  if (blockIdx.x == b0 && threadIdx.x == 0) {
+    CgResult *res = cb.res + inst;
+    res->iter = 0;
+    res->done = 0;
+    res->notconv = 0;
+    res->x_valid = 0;
+    res->rho = res->rho_ref = 0.0;
+    __hip_atomic_store(&res->stop, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   const int lane = threadIdx.x & 63;
   double v[2] = {0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
@@ -291,23 +309,24 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
 template <class T>
 __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ val,
                                                   const T *__restrict__ r, T *__restrict__ w,
-                                                  const T *__restrict__ diag, int64_t n_slices,
-                                                  CgBufs cb) {
+                                                  const T *__restrict__ diag, CgBufs cb) {
+  int64_t lo, hi;
+  int b0, b1;
+  const int inst = block_work(cb, lo, hi, b0, b1);
+  CgResult *res = cb.res + inst;
   double t0[2];
-  sum_partials<2>(cb.part1, gridDim.x, t0);
+  sum_partials<2>(cb.part1, gridDim.x, b0, b1, t0);
   const double rho_ref = t0[0];
-  const bool b0 = blockIdx.x == 0 && threadIdx.x == 0;
+  const bool first = blockIdx.x == b0 && threadIdx.x == 0;
   if (!(rho_ref > 0.0) || !isfinite(rho_ref)) {
-    if (b0) {
-      cb.res->rho_ref = cb.res->rho = rho_ref;
-      cb.res->done = 1;
-      cb.res->notconv = rho_ref == 0.0 ? 0 : 1;
-      cg_set_stop(cb.res);
+    if (first) {
+      res->rho_ref = res->rho = rho_ref;
+      res->done = 1;
+      res->notconv = rho_ref == 0.0 ? 0 : 1;
+      cg_set_stop(res);
     }
     return;
   }
-  int64_t lo, hi;
-  block_slices(n_slices, lo, hi);
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
@@ -320,10 +339,10 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
     v[2] += (double)diag[row] * rr * rr;
   }
   put_partials<3>(v, cb.part0);
-  if (b0) {
-    cb.carry[0] = CgCarry{rho_ref, 0.0, 0.0};
-    cb.res->rho_ref = rho_ref;
-    cb.res->rho = t0[1];
+  if (first) {
+    cb.carry[inst] = CgCarry{rho_ref, 0.0, 0.0};
+    res->rho_ref = rho_ref;
+    res->rho = t0[1];
   }
 }
 
@@ -350,13 +369,17 @@ template <class T>
 __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ val, T *rA, T *rB, T *wA,
                                                  T *wB, T *sA, T *sB, T *__restrict__ p,
                                                  T *__restrict__ x, const T *__restrict__ diag,
-                                                 int64_t n_slices, CgBufs cb, int j, double tol2,
-                                                 int maxit, int guess) {
-  if (cg_stopped(cb.res)) return;
+                                                 CgBufs cb, int j, double tol2, int maxit,
+                                                 int guess) {
+  CgResult *res = cb.res + cb.blk_inst[blockIdx.x];
+  if (cg_stopped(res)) return;
+  int64_t lo, hi;
+  int b0, b1;
+  const int inst = block_work(cb, lo, hi, b0, b1);
   const int par = j & 1;
   double t[3];
-  sum_partials<3>(par ? cb.part1 : cb.part0, gridDim.x, t);
-  const CgCarry c = cb.carry[par];
+  sum_partials<3>(par ? cb.part1 : cb.part0, gridDim.x, b0, b1, t);
+  const CgCarry c = cb.carry[par * cb.n_inst + inst];
   const double gamma = t[0], delta = t[1], rho = t[2];
   int end = 0;  // 1: converged, 2: not converged
   double alpha = 0.0, beta = 0.0;
@@ -374,21 +397,19 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
     if (!(alpha > 0.0) || !isfinite(alpha)) end = 2;  // breakdown
   }
   if (end) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      cb.res->iter = j;
-      cb.res->done = 1;
-      cb.res->notconv = end == 2 ? 1 : 0;
-      cb.res->x_valid = (j > 0 || guess) ? 1 : 0;
-      cb.res->rho = rho;
-      cg_set_stop(cb.res);
+    if (blockIdx.x == b0 && threadIdx.x == 0) {
+      res->iter = j;
+      res->done = 1;
+      res->notconv = end == 2 ? 1 : 0;
+      res->x_valid = (j > 0 || guess) ? 1 : 0;
+      res->rho = rho;
+      cg_set_stop(res);
     }
     return;
   }
   const T *r0 = par ? rB : rA, *w0 = par ? wB : wA, *s0 = par ? sB : sA;
   T *r1 = par ? rA : rB, *w1 = par ? wA : wB, *s1 = par ? sA : sB;
   const T a = (T)alpha, bt = (T)beta;
-  int64_t lo, hi;
-  block_slices(n_slices, lo, hi);
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
@@ -418,9 +439,9 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
     v[2] += (double)diag[row] * rd * rd;
   }
   put_partials<3>(v, par ? cb.part0 : cb.part1);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    cb.carry[par ^ 1] = CgCarry{c.rho_ref, gamma, alpha};
-    cb.res->iter = j + 1;
+  if (blockIdx.x == b0 && threadIdx.x == 0) {
+    cb.carry[(par ^ 1) * cb.n_inst + inst] = CgCarry{c.rho_ref, gamma, alpha};
+    res->iter = j + 1;
   }
 }
 
@@ -442,12 +463,13 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, SubSteps sub,
                                                IonicArgs a, unsigned long long *newton_sum,
-                                               const CgResult *cg) {
+                                               const CgResult *cg,
+                                               const int *__restrict__ slice_inst) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   int newton = 0;
-  // x holds the half-step increment unless that increment is zero.
-  const bool use_x = x && cg->x_valid;
   if (i < a.n) {
+    // x holds the half-step increment unless that increment is zero.
+    const bool use_x = x && cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid;
     T V = Vin[i];
     if (use_x) V += sinv[i] * x[i];
     const T stim = istim ? istim[i] : T(0);
@@ -526,12 +548,13 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
                                                   const T *__restrict__ sinv, T *__restrict__ Vout,
                                                   T *__restrict__ st, const T *__restrict__ istim,
                                                   int64_t row0, int64_t n, int64_t n_alloc,
-                                                  SubSteps sub, const CgResult *cg) {
+                                                  SubSteps sub, const CgResult *cg,
+                                                  const int *__restrict__ slice_inst) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const int64_t row = row0 + i;
   T V = Vin[row];
-  if (x && cg->x_valid) V += sinv[row] * x[row];
+  if (x && cg[slice_inst ? slice_inst[row >> 6] : 0].x_valid) V += sinv[row] * x[row];
   const T stim = istim ? istim[row] : T(0);
   T s[PK_NSTATE];
 #pragma unroll
@@ -548,18 +571,22 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
 // ---- second half-step apply: V_a = V_b + D^-1/2 x, activation, ECG ---------
 // Grid-stride over nodes; with ECG each block stores MAX_ECG partials of
 // sum_n H[e][n] V_n, finished by k_ecg_finish.
+// The step's start time is step_ctr[0] * dt (the counter k_fold advances).
 template <class T, bool AT, bool ECG>
 __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T *__restrict__ x,
                                                const T *__restrict__ sinv, T *__restrict__ Va,
-                                               int64_t n, T *__restrict__ at, T thr, double t0,
+                                               int64_t n, T *__restrict__ at, T thr,
+                                               const int64_t *__restrict__ step_ctr,
                                                double dt, const T *__restrict__ H, int ne,
                                                int64_t n_alloc, double *__restrict__ part,
-                                               const CgResult *cg) {
-  const bool use_x = cg->x_valid != 0;
+                                               const CgResult *cg,
+                                               const int *__restrict__ slice_inst) {
+  const double t0 = (double)step_ctr[0] * dt;
   double v[MAX_ECG];
 #pragma unroll
   for (int e = 0; e < MAX_ECG; e++) v[e] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const bool use_x = cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid != 0;
     const T Vn = use_x ? Vb[i] + sinv[i] * x[i] : Vb[i];
     if constexpr (AT) {
       const T Vo = Va[i];
@@ -574,6 +601,33 @@ __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T
     }
   }
   if constexpr (ECG) put_partials<MAX_ECG>(v, part);
+}
+
+// Running totals of the solves of a captured step (read by the host lazily).
+struct CgTotals {
+  unsigned long long iters, solves, notconv, nonfinite;
+  int max_iter[2];
+  double max_relres;
+};
+
+// One thread: folds the result records of the solve of half-step `which`
+// into the totals (fold != 0) and, with tick != 0, advances the step counter.
+__global__ void k_fold(const CgResult *__restrict__ res, int n_inst, CgTotals *tot, int which,
+                       int fold, int64_t *step_ctr, int tick) {
+  if (threadIdx.x != 0) return;
+  if (fold) {
+    for (int k = 0; k < n_inst; k++) {
+      const CgResult c = res[k];
+      tot->iters += (unsigned long long)c.iter;
+      tot->solves += 1;
+      tot->notconv += (unsigned long long)(c.done == 0 || c.notconv != 0);
+      if (!isfinite(c.rho) || !isfinite(c.rho_ref)) tot->nonfinite += 1;
+      if (c.iter > tot->max_iter[which]) tot->max_iter[which] = c.iter;
+      const double rel = c.rho_ref > 0.0 ? sqrt(c.rho / c.rho_ref) : 0.0;
+      if (rel > tot->max_relres) tot->max_relres = rel;
+    }
+  }
+  if (tick) step_ctr[0] += 1;
 }
 
 // ECG of V alone (partials as k_apply's).
@@ -597,7 +651,7 @@ __global__ __launch_bounds__(BLK) void k_ecg(const T *__restrict__ V, int64_t n,
 __global__ __launch_bounds__(BLK) void k_ecg_finish(const double *__restrict__ part, int nblk,
                                                     int ne, double *__restrict__ out) {
   double tot[MAX_ECG];
-  sum_partials<MAX_ECG>(part, nblk, tot);
+  sum_partials<MAX_ECG>(part, nblk, 0, nblk, tot);
   if (threadIdx.x == 0)
     for (int e = 0; e < ne; e++) out[e] = tot[e];
 }

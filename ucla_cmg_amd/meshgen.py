@@ -336,3 +336,60 @@ def purkinje_tree(mesh: Mesh, generations=8, seg_len=0.06, seed=1, offset=0.02,
                 conn=np.vstack([mesh.conn, conn_b]), fibers=fibers, tissue=tissue,
                 pmj_node=(n_myo + term).astype(np.int32), pmj_data=pmj_data,
                 radius=mesh.radius, meta=meta)
+
+
+def scale_cross_fiber(mesh: Mesh, ratio: float) -> Mesh:
+    """Copy of ``mesh`` whose sheet and normal fiber vectors are scaled by
+    1/sqrt(ratio): with D = (D_l, D_l, D_l) the element tensor
+    sum_p D_p f_p f_p^T (CardiacBody.cc:153-168, which uses the nine fiber
+    numbers as given) becomes D_l f f^T + (D_l/ratio)(s s^T + n n^T), an
+    anisotropy ratio sigma_l/sigma_t = ratio."""
+    fib = np.array(mesh.fibers, dtype=np.float64, copy=True).reshape(-1, 9)
+    fib[:, 3:] /= np.sqrt(ratio)
+    return Mesh(xyz=mesh.xyz, elem_nn=mesh.elem_nn, conn=mesh.conn, fibers=fib,
+                tissue=mesh.tissue, meta=dict(mesh.meta, anisotropy=ratio))
+
+
+def concat(meshes) -> tuple:
+    """Disjoint union of tet meshes: (mesh, instance id per node, node offset
+    of each part).  Parts are placed 1 cm apart along x; no element is
+    shared, so each part is an independent body."""
+    xyz, elem_nn, conn, fib, tissue, inst, offs = [], [], [], [], [], [], []
+    base, x0 = 0, 0.0
+    for k, m in enumerate(meshes):
+        p = np.array(m.xyz, dtype=np.float64, copy=True)
+        p[:, 0] += x0 - p[:, 0].min()
+        x0 = p[:, 0].max() + 1.0
+        c = np.array(m.conn, dtype=np.int64, copy=True)
+        c[c >= 0] += base
+        xyz.append(p)
+        elem_nn.append(m.elem_nn)
+        conn.append(c.astype(np.int32))
+        fib.append(np.asarray(m.fibers, dtype=np.float64).reshape(-1, 9))
+        tissue.append(np.zeros(m.n_nodes, np.int8) if m.tissue is None else m.tissue)
+        inst.append(np.full(m.n_nodes, k, np.int32))
+        offs.append(base)
+        base += m.n_nodes
+    mesh = Mesh(xyz=np.vstack(xyz), elem_nn=np.concatenate(elem_nn), conn=np.vstack(conn),
+                fibers=np.vstack(fib), tissue=np.concatenate(tissue),
+                meta={"kind": "batch", "parts": len(meshes)})
+    return mesh, np.concatenate(inst), np.array(offs + [base], np.int64)
+
+
+def slab_sweep(count=96, n_min=5000, n_max=250000, nz=9, h=0.025, seed=4):
+    """The batched conductance/anisotropy sweep: ``count`` slabs with node
+    counts log-uniform in [n_min, n_max] (nx = ny, nz layers), each with its
+    own G_Na, G_Kr, G_CaL scalings ~ U[0.5, 1.5] and anisotropy ratio
+    ~ U[2, 10].  Returns (list of meshes, scales [count, 5] in the ten
+    Tusscher scaling order gNa gKr gKs gCaL gto, ratios)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), count))
+    meshes, scales, ratios = [], [], []
+    for n in sizes:
+        side = max(4, int(round(np.sqrt(n / nz))))
+        ratio = rng.uniform(2.0, 10.0)
+        meshes.append(scale_cross_fiber(slab(side, side, nz, h), ratio))
+        g = rng.uniform(0.5, 1.5, 3)
+        scales.append([g[0], g[1], 1.0, g[2], 1.0])
+        ratios.append(ratio)
+    return meshes, np.array(scales), np.array(ratios)

@@ -73,7 +73,8 @@ class VgModel(C.Structure):
         ("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("quirks", C.c_uint),
         ("n_param_sets", C.c_int), ("param_scales", C.c_void_p),
         ("param_index", C.c_void_p), ("sell_sigma", C.c_int), ("cg_blocks", C.c_int),
-        ("cg_x0", C.c_int),
+        ("cg_x0", C.c_int), ("n_instances", C.c_int), ("instance", C.c_void_p),
+        ("graph", C.c_int),
     ]
 
 
@@ -188,7 +189,8 @@ class _MeshView:
 def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERENCE,
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
-               param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO):
+               param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO, instance=None,
+               graph=0):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -216,6 +218,12 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
     m.sell_sigma = sell_sigma
     m.cg_blocks = cg_blocks
     m.cg_x0 = cg_x0
+    m.graph = graph
+    if instance is not None:
+        ins = np.ascontiguousarray(instance, np.int32)
+        keep.append(ins)
+        m.n_instances = int(ins.max()) + 1 if ins.size else 0
+        m.instance = _p(ins)
     return m, keep
 
 
