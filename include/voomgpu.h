@@ -191,6 +191,14 @@ int vg_clear_stimulus(vg_handle *h);
 int vg_step(vg_handle *h, int nsteps);
 int vg_synchronize(vg_handle *h);
 
+/* Launch shape, changeable between steps (an autotuner times candidates):
+ * the total CG grid (rounded up to a multiple of 8; 0 = default) and the
+ * captured-step mode of vg_model.graph.  No reference counterpart: the
+ * reference has no launch shapes. */
+int vg_set_cg_blocks(vg_handle *h, int cg_blocks);
+int vg_get_cg_blocks(vg_handle *h, int *cg_blocks);
+int vg_set_graph(vg_handle *h, int enable);
+
 int vg_get_voltage(vg_handle *h, double *V);
 int vg_set_voltage(vg_handle *h, const double *V);
 /* Uniform layout [n_nodes * nvar] (nvar of the body's model); only for

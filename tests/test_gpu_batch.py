@@ -174,3 +174,39 @@ def test_captured_step_with_purkinje_stream(gpu_device):
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
     assert (a[1][n:] >= 0).any()
+
+
+def test_launch_shape_changes_mid_run_keep_parity(gpu_device):
+    """Re-gridding the CG and switching the captured step on and off between
+    steps (what the autotuner does) leaves an FP64 run on the oracle."""
+    parts, mesh, inst, offs, stim = _batch(DIMS[:2], RATIOS[:2], SCALES[:2])
+    dt = 0.02
+    gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_FP64,
+                           integrator=vg.INTEG_REFERENCE, dt=dt, D=(DL, DL, DL), cg_tol=1e-10,
+                           param_scales=SCALES[:2], param_index=inst.astype(np.uint16),
+                           instance=inst)
+    refs = [oracle.Simulation(p, oracle.MODEL_TT, dt=dt, D=(DL, DL, DL), cg_tol=1e-10,
+                              tt_const=_tt_const(SCALES[k]), param_idx=np.zeros(p.n_nodes, np.uint16))
+            for k, p in enumerate(parts)]
+    amp = 50.0 * 2000.0
+    gpu.setStimulusCurrent(stim, amp)
+    for k, r in enumerate(refs):
+        r.istim[meshgen.slab_corner_nodes(*DIMS[k], 3)] = amp
+    shapes = [(0, False), (24, True), (8, False), (64, True), (0, True)]
+    for s in range(250):
+        if s % 50 == 0:
+            g, graph = shapes[s // 50]
+            gpu.setLaunch(cg_blocks=g, graph=graph)
+        if s == 100:
+            gpu.clearStimulus()
+            for r in refs:
+                r.istim[:] = 0.0
+        gpu.Solve(1)
+        for r in refs:
+            r.step()
+    V = gpu.getVoltage()
+    for k, r in enumerate(refs):
+        err = np.abs(V[offs[k]:offs[k + 1]] - r.V).max()
+        print(f"instance {k}: max|dV| {err:.2e} mV")
+        assert err < 1e-6
+    assert gpu.stats()["cg_not_converged"] == 0

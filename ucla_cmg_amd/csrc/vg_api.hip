@@ -34,6 +34,10 @@ struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
   hipError_t alloc(size_t b) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
     bytes = b;
     return hipMalloc(&p, b ? b : 1);
   }
@@ -135,6 +139,7 @@ struct vg_handle {
   unsigned cg_blocks = 0, node_blocks = 0;
   DevBuf part, carry, res, newton;
   DevBuf blk_inst, inst_blk, inst_slice, slice_inst;  // instance tables (CgBufs)
+  std::vector<int64_t> inst_slice_host;               // [n_inst + 1]
   DevBuf H, ecg_part, ecg_out, ecg_frames;
   CgResult *h_res = nullptr;  // pinned
   int nsets = 1;
@@ -704,6 +709,40 @@ int get_states_impl(vg_handle *h, const int64_t *offsets, double *data, const do
   return VG_OK;
 }
 
+// CG grid: g blocks in all (0 = one per 4 slices up to 1024; a multiple of
+// XCDS), split between instances in proportion to their slices (at least
+// one, at most one per slice); instance k owns blocks [inst_blk[k],
+// inst_blk[k+1]).  Reallocates the partials and drops a captured step.
+int set_cg_blocks(vg_handle *h, int cg_blocks) {
+  int64_t g = cg_blocks > 0 ? cg_blocks : std::min<int64_t>(1024, (h->n_slices + WAVES - 1) / WAVES);
+  g = std::max<int64_t>(XCDS, ((g + XCDS - 1) / XCDS) * XCDS);
+  if (g > 65536) return h->fail(VG_ERR_ARG, "cg_blocks above 65536");
+  const int K = h->n_inst;
+  const std::vector<int64_t> &isl = h->inst_slice_host;
+  std::vector<int> inst_blk(K + 1, 0), blk_inst;
+  for (int k = 0; k < K; k++) {
+    const int64_t ns = isl[k + 1] - isl[k];
+    int64_t gk = K == 1 ? g : (g * ns + h->n_slices / 2) / h->n_slices;
+    gk = std::max<int64_t>(1, K == 1 ? gk : std::min<int64_t>(gk, ns));
+    inst_blk[k + 1] = inst_blk[k] + (int)gk;
+    blk_inst.insert(blk_inst.end(), gk, k);
+  }
+  if (inst_blk[K] > 65536) return h->fail(VG_ERR_ARG, "too many CG blocks");
+  if (h->stream) HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->cg_blocks = (unsigned)inst_blk[K];
+  HIPCHK(h, h->blk_inst.alloc(blk_inst.size() * 4));
+  HIPCHK(h, hipMemcpy(h->blk_inst.p, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, h->inst_blk.alloc(inst_blk.size() * 4));
+  HIPCHK(h, hipMemcpy(h->inst_blk.p, inst_blk.data(), inst_blk.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, h->part.alloc((size_t)h->cg_blocks * 6 * 8));
+  HIPCHK(h, hipMemset(h->part.p, 0, (size_t)h->cg_blocks * 6 * 8));
+  if (h->gexec) {
+    HIPCHK(h, hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
+  return VG_OK;
+}
+
 // Device rows of independent cells, rows in state order without the V slot;
 // Mahajan keeps its gate rows in T and its accumulator rows in FP64.
 int cells_put_rows(vg_cells *c, const std::vector<double> &soa) {
@@ -946,42 +985,20 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     if ((rc = upload_raw<float>(H_, H_->sinv, sinv_pad))) return rc;
     if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
   }
-  // Grids.  SpMV-type launches: one block per 4 slices up to 1024 blocks (4
-  // per CU; waves then take several slices each), a multiple of XCDS; the
-  // caller may set it.  Node-wise launches: one thread per node up to 1024
-  // blocks, grid-stride beyond.
+  // Grids: the CG block tables (set_cg_blocks) and the node-wise grid.
   {
-    int64_t g = model->cg_blocks > 0 ? model->cg_blocks
-                                     : std::min<int64_t>(1024, (H_->n_slices + WAVES - 1) / WAVES);
-    g = std::max<int64_t>(XCDS, ((g + XCDS - 1) / XCDS) * XCDS);
-    if (g > 65536) return H_->fail(VG_ERR_ARG, "cg_blocks above 65536");
-    // Blocks per instance in proportion to its slices (at least one, at most
-    // one per slice); instance k owns blocks [inst_blk[k], inst_blk[k+1]).
-    const int K = H_->n_inst;
-    std::vector<int> inst_blk(K + 1, 0), blk_inst;
-    std::vector<int64_t> inst_slice(K + 1);
-    for (int k = 0; k <= K; k++) inst_slice[k] = inst_row0[k] / SELL_C;
-    if (K == 1) inst_slice[1] = H_->n_slices;
-    for (int k = 0; k < K; k++) {
-      const int64_t ns = inst_slice[k + 1] - inst_slice[k];
-      int64_t gk = K == 1 ? g : (g * ns + H_->n_slices / 2) / H_->n_slices;
-      gk = std::max<int64_t>(1, K == 1 ? gk : std::min<int64_t>(gk, ns));
-      inst_blk[k + 1] = inst_blk[k] + (int)gk;
-      blk_inst.insert(blk_inst.end(), gk, k);
-    }
-    if (inst_blk[K] > 65536) return H_->fail(VG_ERR_ARG, "too many CG blocks");
-    H_->cg_blocks = (unsigned)inst_blk[K];
+    H_->inst_slice_host.assign(H_->n_inst + 1, 0);
+    for (int k = 0; k <= H_->n_inst; k++) H_->inst_slice_host[k] = inst_row0[k] / SELL_C;
+    if (H_->n_inst == 1) H_->inst_slice_host[1] = H_->n_slices;
+    if ((rc = set_cg_blocks(H_, model->cg_blocks))) return rc;
     H_->node_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, na / BLK));
-    HIPCHK(H_, H_->blk_inst.alloc(blk_inst.size() * 4));
-    HIPCHK(H_, hipMemcpy(H_->blk_inst.p, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(H_, H_->inst_blk.alloc(inst_blk.size() * 4));
-    HIPCHK(H_, hipMemcpy(H_->inst_blk.p, inst_blk.data(), inst_blk.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(H_, H_->inst_slice.alloc(inst_slice.size() * 8));
-    HIPCHK(H_, hipMemcpy(H_->inst_slice.p, inst_slice.data(), inst_slice.size() * 8, hipMemcpyHostToDevice));
-    if (K > 1) {
+    HIPCHK(H_, H_->inst_slice.alloc(H_->inst_slice_host.size() * 8));
+    HIPCHK(H_, hipMemcpy(H_->inst_slice.p, H_->inst_slice_host.data(),
+                         H_->inst_slice_host.size() * 8, hipMemcpyHostToDevice));
+    if (H_->n_inst > 1) {
       std::vector<int> si(H_->n_slices);
-      for (int k = 0; k < K; k++)
-        for (int64_t q = inst_slice[k]; q < inst_slice[k + 1]; q++) si[q] = k;
+      for (int k = 0; k < H_->n_inst; k++)
+        for (int64_t q = H_->inst_slice_host[k]; q < H_->inst_slice_host[k + 1]; q++) si[q] = k;
       HIPCHK(H_, H_->slice_inst.alloc(si.size() * 4));
       HIPCHK(H_, hipMemcpy(H_->slice_inst.p, si.data(), si.size() * 4, hipMemcpyHostToDevice));
     }
@@ -1006,8 +1023,6 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   }
   // CG scratch: two sets of 3 FP64 partials per block, the carried scalars
   // and the result record.
-  HIPCHK(H_, H_->part.alloc((size_t)H_->cg_blocks * 6 * 8));
-  HIPCHK(H_, hipMemset(H_->part.p, 0, (size_t)H_->cg_blocks * 6 * 8));
   HIPCHK(H_, H_->carry.alloc(2 * sizeof(CgCarry) * H_->n_inst));
   HIPCHK(H_, hipMemset(H_->carry.p, 0, 2 * sizeof(CgCarry) * H_->n_inst));
   HIPCHK(H_, H_->res.alloc(sizeof(CgResult) * H_->n_inst));
@@ -1189,6 +1204,23 @@ int vg_step(vg_handle *h, int nsteps) {
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return h->hipfail(e, "kernel launch");
+  return VG_OK;
+}
+
+int vg_set_cg_blocks(vg_handle *h, int cg_blocks) {
+  if (!h || cg_blocks < 0) return VG_ERR_ARG;
+  return set_cg_blocks(h, cg_blocks);
+}
+
+int vg_get_cg_blocks(vg_handle *h, int *cg_blocks) {
+  if (!h || !cg_blocks) return VG_ERR_ARG;
+  *cg_blocks = (int)h->cg_blocks;
+  return VG_OK;
+}
+
+int vg_set_graph(vg_handle *h, int enable) {
+  if (!h) return VG_ERR_ARG;
+  h->graph = enable ? 1 : 0;
   return VG_OK;
 }
 

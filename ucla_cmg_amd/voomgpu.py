@@ -46,6 +46,7 @@ EXPORTED = [
     "vg_version", "vg_create", "vg_destroy", "vg_last_error", "vg_get_stream",
     "vg_set_stimulus_nodes", "vg_set_stimulus_sphere", "vg_set_stimulus_axis",
     "vg_set_stimulus_plane", "vg_clear_stimulus", "vg_step", "vg_synchronize",
+    "vg_set_cg_blocks", "vg_get_cg_blocks", "vg_set_graph",
     "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
     "vg_get_states_ragged", "vg_set_states_ragged",
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
@@ -115,6 +116,9 @@ def load_library(path: str = None):
         "vg_clear_stimulus": ([H], C.c_int),
         "vg_step": ([H, C.c_int], C.c_int),
         "vg_synchronize": ([H], C.c_int),
+        "vg_set_cg_blocks": ([H, C.c_int], C.c_int),
+        "vg_get_cg_blocks": ([H, C.POINTER(C.c_int)], C.c_int),
+        "vg_set_graph": ([H, C.c_int], C.c_int),
         "vg_get_voltage": ([H, C.c_void_p], C.c_int),
         "vg_set_voltage": ([H, C.c_void_p], C.c_int),
         "vg_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
@@ -316,6 +320,18 @@ class CardiacSolver:
 
     def synchronize(self):
         self._chk(self.L.vg_synchronize(self.h), "synchronize")
+
+    def setLaunch(self, cg_blocks: Optional[int] = None, graph: Optional[bool] = None):
+        """Launch shape between steps: total CG grid and captured-step mode."""
+        if cg_blocks is not None:
+            self._chk(self.L.vg_set_cg_blocks(self.h, int(cg_blocks)), "set_cg_blocks")
+        if graph is not None:
+            self._chk(self.L.vg_set_graph(self.h, int(bool(graph))), "set_graph")
+
+    def cgBlocks(self) -> int:
+        g = C.c_int()
+        self._chk(self.L.vg_get_cg_blocks(self.h, C.byref(g)), "get_cg_blocks")
+        return g.value
 
     def getVoltage(self) -> np.ndarray:
         v = np.zeros(self.n)
