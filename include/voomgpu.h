@@ -253,6 +253,58 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V,
 int vg_cells_get_states(vg_cells *c, int *nvar, double *states);
 int vg_cells_set_states(vg_cells *c, int nvar, const double *states);
 
+/* ---- learned ten Tusscher ionic step ------------------------------------
+ * No counterpart in the reference: a small MLP (25 inputs: V and the 18
+ * other tT states at the start of the ionic update, log dt, the five
+ * conductance scalings; 4 hidden layers of 128, SiLU; 19 state increments)
+ * trained on the GPU with BF16 MFMA kernels on samples recorded from the
+ * plain Rush-Larsen model in tissue, and run in place of that model per node
+ * while the node's inputs lie inside the training envelope (the plain model
+ * runs the others). */
+typedef struct vg_surr vg_surr;
+int vg_surr_create(uint64_t seed, vg_surr **out);
+void vg_surr_destroy(vg_surr *s);
+const char *vg_surr_last_error(const vg_surr *s);
+int vg_surr_reserve(vg_surr *s, int64_t capacity);
+/* x[n][25] raw inputs, y[n][19] raw increments */
+int vg_surr_add_samples(vg_surr *s, const float *x, const float *y, int64_t n);
+int vg_surr_num_samples(vg_surr *s, int64_t *n);
+int vg_surr_get_samples(vg_surr *s, int64_t first, int64_t n, float *x, float *y);
+/* Runs nsteps steps of h (a ten Tusscher body with the Rush-Larsen
+ * integrator), recording every `every`-th step the ionic update of every
+ * node_stride-th myocardium row into s's samples. */
+int vg_surr_collect(vg_handle *h, vg_surr *s, int nsteps, int every, int node_stride);
+/* Input normalisation x' = (x - mu_x) / sd_x and input envelope [lo, hi];
+ * output transform t = (asinh(y / c_y) - mu_y) / sd_y and output envelope
+ * [t_lo, t_hi].  vg_surr_fit_scaling takes them from the samples: c_y a
+ * tenth of the mean |y|, envelopes the samples' ranges widened by
+ * envelope_margin of themselves.  A node is left to the plain model when an
+ * input or a network output falls outside its envelope. */
+typedef struct vg_surr_scaling {
+  float mu_x[25], sd_x[25], lo[25], hi[25];
+  float mu_y[19], sd_y[19], c_y[19], t_lo[19], t_hi[19];
+} vg_surr_scaling;
+int vg_surr_fit_scaling(vg_surr *s, double envelope_margin);
+int vg_surr_get_scaling(vg_surr *s, vg_surr_scaling *out);
+int vg_surr_set_scaling(vg_surr *s, const vg_surr_scaling *in);
+int vg_surr_num_params(int64_t *n);
+/* per layer W[k][n] row-major then b[n]; layers 25->128->128->128->128->19 */
+int vg_surr_get_params(vg_surr *s, float *p);
+int vg_surr_set_params(vg_surr *s, const float *p);
+/* mean squared error of the transformed outputs t and its gradient (n a
+ * multiple of 128) */
+int vg_surr_loss_grad(vg_surr *s, const float *x, const float *y, int n, double *loss,
+                      float *grad);
+int vg_surr_predict(vg_surr *s, const float *x, int n, float *y);
+int vg_surr_train(vg_surr *s, int iters, int batch, double lr, double *loss);
+/* Run h's ten Tusscher ionic update through s (NULL: the plain model);
+ * s must outlive the attachment.  Weights, biases and scaling are copied at
+ * attach, one copy per surrogate shared by the handles it is attached to:
+ * attach again after training further. */
+int vg_surr_attach(vg_handle *h, vg_surr *s);
+/* node updates done by the network and by the plain model since attach */
+int vg_surr_usage(vg_handle *h, int64_t *network, int64_t *fallback);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,6 +155,16 @@ struct vg_handle {
   int glaunch[2] = {0, 0};
   bool g_stim = false, g_at = false;
   unsigned long long g_notconv_seen = 0;
+  // Learned ionic step: the attached network, its device view, the
+  // per-row fallback flags and the usage counters (two per wave of the
+  // persistent grid).  rec_*: recording of the plain update for training.
+  vg_surr *surr = nullptr;
+  SurrDevice sdev{};
+  DevBuf surr_fb, surr_use;
+  unsigned surr_grid = 0;
+  float *rec_x = nullptr, *rec_y = nullptr;
+  int64_t rec_base = 0, rec_cap = 0;
+  int rec_stride = 1;
   // activation
   bool at_enabled = false;
   double at_thr = -40.0;
@@ -363,6 +373,25 @@ template <class T, int MODEL, int INTEG>
 void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   IonicArgs a{h->n_myo, h->n_alloc, h->nsets, h->Cm, 500, (h->quirks & VG_QUIRK_NEWTON_JT) ? 1 : 0,
               (h->quirks & VG_QUIRK_MJ_INT_ABS) ? 1 : 0};
+  a.rec_x = h->rec_x;
+  a.rec_y = h->rec_y;
+  a.rec_base = h->rec_base;
+  a.rec_cap = h->rec_cap;
+  a.rec_stride = h->rec_stride;
+  a.log_dt = (float)std::log(h->dt);
+  a.only = nullptr;
+  if constexpr (MODEL == VG_MODEL_TT) if (h->surr && !h->rec_x) {
+    // The network advances the rows inside its envelope and flags the rest,
+    // which the plain kernel below then updates.
+    hipLaunchKernelGGL(k_ionic_surr<T>, dim3(h->surr_grid), dim3(BLK), 0, h->stream, Vin, x,
+                       h->sinv.as<T>(), Vout, h->states.as<T>(),
+                       h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
+                       h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
+                       h->scales.as<T>(), h->n_myo, h->n_alloc, a.log_dt, h->sdev,
+                       h->surr_fb.as<unsigned char>(), h->surr_use.as<unsigned long long>(),
+                       h->res.as<CgResult>(), slice_inst_of(h));
+    a.only = h->surr_fb.as<unsigned char>();
+  }
   hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3((unsigned)((h->n_myo + BLK - 1) / BLK)),
                      dim3(BLK), 0, h->stream, Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
                      h->acc.p ? h->acc.as<double>() : (double *)nullptr,
@@ -1386,6 +1415,77 @@ int vg_profile_get(vg_handle *h, int kc, double *total_ms, int64_t *launches) {
   drain_profile(h);
   if (total_ms) *total_ms = h->prof[kc].total_ms;
   if (launches) *launches = h->prof[kc].launches;
+  return VG_OK;
+}
+
+int vg_surr_collect(vg_handle *h, vg_surr *s, int nsteps, int every, int node_stride) {
+  if (!h || !s) return VG_ERR_ARG;
+  if (h->model != VG_MODEL_TT) return h->fail(VG_ERR_ARG, "surrogate data needs a ten Tusscher body");
+  if (nsteps < 0 || every < 1 || node_stride < 1) return h->fail(VG_ERR_ARG, "bad collect arguments");
+  float *X = nullptr, *Y = nullptr;
+  int64_t count = 0, cap = 0;
+  if (!surr_dataset(s, &X, &Y, &count, &cap))
+    return h->fail(VG_ERR_ARG, "surrogate has no sample storage (vg_surr_reserve)");
+  const int64_t per = (h->n_myo + node_stride - 1) / node_stride;
+  for (int q = 0; q < nsteps; q++) {
+    int rc = upload_stim(h);
+    if (rc) return rc;
+    // Steps with a stimulus are not recorded: the network has no stimulus input.
+    const bool rec = (q % every) == 0 && !h->stim_active && count < cap && h->n_myo > 0;
+    if (rec) {
+      h->rec_x = X;
+      h->rec_y = Y;
+      h->rec_base = count;
+      h->rec_cap = cap;
+      h->rec_stride = node_stride;
+    }
+    rc = h->tsz == 8 ? step_t<double>(h) : step_t<float>(h);
+    h->rec_x = h->rec_y = nullptr;
+    if (rc) return rc;
+    if (rec) count = std::min(cap, count + per);
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  surr_set_count(s, count);
+  return VG_OK;
+}
+
+int vg_surr_attach(vg_handle *h, vg_surr *s) {
+  if (!h) return VG_ERR_ARG;
+  if (s && h->model != VG_MODEL_TT)
+    return h->fail(VG_ERR_ARG, "the learned ionic step is a ten Tusscher model");
+  if (s) {
+    SurrDevice d{};
+    if (!surr_device(s, &d)) return h->fail(VG_ERR_ARG, "surrogate has no fitted scaling");
+    int cus = 0;
+    HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int64_t tiles = (h->n_myo + 31) / 32;
+    h->surr_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cus, (tiles + WAVES - 1) / WAVES));
+    HIPCHK(h, h->surr_fb.alloc((size_t)h->n_alloc));
+    HIPCHK(h, h->surr_use.alloc((size_t)h->surr_grid * WAVES * 2 * 8));
+    HIPCHK(h, hipMemsetAsync(h->surr_use.p, 0, (size_t)h->surr_grid * WAVES * 2 * 8, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->sdev = d;
+  }
+  h->surr = s;
+  // The captured step holds the ionic launches: capture again.
+  if (h->gexec) {
+    HIPCHK(h, hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
+  return VG_OK;
+}
+
+int vg_surr_usage(vg_handle *h, int64_t *network, int64_t *fallback) {
+  if (!h || !network || !fallback) return VG_ERR_ARG;
+  *network = *fallback = 0;
+  if (!h->surr) return VG_OK;
+  std::vector<unsigned long long> u((size_t)h->surr_grid * WAVES * 2);
+  HIPCHK(h, hipMemcpyAsync(u.data(), h->surr_use.p, u.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < u.size(); k += 2) {
+    *network += (int64_t)u[k];
+    *fallback += (int64_t)u[k + 1];
+  }
   return VG_OK;
 }
 

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "vg_ionic.hpp"
+#include "vg_surrogate.hpp"
 
 namespace vg {
 
@@ -453,6 +454,15 @@ struct IonicArgs {
   int max_newton;
   int newton_jt;
   int mj_int_abs;
+  // ten Tusscher: record the update of every rec_stride-th row into samples
+  // rec_base + row / rec_stride of rec_x[.][25], rec_y[.][19] (off when
+  // rec_x is null; samples at or past rec_cap are dropped).
+  float *rec_x, *rec_y;
+  int64_t rec_base, rec_cap;
+  int rec_stride;
+  float log_dt;
+  // Update only the rows flagged here (the learned step's fallback), or all.
+  const unsigned char *only;
 };
 
 template <class T, int MODEL, int INTEG>
@@ -467,7 +477,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
                                                const int *__restrict__ slice_inst) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   int newton = 0;
-  if (i < a.n) {
+  if (i < a.n && (!a.only || a.only[i])) {
     // x holds the half-step increment unless that increment is zero.
     const bool use_x = x && cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid;
     T V = Vin[i];
@@ -514,6 +524,20 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       for (int k = 1; k < TT_NSTATE; k++) s[k] = st[(k - 1) * a.n_alloc + i];
       const TTG<T> g = tt_conductances<T>(sc);
       const T it = -stim / (Cm * T(TT_XI));
+      float *rx = nullptr, *ry = nullptr;
+      if (a.rec_x && i % a.rec_stride == 0) {
+        const int64_t k = a.rec_base + i / a.rec_stride;
+        if (k < a.rec_cap) {
+          rx = a.rec_x + k * SG_NIN;
+          ry = a.rec_y + k * SG_NOUT;
+          rx[0] = (float)V;
+#pragma unroll
+          for (int q = 1; q < TT_NSTATE; q++) rx[q] = (float)s[q];
+          rx[19] = a.log_dt;
+#pragma unroll
+          for (int q = 0; q < VG_TT_NSCALE; q++) rx[20 + q] = (float)sc[q];
+        }
+      }
       for (int q = 0; q < sub.n; q++) {
         const T dt = (T)sub.dt[q];
         T dv;
@@ -525,6 +549,14 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
           dv = tt_step_rl<T, sizeof(T) == 4>(s, V, it, dt, g);
         }
         V += dt * dv;
+      }
+      if (ry) {
+        // Increments from the unrounded start values (still in memory).
+        T V0 = Vin[i];
+        if (use_x) V0 += sinv[i] * x[i];
+        ry[0] = (float)(V - V0);
+#pragma unroll
+        for (int q = 1; q < TT_NSTATE; q++) ry[q] = (float)(s[q] - st[(q - 1) * a.n_alloc + i]);
       }
 #pragma unroll
       for (int k = 1; k < TT_NSTATE; k++) st[(k - 1) * a.n_alloc + i] = s[k];
@@ -566,6 +598,157 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
 #pragma unroll
   for (int k = 0; k < PK_NSTATE; k++) st[k * n_alloc + i] = s[k];
   Vout[row] = V;
+}
+
+// ---- learned ten Tusscher ionic update ---------------------------------------
+// Lanes of one wave exchange activations through the wave's own LDS buffer.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One hidden or output layer for the wave's 32 rows: A (the wave's activation
+// rows, BF16, SG_ACT_LD apart) times the layer's transposed weights in LDS,
+// NT output tiles of 32 columns accumulated in FP32.
+template <int KS, int NT>
+__device__ inline void surr_layer(const __bf16 *A, const __bf16 *W, int ldw, int r, int kq,
+                                  f32x16 (&acc)[4]) {
+  bf16x8 a[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) a[ks] = *reinterpret_cast<const bf16x8 *>(A + r * SG_ACT_LD + ks * 16 + kq);
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    acc[nt] = f32x16{};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(W + (nt * 32 + r) * ldw + ks * 16 + kq);
+      acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], b, acc[nt], 0, 0, 0);
+    }
+  }
+}
+
+// Bias + SiLU of a hidden layer's outputs back into the activation rows.
+__device__ inline void surr_hidden_out(__bf16 *A, const f32x16 (&acc)[4], const float *b, int l,
+                                       int r) {
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    const int col = nt * 32 + r;
+    const float bc = b[col];
+#pragma unroll
+    for (int j = 0; j < 16; j++) A[mfma_row(l, j) * SG_ACT_LD + col] = (__bf16)sg_silu(acc[nt][j] + bc);
+  }
+}
+
+// ten Tusscher ionic update through the learned step for rows [0, n): a row
+// whose 25 inputs and 19 outputs lie inside the training envelopes and that
+// has no stimulus gets V and its 18 states advanced by the network's
+// increments (gates and R' clamped to [0, 1]); the others are flagged in
+// `fallback` for the plain kernel.  Persistent blocks keep all weights in LDS; each wave takes tiles
+// of 32 rows, lane r and r + 32 sharing row r (the MFMA operand layout).
+// usage[2 * (block * 4 + wave) + {0, 1}] counts network / fallback rows.
+template <class T>
+__global__ __launch_bounds__(BLK) void k_ionic_surr(
+    const T *__restrict__ Vin, const T *__restrict__ x, const T *__restrict__ sinv,
+    T *__restrict__ Vout, T *__restrict__ st, const T *__restrict__ istim,
+    const uint16_t *__restrict__ pidx, const T *__restrict__ scales, int64_t n, int64_t n_alloc,
+    float log_dt, SurrDevice sd, unsigned char *__restrict__ fallback,
+    unsigned long long *__restrict__ usage, const CgResult *cg, const int *__restrict__ slice_inst) {
+  __shared__ __attribute__((aligned(16))) __bf16 wl[SG_WT_TOTAL];
+  __shared__ __attribute__((aligned(16))) __bf16 actb[WAVES][32 * SG_ACT_LD];
+  for (int q = threadIdx.x; q < SG_WT_TOTAL / 8; q += BLK)
+    reinterpret_cast<bf16x8 *>(wl)[q] = reinterpret_cast<const bf16x8 *>(sd.wt)[q];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 31, kq = (l >> 5) * 8;
+  __bf16 *A = actb[wave];
+  float *Y = reinterpret_cast<float *>(A);  // output staging [32][33]
+  unsigned long long n_net = 0, n_fb = 0;
+  const int64_t ntiles = (n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile < ntiles;
+       tile += (int64_t)gridDim.x * WAVES) {
+    const int64_t i = tile * 32 + r;
+    const bool mine = l < 32 && i < n;
+    T s[TT_NSTATE];
+    bool ok = false;
+    if (mine) {
+      T V = Vin[i];
+      if (x && cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid) V += sinv[i] * x[i];
+      s[0] = V;
+#pragma unroll
+      for (int k = 1; k < TT_NSTATE; k++) s[k] = st[(k - 1) * n_alloc + i];
+      const T *sc = scales + (pidx ? (int)pidx[i] : 0) * VG_TT_NSCALE;
+      float f[SG_NIN];
+#pragma unroll
+      for (int k = 0; k < TT_NSTATE; k++) f[k] = (float)s[k];
+      f[19] = log_dt;
+#pragma unroll
+      for (int k = 0; k < VG_TT_NSCALE; k++) f[20 + k] = (float)sc[k];
+      ok = !(istim && istim[i] != T(0));
+#pragma unroll
+      for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= sd.lo[k] && f[k] <= sd.hi[k];
+#pragma unroll
+      for (int k = 0; k < SG_KIN; k++)
+        A[r * SG_ACT_LD + k] = (__bf16)(ok && k < SG_NIN ? (f[k] - sd.mu_x[k]) / sd.sd_x[k] : 0.0f);
+      fallback[i] = ok ? 0 : 1;
+    } else if (l < 32) {
+#pragma unroll
+      for (int k = 0; k < SG_KIN; k++) A[r * SG_ACT_LD + k] = (__bf16)0.0f;
+    }
+    wave_lds_sync();
+    f32x16 acc[4];
+    surr_layer<2, 4>(A, wl + sg_wt_off(0), sg_wt_ld(0), r, kq, acc);
+    wave_lds_sync();
+    surr_hidden_out(A, acc, sd.b[0], l, r);
+    wave_lds_sync();
+#pragma unroll 1
+    for (int L = 1; L < SG_LAYERS - 1; L++) {
+      surr_layer<8, 4>(A, wl + sg_wt_off(L), sg_wt_ld(L), r, kq, acc);
+      wave_lds_sync();
+      surr_hidden_out(A, acc, sd.b[L], l, r);
+      wave_lds_sync();
+    }
+    surr_layer<8, 1>(A, wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, acc);
+    wave_lds_sync();
+    if (r < SG_NOUT) {
+      const float bc = sd.b[SG_LAYERS - 1][r];
+#pragma unroll
+      for (int j = 0; j < 16; j++) Y[mfma_row(l, j) * 33 + r] = acc[0][j] + bc;
+    }
+    wave_lds_sync();
+    if (mine && ok) {
+      // Outputs outside their envelope (or not finite) leave the node to
+      // the plain model too.
+      float y[SG_NOUT];
+#pragma unroll
+      for (int k = 0; k < SG_NOUT; k++) {
+        const float t = Y[r * 33 + k];
+        ok = ok && t >= sd.t_lo[k] && t <= sd.t_hi[k];
+        y[k] = sg_from_target(t, sd.mu_y[k], sd.sd_y[k], sd.c_y[k]);
+      }
+      if (ok) {
+        Vout[i] = s[0] + (T)y[0];
+#pragma unroll
+        for (int k = 1; k < TT_NSTATE; k++) {
+          T v = s[k] + (T)y[k];
+          if ((k >= 4 && k <= 9) || (k >= 11 && k <= 16) || k == 18) v = fmin(fmax(v, T(0)), T(1));
+          st[(k - 1) * n_alloc + i] = v;
+        }
+        n_net++;
+      } else {
+        fallback[i] = 1;
+      }
+    }
+    if (mine && !ok) n_fb++;
+    wave_lds_sync();
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    n_net += __shfl_xor(n_net, o, 64);
+    n_fb += __shfl_xor(n_fb, o, 64);
+  }
+  if (l == 0) {
+    usage[2 * (blockIdx.x * WAVES + wave)] += n_net;
+    usage[2 * (blockIdx.x * WAVES + wave) + 1] += n_fb;
+  }
 }
 
 // ---- second half-step apply: V_a = V_b + D^-1/2 x, activation, ECG ---------

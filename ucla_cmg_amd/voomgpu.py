@@ -54,7 +54,17 @@ EXPORTED = [
     "vg_profile_get", "vg_stats_get", "vg_assemble_csr", "vg_cells_create",
     "vg_cells_destroy", "vg_cells_set_scales", "vg_cells_set_quirks", "vg_cells_compute_ion",
     "vg_cells_get_states", "vg_cells_set_states",
+    "vg_surr_create", "vg_surr_destroy", "vg_surr_last_error", "vg_surr_reserve",
+    "vg_surr_add_samples", "vg_surr_num_samples", "vg_surr_get_samples", "vg_surr_collect",
+    "vg_surr_fit_scaling", "vg_surr_get_scaling", "vg_surr_set_scaling", "vg_surr_num_params",
+    "vg_surr_get_params", "vg_surr_set_params", "vg_surr_loss_grad", "vg_surr_predict",
+    "vg_surr_train", "vg_surr_attach", "vg_surr_usage",
 ]
+
+# Learned ten Tusscher step: inputs V, 18 states, log(dt), 5 scalings;
+# outputs the 19 increments; 4 hidden layers of 128.
+SURR_NIN, SURR_NOUT, SURR_HID = 25, 19, 128
+SURR_LAYERS = [(25, 128), (128, 128), (128, 128), (128, 128), (128, 19)]
 
 
 class VgMesh(C.Structure):
@@ -146,6 +156,25 @@ def load_library(path: str = None):
                                   C.c_void_p], C.c_int),
         "vg_cells_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
         "vg_cells_set_states": ([H, C.c_int, C.c_void_p], C.c_int),
+        "vg_surr_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+        "vg_surr_destroy": ([H], None),
+        "vg_surr_last_error": ([H], C.c_char_p),
+        "vg_surr_reserve": ([H, C.c_int64], C.c_int),
+        "vg_surr_add_samples": ([H, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
+        "vg_surr_num_samples": ([H, C.POINTER(C.c_int64)], C.c_int),
+        "vg_surr_get_samples": ([H, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
+        "vg_surr_collect": ([H, H, C.c_int, C.c_int, C.c_int], C.c_int),
+        "vg_surr_fit_scaling": ([H, C.c_double], C.c_int),
+        "vg_surr_get_scaling": ([H, C.c_void_p], C.c_int),
+        "vg_surr_set_scaling": ([H, C.c_void_p], C.c_int),
+        "vg_surr_num_params": ([C.POINTER(C.c_int64)], C.c_int),
+        "vg_surr_get_params": ([H, C.c_void_p], C.c_int),
+        "vg_surr_set_params": ([H, C.c_void_p], C.c_int),
+        "vg_surr_loss_grad": ([H, C.c_void_p, C.c_void_p, C.c_int, dp, C.c_void_p], C.c_int),
+        "vg_surr_predict": ([H, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+        "vg_surr_train": ([H, C.c_int, C.c_int, C.c_double, dp], C.c_int),
+        "vg_surr_attach": ([H, H], C.c_int),
+        "vg_surr_usage": ([H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -484,3 +513,145 @@ class IonicCells:
         rc = self.L.vg_cells_set_states(self.h, st.shape[1], _p(st))
         if rc != VG_OK:
             raise VoomError(f"vg_cells_set_states failed ({rc})")
+
+
+class VgSurrScaling(C.Structure):
+    _fields_ = [(k, C.c_float * (SURR_NIN if k in ("mu_x", "sd_x", "lo", "hi") else SURR_NOUT))
+                for k in ("mu_x", "sd_x", "lo", "hi", "mu_y", "sd_y", "c_y", "t_lo", "t_hi")]
+
+
+class Surrogate:
+    """Learned ten Tusscher ionic step (an MLP trained on the device with BF16
+    MFMA kernels) and its sample store.  Attached to a solver, it advances
+    every myocardium node whose inputs lie inside the training envelope; the
+    plain model updates the rest."""
+
+
+    def __init__(self, seed: int = 1):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.vg_surr_create(seed, C.byref(h))
+        if rc != VG_OK:
+            raise VoomError(f"vg_surr_create failed ({rc})")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vg_surr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != VG_OK:
+            raise VoomError(f"{what}: {self.L.vg_surr_last_error(self.h).decode()} ({rc})")
+
+    def reserve(self, capacity: int):
+        self._chk(self.L.vg_surr_reserve(self.h, int(capacity)), "reserve")
+
+    def add_samples(self, x, y):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, SURR_NIN)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1, SURR_NOUT)
+        assert x.shape[0] == y.shape[0]
+        self._chk(self.L.vg_surr_add_samples(self.h, _p(x), _p(y), x.shape[0]), "add_samples")
+
+    def num_samples(self) -> int:
+        n = C.c_int64()
+        self._chk(self.L.vg_surr_num_samples(self.h, C.byref(n)), "num_samples")
+        return n.value
+
+    def get_samples(self, first: int = 0, n: int = None):
+        n = self.num_samples() - first if n is None else n
+        x = np.zeros((n, SURR_NIN), np.float32)
+        y = np.zeros((n, SURR_NOUT), np.float32)
+        self._chk(self.L.vg_surr_get_samples(self.h, first, n, _p(x), _p(y)), "get_samples")
+        return x, y
+
+    def collect(self, solver, nsteps: int, every: int = 1, node_stride: int = 1):
+        """Runs nsteps steps of `solver`, recording the plain ionic update of
+        every node_stride-th node on every `every`-th unstimulated step."""
+        rc = self.L.vg_surr_collect(solver.h, self.h, nsteps, every, node_stride)
+        solver._chk(rc, "surr_collect")
+
+    def fit_scaling(self, margin: float = 0.0):
+        self._chk(self.L.vg_surr_fit_scaling(self.h, margin), "fit_scaling")
+
+    def get_scaling(self) -> dict:
+        sc = VgSurrScaling()
+        self._chk(self.L.vg_surr_get_scaling(self.h, C.byref(sc)), "get_scaling")
+        return {k: np.array(getattr(sc, k), np.float32) for k, _ in VgSurrScaling._fields_}
+
+    def set_scaling(self, d: dict):
+        sc = VgSurrScaling()
+        for k, _ in VgSurrScaling._fields_:
+            v = np.ascontiguousarray(d[k], np.float32)
+            getattr(sc, k)[:] = list(v)
+        self._chk(self.L.vg_surr_set_scaling(self.h, C.byref(sc)), "set_scaling")
+
+    @staticmethod
+    def num_params() -> int:
+        L = load_library()
+        n = C.c_int64()
+        L.vg_surr_num_params(C.byref(n))
+        return n.value
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params(), np.float32)
+        self._chk(self.L.vg_surr_get_params(self.h, _p(p)), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params()
+        self._chk(self.L.vg_surr_set_params(self.h, _p(p)), "set_params")
+
+    def loss_grad(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.num_params(), np.float32)
+        self._chk(self.L.vg_surr_loss_grad(self.h, _p(x), _p(y), x.shape[0], C.byref(loss), _p(g)),
+                  "loss_grad")
+        return loss.value, g
+
+    def predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], SURR_NOUT), np.float32)
+        self._chk(self.L.vg_surr_predict(self.h, _p(x), x.shape[0], _p(y)), "predict")
+        return y
+
+    def train(self, iters: int, batch: int = 4096, lr: float = 1e-3) -> float:
+        loss = C.c_double()
+        self._chk(self.L.vg_surr_train(self.h, iters, batch, lr, C.byref(loss)), "train")
+        return loss.value
+
+    def attach(self, solver):
+        solver._chk(self.L.vg_surr_attach(solver.h, self.h), "surr_attach")
+        solver._surr = self
+
+    @staticmethod
+    def detach(solver):
+        solver._chk(solver.L.vg_surr_attach(solver.h, None), "surr_attach")
+        solver._surr = None
+
+    @staticmethod
+    def usage(solver):
+        a, b = C.c_int64(), C.c_int64()
+        solver._chk(solver.L.vg_surr_usage(solver.h, C.byref(a), C.byref(b)), "surr_usage")
+        return a.value, b.value
+
+
+def unpack_params(p: np.ndarray):
+    """Flat parameters -> [(W[k][n], b[n]) per layer]."""
+    out, o = [], 0
+    for k, n in SURR_LAYERS:
+        W = p[o:o + k * n].reshape(k, n)
+        o += k * n
+        b = p[o:o + n]
+        o += n
+        out.append((W, b))
+    return out
