@@ -198,6 +198,185 @@ def sweep_bench(vg, torch, steps, tune):
     return out
 
 
+def surrogate_kernel_timing(args, vg):
+    """Cost of the learned step's kernel alone on the LV: an untrained
+    network whose envelopes admit every node (increments scaled to ~1e-12 so
+    the run stays physical), against the plain ionic kernel."""
+    mesh, scales, pidx = lv_workload(args.scale, args.purkinje)
+    h = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED,
+                         integrator=vg.INTEG_RUSH_LARSEN, param_scales=scales, param_index=pidx,
+                         cg_x0=vg.X0_LAST, **MODEL_KW)
+    s = vg.Surrogate(args.surr_seed)
+    s.reserve(1 << 20)
+    s.collect(h, 1, every=1, node_stride=50)
+    s.fit_scaling(0.0)
+    sc = s.get_scaling()
+    sc["lo"][:], sc["hi"][:] = -1e30, 1e30
+    sc["t_lo"][:], sc["t_hi"][:] = -8.0, 8.0
+    sc["mu_y"][:], sc["sd_y"][:], sc["c_y"][:] = 0.0, 1.0, 1e-12
+    s.set_scaling(sc)
+
+    def ionic_us(steps=100):
+        h.profile(True)
+        h.Solve(steps)
+        h.synchronize()
+        ms, k = h.profileGet(vg.KC_IONIC)
+        h.profile(False)
+        return ms * 1e3 / k
+
+    h.Solve(20)
+    plain = ionic_us()
+    s.attach(h)
+    h.Solve(20)
+    surr = ionic_us()
+    used, fb = vg.Surrogate.usage(h)
+    n = mesh.n_nodes
+    print(json.dumps({"metric": "learned ionic step kernel time, 1.03M-node LV", "unit": "us",
+                      "plain_ionic_us": plain, "surrogate_ionic_us": surr,
+                      "network_rows": used, "fallback_rows": fb,
+                      "surrogate_tflops": 2 * 57344 * int(mesh.meta.get("n_myocardium", n))
+                      / (surr * 1e-6) / 1e12}))
+
+
+def surrogate_bench(args, vg, torch):
+    """The learned ten Tusscher step on the LV: record the plain model's
+    ionic updates from a plain mixed-precision run, train the network on the
+    GPU, then run the plain FP64, plain mixed and surrogate forms over the
+    same beat and compare activation times, ECG and speed."""
+    import math
+    mesh, scales, pidx = lv_workload(args.scale, args.purkinje)
+    if args.purkinje == "none":
+        stim, amp = endo_apex_stimulus(mesh), 50.0 * 2000.0
+    else:
+        stim, amp = his_stimulus(mesh), 2.0 * 50.0 * 2500.0
+    dt = MODEL_KW["dt"]
+    T = int(round(args.surr_ms / dt))
+    stim_off = int(round(2.0 / dt))
+    el = electrodes10()
+    kw = dict(cell_model=vg.MODEL_TT, integrator=vg.INTEG_RUSH_LARSEN, param_scales=scales,
+              param_index=pidx, cg_x0=vg.X0_LAST, **MODEL_KW)
+
+    def make(prec, graph):
+        h = vg.CardiacSolver(mesh, precision=prec, graph=graph, **kw)
+        h.enableActivation(-40.0)
+        h.setupECG(el)
+        h.recordECG(50, T // 50 + 2)
+        h.setStimulusCurrent(stim, amp)
+        return h
+
+    def result(h, wall=None):
+        h.synchronize()
+        return dict(at=h.activationTimes(), ecg=h.ecgFrames(), V=h.getVoltage(), wall=wall)
+
+    def log(msg):
+        print(f"[surrogate] {msg}", file=sys.stderr, flush=True)
+
+    def timed_run(h, name):
+        h.Solve(stim_off)
+        h.clearStimulus()
+        h.synchronize()
+        t0 = time.perf_counter()
+        k = stim_off
+        while k < T:  # chunks, each ending in a progress line
+            m = min(2500, T - k)
+            h.Solve(m)
+            h.synchronize()
+            k += m
+            log(f"{name}: step {k}/{T}, {time.perf_counter() - t0:.1f} s")
+        return time.perf_counter() - t0
+
+    def ionic_us(h, steps=100):
+        h.setLaunch(graph=False)
+        h.profile(True)
+        h.Solve(steps)
+        h.synchronize()
+        ms, n = h.profileGet(vg.KC_IONIC)
+        h.profile(False)
+        return ms * 1e3 / max(1, n)
+
+    n = mesh.n_nodes
+    out = {"metric": "node-steps/s, 1M-node tenTusscher heart, 1 GPU, learned ionic step",
+           "unit": "node-steps/s", "n_gpus": 1, "higher_is_better": True, "steps": T,
+           "data": "synthetic LV shell mesh; network trained on this run's plain-model updates"}
+    # 1. Samples from the plain mixed run.
+    t0 = time.perf_counter()
+    sur = vg.Surrogate(args.surr_seed)
+    h = make(vg.PREC_MIXED, 0)
+    n_myo = int(mesh.meta.get("n_myocardium", n))
+    per = -(-n_myo // args.surr_stride)
+    records = -(-(T - stim_off) // args.surr_every)
+    sur.reserve(per * records)
+    h.Solve(stim_off)
+    h.clearStimulus()
+    k = stim_off
+    while k < T:
+        m = min(2500, T - k)
+        sur.collect(h, m, every=args.surr_every, node_stride=args.surr_stride)
+        k += m
+        log(f"collect: step {k}/{T}, {sur.num_samples()} samples")
+    h.close()
+    out["samples"] = sur.num_samples()
+    out["collect_s"] = time.perf_counter() - t0
+    # 2. Training (cosine learning-rate decay).
+    sur.fit_scaling(0.02)
+    t0 = time.perf_counter()
+    its, done, curve = args.surr_iters, 0, []
+    while done < its:
+        m = min(2000, its - done)
+        lr = 2e-5 + 0.5 * (args.surr_lr - 2e-5) * (1 + math.cos(math.pi * done / its))
+        loss = sur.train(m, batch=args.surr_batch, lr=lr)
+        done += m
+        if done % 20000 == 0 or done == its:
+            curve.append((done, loss))
+            log(f"train: {done}/{its} loss {loss:.3e}")
+    train_s = time.perf_counter() - t0
+    out["train"] = {"iterations": its, "batch": args.surr_batch, "seconds": train_s, "lr": args.surr_lr,
+                    "loss_curve": curve,
+                    "tflops": 3 * 2 * 57344 * its * args.surr_batch / train_s / 1e12}
+    # 3. The three forms over the same beat.
+    runs = {}
+    for name, prec, use in (("fp64", vg.PREC_FP64, False), ("plain", vg.PREC_MIXED, False),
+                            ("surrogate", vg.PREC_MIXED, True)):
+        h = make(prec, 1)
+        if use:
+            sur.attach(h)
+        wall = timed_run(h, name)
+        runs[name] = result(h, wall)
+        if use:
+            runs[name]["usage"] = vg.Surrogate.usage(h)
+        if name != "fp64":
+            runs[name]["ionic_us"] = ionic_us(h)
+        h.close()
+    ref = runs["fp64"]
+    for name in ("plain", "surrogate"):
+        r = runs[name]
+        both = (r["at"] >= 0) & (ref["at"] >= 0)
+        dat = np.abs(r["at"][both] - ref["at"][both])
+        out[name] = {
+            "node_steps_per_s": n * (T - stim_off) / r["wall"],
+            "ms_per_step": r["wall"] * 1e3 / (T - stim_off),
+            "ionic_kernel_us": r["ionic_us"],
+            "at_max_err_ms": float(dat.max()), "at_mean_err_ms": float(dat.mean()),
+            "activated_frac": float(both.mean()),
+            "activated_mismatch": int(((r["at"] >= 0) != (ref["at"] >= 0)).sum()),
+            "ecg_rel_l2": float(np.linalg.norm(r["ecg"] - ref["ecg"]) / np.linalg.norm(ref["ecg"])),
+            "v_rms_mv_end": float(np.sqrt(np.mean((r["V"] - ref["V"]) ** 2))),
+        }
+    u = runs["surrogate"]["usage"]
+    out["surrogate"]["network_frac"] = u[0] / max(1, u[0] + u[1])
+    out["value"] = out["surrogate"]["node_steps_per_s"]
+    out["ms_per_step"] = out["surrogate"]["ms_per_step"]
+    out["speedup_vs_plain"] = out["plain"]["ms_per_step"] / out["surrogate"]["ms_per_step"]
+    out["ionic_speedup_vs_plain"] = out["plain"]["ionic_kernel_us"] / out["surrogate"]["ionic_kernel_us"]
+    out["gates"] = {"at_le_1ms": out["surrogate"]["at_max_err_ms"] <= 1.0,
+                    "ecg_le_3pct": out["surrogate"]["ecg_rel_l2"] <= 0.03,
+                    "speedup_ge_2": out["speedup_vs_plain"] >= 2.0}
+    out["config"] = {"workload": f"tT2006 LV shell + Purkinje tree ({args.purkinje}), {n} nodes, "
+                                 f"{args.surr_ms:g} ms beat, dt {dt} ms, network 25-128x4-19 SiLU BF16",
+                     "sample_every_steps": args.surr_every, "sample_node_stride": args.surr_stride}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +399,17 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB 20 times (PMC byte "
                          "counter calibration) and exit")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="evaluate the learned ionic step on the LV instead (collect, train, compare)")
+    ap.add_argument("--surr-ms", type=float, default=400.0)
+    ap.add_argument("--surr-every", type=int, default=20)
+    ap.add_argument("--surr-stride", type=int, default=5)
+    ap.add_argument("--surr-iters", type=int, default=160000)
+    ap.add_argument("--surr-batch", type=int, default=16384)
+    ap.add_argument("--surr-seed", type=int, default=1)
+    ap.add_argument("--surr-lr", type=float, default=1e-3, help="peak Adam step (cosine decay)")
+    ap.add_argument("--surr-kernel-only", action="store_true",
+                    help="with --surrogate: time the learned step's kernel alone")
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",
                     help="Purkinje tree with high/low PMJ resistance, or none")
     args = ap.parse_args()
@@ -240,6 +430,12 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 20}))
+        return
+    if args.surrogate:
+        if args.surr_kernel_only:
+            surrogate_kernel_timing(args, vg)
+        else:
+            surrogate_bench(args, vg, torch)
         return
     t_setup = time.perf_counter()
     mesh, scales, pidx = lv_workload(args.scale, args.purkinje)

@@ -383,7 +383,7 @@ void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   if constexpr (MODEL == VG_MODEL_TT) if (h->surr && !h->rec_x) {
     // The network advances the rows inside its envelope and flags the rest,
     // which the plain kernel below then updates.
-    hipLaunchKernelGGL(k_ionic_surr<T>, dim3(h->surr_grid), dim3(BLK), 0, h->stream, Vin, x,
+    hipLaunchKernelGGL(k_ionic_surr<T>, dim3(h->surr_grid), dim3(SBLK), 0, h->stream, Vin, x,
                        h->sinv.as<T>(), Vout, h->states.as<T>(),
                        h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
                        h->pidx.p ? h->pidx.as<uint16_t>() : (const uint16_t *)nullptr,
@@ -1459,10 +1459,10 @@ int vg_surr_attach(vg_handle *h, vg_surr *s) {
     int cus = 0;
     HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     const int64_t tiles = (h->n_myo + 31) / 32;
-    h->surr_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cus, (tiles + WAVES - 1) / WAVES));
+    h->surr_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cus, (tiles + SWAVES - 1) / SWAVES));
     HIPCHK(h, h->surr_fb.alloc((size_t)h->n_alloc));
-    HIPCHK(h, h->surr_use.alloc((size_t)h->surr_grid * WAVES * 2 * 8));
-    HIPCHK(h, hipMemsetAsync(h->surr_use.p, 0, (size_t)h->surr_grid * WAVES * 2 * 8, h->stream));
+    HIPCHK(h, h->surr_use.alloc((size_t)h->surr_grid * SWAVES * 2 * 8));
+    HIPCHK(h, hipMemsetAsync(h->surr_use.p, 0, (size_t)h->surr_grid * SWAVES * 2 * 8, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->sdev = d;
   }
@@ -1479,7 +1479,7 @@ int vg_surr_usage(vg_handle *h, int64_t *network, int64_t *fallback) {
   if (!h || !network || !fallback) return VG_ERR_ARG;
   *network = *fallback = 0;
   if (!h->surr) return VG_OK;
-  std::vector<unsigned long long> u((size_t)h->surr_grid * WAVES * 2);
+  std::vector<unsigned long long> u((size_t)h->surr_grid * SWAVES * 2);
   HIPCHK(h, hipMemcpyAsync(u.data(), h->surr_use.p, u.size() * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (size_t k = 0; k < u.size(); k += 2) {

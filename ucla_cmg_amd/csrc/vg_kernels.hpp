@@ -601,42 +601,108 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
 }
 
 // ---- learned ten Tusscher ionic update ---------------------------------------
-// Lanes of one wave exchange activations through the wave's own LDS buffer.
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// The network runs transposed, Z^T = W^T A^T: the weights are the MFMA A
+// operand (read from LDS, row = output unit), the activations the B operand
+// (column = node), so every layer's outputs stay in the registers of the
+// lanes that own their node.  Lane l holds node l % 32 and half h = l / 32;
+// for unit tile nt its 16 results j are units 32 nt + 8 g + 4 h + r4
+// (g = j / 4, r4 = j % 4).  The next layer's operand for k step
+// ks = 2 nt + s needs units 16 ks + 8 h + q (q < 8): four the lane holds
+// (g = 2 s + h), four its partner l ^ 32 holds (same g), so a layer hands
+// over with one exchange of four BF16 per k step and no LDS traffic.
+
+constexpr int SBLK = 512;             // 8 waves, two per SIMD
+constexpr int SWAVES = SBLK / 64;
+constexpr int SG_BIAS_TOTAL = (SG_LAYERS - 1) * SG_HID + SG_NOUTP;
+
+__device__ inline uint32_t sg_pack2(float a, float b) {
+  const __bf16 x = (__bf16)a, y = (__bf16)b;
+  return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
 }
 
-// One hidden or output layer for the wave's 32 rows: A (the wave's activation
-// rows, BF16, SG_ACT_LD apart) times the layer's transposed weights in LDS,
-// NT output tiles of 32 columns accumulated in FP32.
+__device__ inline bf16x8 sg_frag(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  return __builtin_bit_cast(bf16x8, u32x4{d0, d1, d2, d3});
+}
+
+// acc[nt] = W^T (tiles of 32 units) times the KS operand fragments b.
 template <int KS, int NT>
-__device__ inline void surr_layer(const __bf16 *A, const __bf16 *W, int ldw, int r, int kq,
-                                  f32x16 (&acc)[4]) {
-  bf16x8 a[KS];
+__device__ inline void surr_mm(const __bf16 *W, int ldw, int r, int kq, const bf16x8 (&b)[8],
+                               f32x16 (&acc)[4]) {
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) a[ks] = *reinterpret_cast<const bf16x8 *>(A + r * SG_ACT_LD + ks * 16 + kq);
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f32x16{};
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) {
-    acc[nt] = f32x16{};
+  for (int ks = 0; ks < KS; ks++) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(W + (nt * 32 + r) * ldw + ks * 16 + kq);
-      acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], b, acc[nt], 0, 0, 0);
+    for (int nt = 0; nt < NT; nt++) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(W + (nt * 32 + r) * ldw + ks * 16 + kq);
+      acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[ks], acc[nt], 0, 0, 0);
+    }
+    // Keep the weight fragments of one k step in flight at a time (the
+    // registers go to a second wave per SIMD instead).
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Bias + SiLU of a hidden layer, handed over as the next layer's fragments.
+__device__ inline void surr_hand_over(const f32x16 (&acc)[4], const float *bias, int h,
+                                      bf16x8 (&b)[8]) {
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    float a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      a[j] = sg_silu(acc[nt][j] + bias[nt * 32 + 8 * (j >> 2) + 4 * h + (j & 3)]);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      // Own values: g = 2 s + h; the partner needs g = 2 s + 1 - h.
+      const int go = 4 * (2 * s), g1 = 4 * (2 * s + 1);
+      const uint32_t own0 = h ? sg_pack2(a[g1], a[g1 + 1]) : sg_pack2(a[go], a[go + 1]);
+      const uint32_t own1 = h ? sg_pack2(a[g1 + 2], a[g1 + 3]) : sg_pack2(a[go + 2], a[go + 3]);
+      const uint32_t snd0 = h ? sg_pack2(a[go], a[go + 1]) : sg_pack2(a[g1], a[g1 + 1]);
+      const uint32_t snd1 = h ? sg_pack2(a[go + 2], a[go + 3]) : sg_pack2(a[g1 + 2], a[g1 + 3]);
+      const uint32_t rcv0 = __shfl_xor(snd0, 32, 64), rcv1 = __shfl_xor(snd1, 32, 64);
+      b[2 * nt + s] = h ? sg_frag(rcv0, rcv1, own0, own1) : sg_frag(own0, own1, rcv0, rcv1);
     }
   }
 }
 
-// Bias + SiLU of a hidden layer's outputs back into the activation rows.
-__device__ inline void surr_hidden_out(__bf16 *A, const f32x16 (&acc)[4], const float *b, int l,
-                                       int r) {
+// Output units of lane half H: 8 g + 4 H + r4.  Checks the real ones against
+// their envelope (oc rows: t_lo, t_hi, mu_y, sd_y, c_y) and turns them into
+// increments y (indexed by state).
+template <int H>
+__device__ inline bool surr_outputs(const f32x16 &acc, const float *bias, const float *oc,
+                                    float (&y)[SG_NOUT]) {
+  bool ok = true;
 #pragma unroll
-  for (int nt = 0; nt < 4; nt++) {
-    const int col = nt * 32 + r;
-    const float bc = b[col];
+  for (int j = 0; j < 16; j++) {
+    const int u = 8 * (j >> 2) + 4 * H + (j & 3);
+    if (u < SG_NOUT) {
+      const float t = acc[j] + bias[u];
+      ok = ok && t >= oc[u] && t <= oc[32 + u];
+      y[u] = sg_from_target(t, oc[64 + u], oc[96 + u], oc[128 + u]);
+    }
+  }
+  return ok;
+}
+
+// New V and states; the start values are read again (cache hits) rather
+// than held across the network.
+template <class T, int H>
+__device__ inline void surr_store(const float (&y)[SG_NOUT], T V, int64_t i, int64_t n_alloc,
+                                  T *__restrict__ Vout, T *__restrict__ st) {
 #pragma unroll
-    for (int j = 0; j < 16; j++) A[mfma_row(l, j) * SG_ACT_LD + col] = (__bf16)sg_silu(acc[nt][j] + bc);
+  for (int j = 0; j < 16; j++) {
+    const int k = 8 * (j >> 2) + 4 * H + (j & 3);
+    if (k < SG_NOUT) {
+      T v = (k == 0 ? V : st[(k - 1) * n_alloc + i]) + (T)y[k];
+      if (k == 0) {
+        Vout[i] = v;
+      } else {
+        if ((k >= 4 && k <= 9) || (k >= 11 && k <= 16) || k == 18) v = fmin(fmax(v, T(0)), T(1));
+        st[(k - 1) * n_alloc + i] = v;
+      }
+    }
   }
 }
 
@@ -644,110 +710,108 @@ __device__ inline void surr_hidden_out(__bf16 *A, const f32x16 (&acc)[4], const 
 // whose 25 inputs and 19 outputs lie inside the training envelopes and that
 // has no stimulus gets V and its 18 states advanced by the network's
 // increments (gates and R' clamped to [0, 1]); the others are flagged in
-// `fallback` for the plain kernel.  Persistent blocks keep all weights in LDS; each wave takes tiles
-// of 32 rows, lane r and r + 32 sharing row r (the MFMA operand layout).
-// usage[2 * (block * 4 + wave) + {0, 1}] counts network / fallback rows.
+// `fallback` for the plain kernel.  Persistent blocks keep weights, biases
+// and output constants in LDS; each wave takes tiles of 32 rows.
+// usage[2 * (block * SWAVES + wave) + {0, 1}] counts network / fallback rows.
 template <class T>
-__global__ __launch_bounds__(BLK) void k_ionic_surr(
+__global__ __launch_bounds__(SBLK) void k_ionic_surr(
     const T *__restrict__ Vin, const T *__restrict__ x, const T *__restrict__ sinv,
     T *__restrict__ Vout, T *__restrict__ st, const T *__restrict__ istim,
     const uint16_t *__restrict__ pidx, const T *__restrict__ scales, int64_t n, int64_t n_alloc,
     float log_dt, SurrDevice sd, unsigned char *__restrict__ fallback,
     unsigned long long *__restrict__ usage, const CgResult *cg, const int *__restrict__ slice_inst) {
   __shared__ __attribute__((aligned(16))) __bf16 wl[SG_WT_TOTAL];
-  __shared__ __attribute__((aligned(16))) __bf16 actb[WAVES][32 * SG_ACT_LD];
-  for (int q = threadIdx.x; q < SG_WT_TOTAL / 8; q += BLK)
+  __shared__ float bl[SG_BIAS_TOTAL];
+  __shared__ float oc[5 * 32];
+  __shared__ float ic[4 * 32];  // lo, hi, mu_x, sd_x
+  for (int q = threadIdx.x; q < SG_WT_TOTAL / 8; q += SBLK)
     reinterpret_cast<bf16x8 *>(wl)[q] = reinterpret_cast<const bf16x8 *>(sd.wt)[q];
+  for (int q = threadIdx.x; q < SG_BIAS_TOTAL; q += SBLK) {
+    const int L = q / SG_HID < SG_LAYERS - 1 ? q / SG_HID : SG_LAYERS - 1;
+    bl[q] = sd.b[L][q - L * SG_HID];
+  }
+  for (int q = threadIdx.x; q < 5 * 32; q += SBLK) {
+    const int row = q / 32, u = q % 32;
+    const float *src[5] = {sd.t_lo, sd.t_hi, sd.mu_y, sd.sd_y, sd.c_y};
+    oc[q] = u < SG_NOUT ? src[row][u] : 0.0f;
+  }
+  for (int q = threadIdx.x; q < 4 * 32; q += SBLK) {
+    const int row = q / 32, u = q % 32;
+    const float *src[4] = {sd.lo, sd.hi, sd.mu_x, sd.sd_x};
+    ic[q] = u < SG_NIN ? src[row][u] : 1.0f;
+  }
   __syncthreads();
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 31, kq = (l >> 5) * 8;
-  __bf16 *A = actb[wave];
-  float *Y = reinterpret_cast<float *>(A);  // output staging [32][33]
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 31, h = l >> 5, kq = h * 8;
   unsigned long long n_net = 0, n_fb = 0;
   const int64_t ntiles = (n + 31) / 32;
-  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile < ntiles;
-       tile += (int64_t)gridDim.x * WAVES) {
+  for (int64_t tile = (int64_t)blockIdx.x * SWAVES + wave; tile < ntiles;
+       tile += (int64_t)gridDim.x * SWAVES) {
     const int64_t i = tile * 32 + r;
-    const bool mine = l < 32 && i < n;
-    T s[TT_NSTATE];
+    const bool mine = i < n;
+    T V = T(0);
+    float xn[SG_KIN];
     bool ok = false;
     if (mine) {
-      T V = Vin[i];
+      V = Vin[i];
       if (x && cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid) V += sinv[i] * x[i];
-      s[0] = V;
-#pragma unroll
-      for (int k = 1; k < TT_NSTATE; k++) s[k] = st[(k - 1) * n_alloc + i];
       const T *sc = scales + (pidx ? (int)pidx[i] : 0) * VG_TT_NSCALE;
       float f[SG_NIN];
+      f[0] = (float)V;
 #pragma unroll
-      for (int k = 0; k < TT_NSTATE; k++) f[k] = (float)s[k];
+      for (int k = 1; k < TT_NSTATE; k++) f[k] = (float)st[(k - 1) * n_alloc + i];
       f[19] = log_dt;
 #pragma unroll
       for (int k = 0; k < VG_TT_NSCALE; k++) f[20 + k] = (float)sc[k];
       ok = !(istim && istim[i] != T(0));
 #pragma unroll
-      for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= sd.lo[k] && f[k] <= sd.hi[k];
+      for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= ic[k] && f[k] <= ic[32 + k];
 #pragma unroll
-      for (int k = 0; k < SG_KIN; k++)
-        A[r * SG_ACT_LD + k] = (__bf16)(ok && k < SG_NIN ? (f[k] - sd.mu_x[k]) / sd.sd_x[k] : 0.0f);
-      fallback[i] = ok ? 0 : 1;
-    } else if (l < 32) {
+      for (int k = 0; k < SG_KIN; k++) xn[k] = ok && k < SG_NIN ? (f[k] - ic[64 + k]) / ic[96 + k] : 0.0f;
+    } else {
 #pragma unroll
-      for (int k = 0; k < SG_KIN; k++) A[r * SG_ACT_LD + k] = (__bf16)0.0f;
+      for (int k = 0; k < SG_KIN; k++) xn[k] = 0.0f;
     }
-    wave_lds_sync();
+    // Layer 0 operand: features 16 ks + 8 h + q.
+    bf16x8 b[8];
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = h ? xn[16 * ks + 8 + q] : xn[16 * ks + q];
+      b[ks] = sg_frag(sg_pack2(v[0], v[1]), sg_pack2(v[2], v[3]), sg_pack2(v[4], v[5]),
+                      sg_pack2(v[6], v[7]));
+    }
     f32x16 acc[4];
-    surr_layer<2, 4>(A, wl + sg_wt_off(0), sg_wt_ld(0), r, kq, acc);
-    wave_lds_sync();
-    surr_hidden_out(A, acc, sd.b[0], l, r);
-    wave_lds_sync();
+    surr_mm<2, 4>(wl + sg_wt_off(0), sg_wt_ld(0), r, kq, b, acc);
+    surr_hand_over(acc, bl, h, b);
 #pragma unroll 1
     for (int L = 1; L < SG_LAYERS - 1; L++) {
-      surr_layer<8, 4>(A, wl + sg_wt_off(L), sg_wt_ld(L), r, kq, acc);
-      wave_lds_sync();
-      surr_hidden_out(A, acc, sd.b[L], l, r);
-      wave_lds_sync();
+      surr_mm<8, 4>(wl + sg_wt_off(L), sg_wt_ld(L), r, kq, b, acc);
+      surr_hand_over(acc, bl + L * SG_HID, h, b);
     }
-    surr_layer<8, 1>(A, wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, acc);
-    wave_lds_sync();
-    if (r < SG_NOUT) {
-      const float bc = sd.b[SG_LAYERS - 1][r];
-#pragma unroll
-      for (int j = 0; j < 16; j++) Y[mfma_row(l, j) * 33 + r] = acc[0][j] + bc;
-    }
-    wave_lds_sync();
+    surr_mm<8, 1>(wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, b, acc);
+    float y[SG_NOUT];
+    const bool ok_out = h ? surr_outputs<1>(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, y)
+                          : surr_outputs<0>(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, y);
+    const int both = __shfl_xor((int)ok_out, 32, 64) & (int)ok_out;
+    ok = ok && both;
     if (mine && ok) {
-      // Outputs outside their envelope (or not finite) leave the node to
-      // the plain model too.
-      float y[SG_NOUT];
-#pragma unroll
-      for (int k = 0; k < SG_NOUT; k++) {
-        const float t = Y[r * 33 + k];
-        ok = ok && t >= sd.t_lo[k] && t <= sd.t_hi[k];
-        y[k] = sg_from_target(t, sd.mu_y[k], sd.sd_y[k], sd.c_y[k]);
-      }
-      if (ok) {
-        Vout[i] = s[0] + (T)y[0];
-#pragma unroll
-        for (int k = 1; k < TT_NSTATE; k++) {
-          T v = s[k] + (T)y[k];
-          if ((k >= 4 && k <= 9) || (k >= 11 && k <= 16) || k == 18) v = fmin(fmax(v, T(0)), T(1));
-          st[(k - 1) * n_alloc + i] = v;
-        }
-        n_net++;
-      } else {
-        fallback[i] = 1;
-      }
+      if (h) surr_store<T, 1>(y, V, i, n_alloc, Vout, st);
+      else surr_store<T, 0>(y, V, i, n_alloc, Vout, st);
     }
-    if (mine && !ok) n_fb++;
-    wave_lds_sync();
+    if (mine && h == 0) {
+      fallback[i] = ok ? 0 : 1;
+      if (ok) n_net++;
+      else n_fb++;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     n_net += __shfl_xor(n_net, o, 64);
     n_fb += __shfl_xor(n_fb, o, 64);
   }
   if (l == 0) {
-    usage[2 * (blockIdx.x * WAVES + wave)] += n_net;
-    usage[2 * (blockIdx.x * WAVES + wave) + 1] += n_fb;
+    usage[2 * (blockIdx.x * SWAVES + wave)] += n_net;
+    usage[2 * (blockIdx.x * SWAVES + wave) + 1] += n_fb;
   }
 }
 

@@ -167,7 +167,7 @@ __global__ void k_sg_act_back(const float *__restrict__ dA, const float *__restr
   float g = 0.0f;
   if (n < nreal) {
     const float z = Z[i];
-    const float sg = 1.0f / (1.0f + __expf(-z));
+    const float sg = sg_sigmoid(z);
     g = dA[i] * sg * (1.0f + z * (1.0f - sg));
   }
   dZf[i] = g;

@@ -41,7 +41,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // 8*(j/4) + 4*(l/32) + j%4.
 __device__ inline int mfma_row(int lane, int j) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
 
-__device__ inline float sg_silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU z * sigmoid(z) with the hardware reciprocal (training and the step
+// evaluate the same expression).
+__device__ inline float sg_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float sg_silu(float z) { return z * sg_sigmoid(z); }
 
 // The weights as the step's kernel keeps them in LDS: per layer W^T[np][kp]
 // (row = output column) with rows padded by 8 (16-byte reads of 32 rows then
