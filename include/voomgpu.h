@@ -148,6 +148,10 @@ typedef struct {
    * launches is counted in vg_stats.cg_not_converged and the step is
    * re-captured with more. */
   int graph;
+  /* Diffusive smoothing factor alpha (EPSimulator's SMOOTHING keyword,
+   * EP_CrankNicholson.cc:31-36): A and B get +-alpha (4/dt) K on top of
+   * +-dt/4 K.  0 = off. */
+  double smoothing;
 } vg_model;
 
 /* Initial guess of the CG solves.  Either way a solve stops when
@@ -252,6 +256,39 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V,
                          const double *istim, double *dVdt);
 int vg_cells_get_states(vg_cells *c, int *nvar, double *states);
 int vg_cells_set_states(vg_cells *c, int nvar, const double *states);
+
+/* ---- the reference's files ----------------------------------------------
+ * Mesh input as Mesh::New reads it (Mesh/Mesh.cc:142-416): <model>.node
+ * ("n 3 x x", then "id x y z"), <model>.ele ("n x x", then "id k n1..nk",
+ * 1-based; k = 4 tetrahedra or 2 bars) and, when present, <model>.fibers
+ * (9 numbers per element), .purk ("n", then node ids -> tissue 11), .cell
+ * ("id type" pairs) and .angle ("id c0 c1 c2 L" PMJ overrides).  Ids become
+ * 0-based.  The partition files (.metis.*) are not needed on one GPU.
+ * Errors without a handle go to err[err_len]. */
+typedef struct vg_meshfile vg_meshfile;
+int vg_mesh_read(const char *model, double radius, vg_meshfile **out, char *err, int err_len);
+const vg_mesh *vg_meshfile_mesh(const vg_meshfile *m); /* owned by m */
+void vg_meshfile_free(vg_meshfile *m);
+/* Writes a mesh in the same files (plus single-partition .metis.npart.1 and
+ * .metis.epart.1, which the reference's reader requires). */
+int vg_mesh_write(const char *model, const vg_mesh *mesh, char *err, int err_len);
+/* "<dir>/<prefix>_time_%08.2f.%02.0f" (EPSolver.cc:121,159). */
+int vg_frame_name(char *buf, int len, const char *dir, const char *prefix, double time, int rank);
+/* Voltage_time_* : raw FP64 V in node order (EPSolver::WriteOutput,
+ * EPSolver.cc:116-136). */
+int vg_write_voltage(vg_handle *h, const char *dir, double time, int rank);
+/* InterVariables_time_* : per node int32 state count then the FP64 states
+ * (EPSolver::writeRestart, EPSolver.cc:155-177). */
+int vg_write_restart(vg_handle *h, const char *dir, double time, int rank);
+/* Both files back into V and the states (EPSolver::readRestart,
+ * EPSolver.cc:180-211). */
+int vg_read_restart(vg_handle *h, const char *dir, double time, int rank);
+/* Host-only lead field of the pseudo-ECG, H[ne][n_nodes] with
+ * ecg_e = sum_n H[e][n] V_n: computeGradOneByR, computeDiffusionTensor and
+ * computeECG (Applications/CardiacEP/ComputeECG.h:30-189) folded into one
+ * matrix; no GPU needed (post-processing of written frames). */
+int vg_ecg_leadfield(const vg_mesh *mesh, unsigned quirks, const double *electrodes, int ne,
+                     const double D[3], double radius, double *H, char *err, int err_len);
 
 /* ---- learned ten Tusscher ionic step ------------------------------------
  * No counterpart in the reference: a small MLP (25 inputs: V and the 18

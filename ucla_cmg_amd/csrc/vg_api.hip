@@ -835,6 +835,10 @@ std::vector<int64_t> state_offsets(const vg_handle *h) {
 
 }  // namespace
 
+namespace vg {
+int handle_fail(vg_handle *h, int code, const std::string &msg) { return h->fail(code, msg); }
+}  // namespace vg
+
 extern "C" {
 
 const char *vg_version(void) { return "voomgpu 0.1 (gfx950)"; }
@@ -953,7 +957,10 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->n_pk = (int64_t)base.size() - h->n_myo;
   h->n_pk_alloc = ((h->n_pk + BLK - 1) / BLK) * BLK;
   HostSell sell;
-  e = build_sell(csr, h->dt, h->Cm, model->sell_sigma, base, h->n_myo, sell);
+  // Diffusive smoothing adds alpha (4/dT) K to A and subtracts it from B
+  // (EP_CrankNicholson.cc:31-36).
+  const double kc = h->dt / 4.0 + (model->smoothing > 0.0 ? 4.0 * model->smoothing / h->dt : 0.0);
+  e = build_sell(csr, kc, h->Cm, model->sell_sigma, base, h->n_myo, sell);
   if (!e.empty()) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, e);

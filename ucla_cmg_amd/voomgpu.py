@@ -59,6 +59,8 @@ EXPORTED = [
     "vg_surr_fit_scaling", "vg_surr_get_scaling", "vg_surr_set_scaling", "vg_surr_num_params",
     "vg_surr_get_params", "vg_surr_set_params", "vg_surr_loss_grad", "vg_surr_predict",
     "vg_surr_train", "vg_surr_attach", "vg_surr_usage",
+    "vg_mesh_read", "vg_meshfile_mesh", "vg_meshfile_free", "vg_mesh_write", "vg_frame_name",
+    "vg_write_voltage", "vg_write_restart", "vg_read_restart",
 ]
 
 # Learned ten Tusscher step: inputs V, 18 states, log(dt), 5 scalings;
@@ -85,7 +87,7 @@ class VgModel(C.Structure):
         ("n_param_sets", C.c_int), ("param_scales", C.c_void_p),
         ("param_index", C.c_void_p), ("sell_sigma", C.c_int), ("cg_blocks", C.c_int),
         ("cg_x0", C.c_int), ("n_instances", C.c_int), ("instance", C.c_void_p),
-        ("graph", C.c_int),
+        ("graph", C.c_int), ("smoothing", C.c_double),
     ]
 
 
@@ -175,6 +177,16 @@ def load_library(path: str = None):
         "vg_surr_train": ([H, C.c_int, C.c_int, C.c_double, dp], C.c_int),
         "vg_surr_attach": ([H, H], C.c_int),
         "vg_surr_usage": ([H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "vg_mesh_read": ([C.c_char_p, C.c_double, C.POINTER(C.c_void_p), C.c_char_p, C.c_int],
+                         C.c_int),
+        "vg_meshfile_mesh": ([H], C.POINTER(VgMesh)),
+        "vg_meshfile_free": ([H], None),
+        "vg_mesh_write": ([C.c_char_p, C.POINTER(VgMesh), C.c_char_p, C.c_int], C.c_int),
+        "vg_frame_name": ([C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_double, C.c_int],
+                          C.c_int),
+        "vg_write_voltage": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
+        "vg_write_restart": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
+        "vg_read_restart": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -223,7 +235,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
                param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO, instance=None,
-               graph=0):
+               graph=0, smoothing=0.0):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -252,6 +264,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
     m.cg_blocks = cg_blocks
     m.cg_x0 = cg_x0
     m.graph = graph
+    m.smoothing = smoothing
     if instance is not None:
         ins = np.ascontiguousarray(instance, np.int32)
         keep.append(ins)
@@ -284,6 +297,57 @@ def assemble_csr(mesh, **model_kw):
 
 class VoomError(RuntimeError):
     pass
+
+
+def read_mesh(model: str, radius: float = 0.04):
+    """``Mesh::New``'s files (<model>.node/.ele[/.fibers/.purk/.cell/.angle],
+    Mesh/Mesh.cc:142-416) -> a 0-based :class:`meshgen.Mesh`."""
+    from . import meshgen
+    L = load_library()
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = L.vg_mesh_read(model.encode(), radius, C.byref(h), err, 512)
+    if rc != VG_OK:
+        raise VoomError(f"vg_mesh_read: {err.value.decode()} ({rc})")
+    try:
+        m = L.vg_meshfile_mesh(h).contents
+        n, ne = m.n_nodes, m.n_elems
+
+        def arr(ptr, ctype, count, shape):
+            if not ptr or count == 0:
+                return None
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), (count,)).reshape(shape).copy()
+
+        mesh = meshgen.Mesh(
+            xyz=arr(m.xyz, C.c_double, 3 * n, (n, 3)),
+            elem_nn=arr(m.elem_nn, C.c_int8, ne, (ne,)),
+            conn=arr(m.conn, C.c_int32, 4 * ne, (ne, 4)),
+            fibers=arr(m.fibers, C.c_double, 9 * ne, (ne, 9)),
+            tissue=arr(m.tissue, C.c_int8, n, (n,)),
+            pmj_node=arr(m.pmj_node, C.c_int32, m.n_pmj, (m.n_pmj,)),
+            pmj_data=arr(m.pmj_data, C.c_double, 4 * m.n_pmj, (m.n_pmj, 4)),
+            radius=m.radius, meta={"kind": "file", "model": model})
+    finally:
+        L.vg_meshfile_free(h)
+    return mesh
+
+
+def write_mesh(model: str, mesh):
+    """Writes ``mesh`` in the reference's files (see :func:`read_mesh`)."""
+    L = load_library()
+    view = _MeshView(mesh)
+    err = C.create_string_buffer(512)
+    rc = L.vg_mesh_write(model.encode(), C.byref(view.struct), err, 512)
+    if rc != VG_OK:
+        raise VoomError(f"vg_mesh_write: {err.value.decode()} ({rc})")
+
+
+def frame_name(directory: str, prefix: str, time: float, rank: int = 0) -> str:
+    L = load_library()
+    buf = C.create_string_buffer(4096)
+    if L.vg_frame_name(buf, 4096, directory.encode(), prefix.encode(), time, rank) != VG_OK:
+        raise VoomError("vg_frame_name failed")
+    return buf.value.decode()
 
 
 class CardiacSolver:
@@ -443,6 +507,16 @@ class CardiacSolver:
         ms, n = C.c_double(), C.c_int64()
         self._chk(self.L.vg_profile_get(self.h, kc, C.byref(ms), C.byref(n)), "profile_get")
         return ms.value, n.value
+
+    # -- EPSolver::WriteOutput / writeRestart / readRestart (EPSolver.cc:116-211)
+    def WriteOutput(self, time: float, path: str, rank: int = 0):
+        self._chk(self.L.vg_write_voltage(self.h, path.encode(), time, rank), "write_voltage")
+
+    def writeRestart(self, time: float, path: str, rank: int = 0):
+        self._chk(self.L.vg_write_restart(self.h, path.encode(), time, rank), "write_restart")
+
+    def readRestart(self, time: float, path: str, rank: int = 0):
+        self._chk(self.L.vg_read_restart(self.h, path.encode(), time, rank), "read_restart")
 
     def stats(self) -> dict:
         s = VgStats()
