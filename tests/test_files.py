@@ -80,27 +80,11 @@ def test_frame_names_follow_the_reference_format(t, rank):
     assert vg.frame_name("o", "InterVariables", 1.0, 0) == "o/InterVariables_time_00001.00.00"
 
 
-def _leadfield(mesh, el):
-    import ctypes as C
-    L = vg.load_library()
-    view = vg._MeshView(mesh)
-    el = np.ascontiguousarray(el, np.float64)
-    H = np.zeros((el.shape[0], mesh.n_nodes))
-    err = C.create_string_buffer(256)
-    D = (C.c_double * 3)(1e-3, 5e-4, 2.5e-4)
-    L.vg_ecg_leadfield.argtypes = [C.POINTER(vg.VgMesh), C.c_uint, C.c_void_p, C.c_int,
-                                   C.POINTER(C.c_double), C.c_double, C.c_void_p, C.c_char_p, C.c_int]
-    rc = L.vg_ecg_leadfield(C.byref(view.struct), 0, el.ctypes.data, el.shape[0], D, 0.0275,
-                            H.ctypes.data, err, 256)
-    assert rc == 0, err.value
-    return H
-
-
 def test_host_lead_field_equals_the_oracle_direct_ecg():
     mesh = meshgen.purkinje_tree(meshgen.lv_shell(n_trans=4, n_circ=16, n_long=8), generations=3,
                                  pmj_length=meshgen.pmj_length(500.0))
     el = np.array([[4.0, 0.5, -2.0], [0.0, -5.0, 1.0], [0.2, 0.1, -9.0]])
-    H = _leadfield(mesh, el)
+    H = vg.ecg_leadfield(mesh, el)
     rng = np.random.default_rng(3)
     for _ in range(3):
         V = rng.uniform(-90, 30, mesh.n_nodes)

@@ -60,7 +60,7 @@ EXPORTED = [
     "vg_surr_get_params", "vg_surr_set_params", "vg_surr_loss_grad", "vg_surr_predict",
     "vg_surr_train", "vg_surr_attach", "vg_surr_usage",
     "vg_mesh_read", "vg_meshfile_mesh", "vg_meshfile_free", "vg_mesh_write", "vg_frame_name",
-    "vg_write_voltage", "vg_write_restart", "vg_read_restart",
+    "vg_write_voltage", "vg_write_restart", "vg_read_restart", "vg_ecg_leadfield",
 ]
 
 # Learned ten Tusscher step: inputs V, 18 states, log(dt), 5 scalings;
@@ -187,6 +187,8 @@ def load_library(path: str = None):
         "vg_write_voltage": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
         "vg_write_restart": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
         "vg_read_restart": ([H, C.c_char_p, C.c_double, C.c_int], C.c_int),
+        "vg_ecg_leadfield": ([C.POINTER(VgMesh), C.c_uint, C.c_void_p, C.c_int, dp, C.c_double,
+                              C.c_void_p, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -340,6 +342,20 @@ def write_mesh(model: str, mesh):
     rc = L.vg_mesh_write(model.encode(), C.byref(view.struct), err, 512)
     if rc != VG_OK:
         raise VoomError(f"vg_mesh_write: {err.value.decode()} ({rc})")
+
+
+def ecg_leadfield(mesh, electrodes, D=(1e-3, 5e-4, 2.5e-4), radius=0.0275, quirks=0):
+    """Host lead field H[ne][n] of the pseudo-ECG (no GPU): ecg = H @ V."""
+    L = load_library()
+    view = _MeshView(mesh)
+    el = np.ascontiguousarray(electrodes, np.float64).reshape(-1, 3)
+    H = np.zeros((el.shape[0], view.xyz.shape[0]))
+    err = C.create_string_buffer(512)
+    rc = L.vg_ecg_leadfield(C.byref(view.struct), quirks, _p(el), el.shape[0], _d3(D), radius,
+                            _p(H), err, 512)
+    if rc != VG_OK:
+        raise VoomError(f"vg_ecg_leadfield: {err.value.decode()} ({rc})")
+    return H
 
 
 def frame_name(directory: str, prefix: str, time: float, rank: int = 0) -> str:
