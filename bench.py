@@ -198,6 +198,106 @@ def sweep_bench(vg, torch, steps, tune):
     return out
 
 
+def phase_singularities(Vt, Vlag, vstar=-40.0):
+    """Phase singularities of a surface frame: phase from the time-delay
+    embedding theta = atan2(V(t) - V*, V(t - tau) - V*), charge of every
+    grid plaquette = winding of theta around it / 2 pi.  Returns (number of
+    plaquettes with nonzero charge, net charge)."""
+    th = np.arctan2(Vt - vstar, Vlag - vstar)
+
+    def wrap(d):
+        return (d + np.pi) % (2.0 * np.pi) - np.pi
+
+    a, b, c, d = th[:-1, :-1], th[:-1, 1:], th[1:, 1:], th[1:, :-1]
+    q = np.rint((wrap(b - a) + wrap(c - b) + wrap(d - c) + wrap(a - d)) / (2.0 * np.pi))
+    return int(np.count_nonzero(q)), int(q.sum())
+
+
+def mahajan_bench(args, vg):
+    """Mahajan rabbit ventricle on a 6 x 6 x 1 cm slab at h = 0.02 cm
+    (301 x 301 x 51 = 4,620,651 nodes), isotropic sigma = 1 mS/cm: an S1
+    plane wave from x = 0, then an S2 cross-field stimulus over y <= 3 cm at
+    a seeded coupling interval in [180, 230] ms; dt = 0.01 ms.  Reports
+    node-steps/s, the phase-singularity count on the z = 0 surface and a
+    10-lead pseudo-ECG."""
+    from ucla_cmg_amd import meshgen
+    nx = ny = 301
+    nz = 51
+    h_ = 0.02
+    t0 = time.perf_counter()
+    mesh = meshgen.slab(nx, ny, nz, h_)
+    dt = 0.01
+    steps = int(round(args.mj_ms / dt))
+    rng = np.random.default_rng(args.mj_seed)
+    s2 = float(rng.uniform(180.0, 230.0))
+    amp = 50.0 * 6007.8  # 50 uA/cm^2 through the model's own chi
+    sigma = 1.0 / 1400.0
+    s = vg.CardiacSolver(mesh, cell_model=vg.MODEL_MJ, precision=vg.PREC_MIXED,
+                         integrator=vg.INTEG_REFERENCE, D=(sigma, sigma, sigma), dt=dt,
+                         dt_ionic=dt, cg_tol=1e-7, cg_x0=vg.X0_LAST, graph=1)
+    setup_s = time.perf_counter() - t0
+    ring = meshgen.electrodes_around((3.0, 3.0, 0.5), 5.0, 8)
+    el = np.vstack([ring, [[3.0, 3.0, 3.0], [3.0, 3.0, -2.0]]])
+    s.setupECG(el)
+    s.enableActivation(-40.0)
+    s.recordECG(100, steps // 100 + 2)
+    frame_every = int(round(5.0 / dt))
+    events = [(0, "s1_on"), (int(round(2.0 / dt)), "s1_off"), (int(round(s2 / dt)), "s2_on"),
+              (int(round((s2 + 2.0) / dt)), "s2_off")]
+    surf = []
+    ps = []
+    k = 0
+    t_run = time.perf_counter()
+    while k < steps:
+        for kk, what in events:
+            if kk == k:
+                if what == "s1_on":
+                    s.setStimulusCurrent(0, 0.06, amp)
+                elif what == "s2_on":
+                    s.setStimulusCurrent(1, 3.0, amp)
+                else:
+                    s.clearStimulus()
+        nxt = min([steps] + [kk for kk, _ in events if kk > k] + [(k // frame_every + 1) * frame_every])
+        s.Solve(nxt - k)
+        k = nxt
+        if k % frame_every == 0:
+            V = s.getVoltage()[: nx * ny].reshape(ny, nx)
+            surf.append(V)
+            if len(surf) >= 3:
+                ps.append((k * dt, *phase_singularities(surf[-1], surf[-3])))
+                surf.pop(0)
+            if k % (frame_every * 10) == 0:
+                print(f"[mahajan] t = {k * dt:.0f} ms, {time.perf_counter() - t_run:.1f} s, "
+                      f"PS {ps[-1][1] if ps else 0}", file=sys.stderr, flush=True)
+    s.synchronize()
+    wall = time.perf_counter() - t_run
+    st = s.stats()
+    at = s.activationTimes()
+    ecg = s.ecgFrames()
+    n = mesh.n_nodes
+    late = [p for p in ps if p[0] >= s2 + 100.0]
+    out = {
+        "metric": "node-steps/s, 4.6M-node Mahajan slab, S1-S2 cross-field, 1 GPU",
+        "value": n * steps / wall, "unit": "node-steps/s", "n_gpus": 1, "steps": steps,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True,
+        "dtype": "fp32 gates/voltage/SpMV + fp64 Ca-cycling and Na_i accumulators, CG dots",
+        "config": {"workload": f"Mahajan 2008 slab 6x6x1 cm, h 0.02 cm, {n} nodes, sigma 1 mS/cm "
+                               f"isotropic, S1 x<=0.06 cm at 0 ms, S2 y<=3 cm at {s2:.1f} ms, "
+                               f"dt 0.01 ms, {args.mj_ms:g} ms",
+                   "nodes": n, "tets": mesh.n_elems, "nnz": st["nnz"]},
+        "cg_iters_per_step": st["cg_iterations"] / max(1, st["steps"]),
+        "cg_not_converged": st["cg_not_converged"],
+        "activated_frac": float((at >= 0).mean()),
+        "phase_singularities": {"series_ms_count_charge": ps[::4],
+                                "max": max((p[1] for p in ps), default=0),
+                                "mean_after_s2_plus_100ms": float(np.mean([p[1] for p in late])) if late else None},
+        "ecg_frames": int(ecg.shape[0]), "ecg_peak_abs": float(np.abs(ecg).max()) if ecg.size else None,
+        "setup_s": setup_s,
+    }
+    s.close()
+    print(json.dumps(out))
+
+
 def surrogate_kernel_timing(args, vg):
     """Cost of the learned step's kernel alone on the LV: an untrained
     network whose envelopes admit every node (increments scaled to ~1e-12 so
@@ -399,6 +499,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB 20 times (PMC byte "
                          "counter calibration) and exit")
+    ap.add_argument("--mahajan", action="store_true",
+                    help="the 4.6M-node Mahajan S1-S2 slab instead (phase singularities, ECG)")
+    ap.add_argument("--mj-ms", type=float, default=1000.0)
+    ap.add_argument("--mj-seed", type=int, default=7)
     ap.add_argument("--surrogate", action="store_true",
                     help="evaluate the learned ionic step on the LV instead (collect, train, compare)")
     ap.add_argument("--surr-ms", type=float, default=400.0)
@@ -430,6 +534,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 20}))
+        return
+    if args.mahajan:
+        mahajan_bench(args, vg)
         return
     if args.surrogate:
         if args.surr_kernel_only:
