@@ -152,7 +152,20 @@ typedef struct {
    * EP_CrankNicholson.cc:31-36): A and B get +-alpha (4/dt) K on top of
    * +-dt/4 K.  0 = off. */
   double smoothing;
+  /* Step scheme (EPSolver::New's Method, EPSolver.cc:339-361): VG_SCHEME_CN
+   * (default), VG_SCHEME_BE (backward-Euler half-steps with the lumped
+   * capacitance, EP_BackwardEuler.cc) or VG_SCHEME_EXPLICIT (forward-Euler
+   * half-steps in strides of dt_diffusion, EP_ExplicitSolver.cc).  The BE
+   * and explicit schemes advance the cells through
+   * CardiacBody::getVoltageFromIonicCurrent's per-node sub-steps
+   * (CardiacBody.cc:819-902). */
+  int scheme;
+  double dt_diffusion; /* explicit stride (ms); 0 = 0.1, Solve's default argument */
 } vg_model;
+
+#define VG_SCHEME_CN 0
+#define VG_SCHEME_BE 1
+#define VG_SCHEME_EXPLICIT 2
 
 /* Initial guess of the CG solves.  Either way a solve stops when
  * ||r||_D <= cg_tol ||K_rhs V||_D, the reference's ||r_0|| when it starts from

@@ -87,7 +87,11 @@ class _Sim(C.Structure):
         ("nthreads", C.c_int), ("tt_newton_jt", C.c_int),
         ("pk_flag", C.c_void_p), ("pk_states", C.c_void_p),
         ("mj_const", C.c_void_p), ("mj_int_abs", C.c_int),
+        ("scheme", C.c_int), ("dt_diff", C.c_double),
     ]
+
+
+SCHEME_CN, SCHEME_BE, SCHEME_EXPLICIT = 0, 1, 2
 
 
 def _declare(L):
@@ -118,11 +122,23 @@ def _declare(L):
     L.or_sim_free.argtypes = [C.POINTER(_Sim)]
     L.or_cn_step.argtypes = [C.POINTER(_Sim)]
     L.or_cn_step.restype = C.c_int
+    L.or_step.argtypes = [C.POINTER(_Sim)]
+    L.or_step.restype = C.c_int
+    L.or_diffuse_half.argtypes = [C.POINTER(_Sim)]
+    L.or_adaptive_steps.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int]
+    L.or_adaptive_steps.restype = C.c_int
     L.or_ecg_direct.argtypes = [C.POINTER(_Mesh), _dp, C.c_double, _dp, C.c_int, _dp, _dp]
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def adaptive_steps(dT, dt_ion, dv_first):
+    """Sub-steps of CardiacBody::getVoltageFromIonicCurrent for one node."""
+    out = np.zeros(4096)
+    k = lib().or_adaptive_steps(dT, dt_ion, dv_first, out, 4096)
+    return out[:k]
 
 
 class _MeshKeep:
@@ -291,7 +307,7 @@ class Simulation:
     def __init__(self, mesh, model, dt, dt_ion=None, Cm=1.0, cg_tol=1e-10, cg_maxit=1000,
                  D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, DpLeft=None, V0=None, states=None,
                  tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None,
-                 newton_jt=False, mj_const=None, mj_int_abs=False):
+                 newton_jt=False, mj_const=None, mj_int_abs=False, scheme=0, dt_diff=0.1):
         """Mahajan (model 3) takes each node's tissue type 0-8 from
         ``param_idx`` (default 0, Mesh.cc:186,375; types beyond 8 run the
         reference's fallback, type 2) and its initial state from that type's
@@ -345,6 +361,8 @@ class Simulation:
         self.mj_const = None if mj_const is None else np.ascontiguousarray(mj_const, np.float64)
         s.mj_const = _ptr(self.mj_const)
         s.mj_int_abs = int(bool(mj_int_abs))
+        s.scheme = scheme
+        s.dt_diff = dt_diff
         tissue = getattr(mesh, "tissue", None)
         self.pk_flag = None
         self.pk_states = None
@@ -378,9 +396,13 @@ class Simulation:
     def newton_iters_total(self):
         return self.s.newton_iters_total
 
+    def diffuse_half(self):
+        """The diffusion half-step of the scheme alone."""
+        lib().or_diffuse_half(C.byref(self.s))
+
     def step(self):
         Vold = self.V.copy()
-        lib().or_cn_step(C.byref(self.s))
+        lib().or_step(C.byref(self.s))
         t0 = self.t
         self.count += 1
         self.t = self.count * self.dt

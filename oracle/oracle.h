@@ -151,12 +151,30 @@ typedef struct {
   double *pk_states;    /* [n*22] AoS, used where pk_flag is set */
   const double *mj_const; /* [9*35] Mahajan constants by tissue type, or NULL */
   int mj_int_abs;       /* 1 = adaptive threshold read through int abs() */
+  int scheme;           /* OR_SCHEME_*: the step of EPSolver::New's Method */
+  double dt_diff;       /* explicit diffusion sub-step (Solve's dtDiff, 0.1 ms) */
 } or_sim;
+
+#define OR_SCHEME_CN 0       /* EP_CrankNicholson */
+#define OR_SCHEME_BE 1       /* EP_BackwardEuler */
+#define OR_SCHEME_EXPLICIT 2 /* EP_ExplicitSolver */
+
+/* Sub-steps of CardiacBody::getVoltageFromIonicCurrent
+ * (CardiacBody.cc:819-902) for a step dT, ionic step dt_ion and the dV/dt
+ * of the first sub-step: dt_ion, then strides of dt_ion (5 dt_ion when that
+ * first dV/dt truncates to int 0) up to dT, the last one clipped; a
+ * zero-length final call (reached when the strides land on dT exactly)
+ * changes nothing and is left out.  Returns the count (<= max). */
+int or_adaptive_steps(double dT, double dt_ion, double dv_first, double *out, int max);
 
 int or_sim_init(or_sim *s);
 void or_sim_free(or_sim *s);
 /* One EP_CrankNicholson::Solve. */
 int or_cn_step(or_sim *s);
+/* One EPSolver::Solve of the sim's scheme (CN, backward Euler, explicit). */
+int or_step(or_sim *s);
+/* The diffusion half of the sim's scheme alone (tests). */
+void or_diffuse_half(or_sim *s);
 
 /* ---- pseudo-ECG (ComputeECG.h:30-189) ---------------------------------- */
 void or_ecg_direct(const or_mesh *m, const double Decg[3], double radius,

@@ -346,10 +346,19 @@ int or_sim_init(or_sim *s) {
   s->B = malloc(sizeof(double) * nnz);
   s->rhs = malloc(sizeof(double) * n);
   if (!s->A || !s->B || !s->rhs) return -1;
-  /* EP_CrankNicholson.cc:22-29: A = (dT/4) K + Cm M, B = -(dT/4) K + Cm M. */
-  for (int64_t k = 0; k < nnz; k++) {
-    s->A[k] = s->dt / 4. * s->csr->K[k] + s->Cm * s->csr->M[k];
-    s->B[k] = -s->dt / 4. * s->csr->K[k] + s->Cm * s->csr->M[k];
+  if (s->scheme == OR_SCHEME_BE) {
+    /* EP_BackwardEuler.cc:14-27: A = (dT/2) K + diag(C_L). */
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t k = s->csr->row_ptr[i]; k < s->csr->row_ptr[i + 1]; k++) {
+        s->A[k] = s->dt / 2. * s->csr->K[k] + (s->csr->col[k] == i ? s->csr->CL[i] : 0.0);
+        s->B[k] = 0.0;
+      }
+  } else {
+    /* EP_CrankNicholson.cc:22-29: A = (dT/4) K + Cm M, B = -(dT/4) K + Cm M. */
+    for (int64_t k = 0; k < nnz; k++) {
+      s->A[k] = s->dt / 4. * s->csr->K[k] + s->Cm * s->csr->M[k];
+      s->B[k] = -s->dt / 4. * s->csr->K[k] + s->Cm * s->csr->M[k];
+    }
   }
   s->cg_iters_total = 0;
   s->newton_iters_total = 0;
@@ -440,6 +449,113 @@ int or_cn_step(or_sim *s) {
   }
   s->newton_iters_total += newton;
   diffusion_half(s);
+  return 0;
+}
+
+int or_adaptive_steps(double dT, double dt_ion, double dv_first, double *out, int max) {
+  int k = 0;
+  double dt = dt_ion;
+  if (!(dT / dt > 1.0)) {
+    if (max > 0) out[k++] = dt; /* one stride of dt_ion, even past dT */
+    return k;
+  }
+  if (k < max) out[k++] = dt;
+  double tot = dt;
+  /* int slope = iion; if (!slope) dt *= 5 (CardiacBody.cc:861-863) */
+  if (fabs(dv_first) < 1.0) dt *= 5.0;
+  for (;;) {
+    const int last = tot + dt > dT;
+    const double del = last ? dT - tot : dt;
+    if (del > 0.0 && k < max) out[k++] = del;
+    tot += del;
+    if (last) break;
+  }
+  return k;
+}
+
+/* dV/dt of node i over dt through its model (states advance). */
+static double node_ion(or_sim *s, int64_t i, double dt, const double *tc, const double *mt,
+                       const double *pkc, int64_t *newton) {
+  if (s->pk_flag && s->pk_flag[i])
+    return or_pk_compute_ion(pkc, s->pk_states + OR_PK_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i]);
+  if (s->model == OR_LR)
+    return or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i]);
+  if (s->model == OR_MJ) {
+    const int t = s->param_idx ? s->param_idx[i] : 0;
+    return or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 2),
+                             s->states + OR_MJ_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i],
+                             s->mj_int_abs);
+  }
+  const double *c = tc;
+  if (s->tt_const && s->param_idx) c = s->tt_const + (int64_t)OR_TT_NCONST * s->param_idx[i];
+  int it = 0;
+  const double dv = or_tt_compute_ion(c, s->states + OR_TT_NSTATE * i, s->Cm, dt, s->V[i],
+                                      s->istim[i], s->tt_newton_jt, &it);
+  *newton += it;
+  return dv;
+}
+
+/* CardiacBody::getVoltageFromIonicCurrent (CardiacBody.cc:819-902): each
+ * node steps on its own sub-step sequence. */
+static void ionic_adaptive(or_sim *s) {
+  double tc[OR_TT_NCONST], pkc[OR_PK_NCONST], mc[9 * OR_MJ_NCONST];
+  or_tt_constants(tc);
+  or_pk_constants(pkc);
+  for (int t = 0; t < 9; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
+  const double *mt = s->mj_const ? s->mj_const : mc;
+  int64_t newton = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : newton)
+  for (int64_t i = 0; i < s->n; i++) {
+    double dt = s->dt_ion;
+    double dv = node_ion(s, i, dt, tc, mt, pkc, &newton);
+    s->V[i] += dt * dv;
+    double seq[4096];
+    const int k = or_adaptive_steps(s->dt, s->dt_ion, dv, seq, 4096);
+    for (int q = 1; q < k; q++) {
+      dv = node_ion(s, i, seq[q], tc, mt, pkc, &newton);
+      s->V[i] += seq[q] * dv;
+    }
+  }
+  s->newton_iters_total += newton;
+}
+
+/* One half of EP_ExplicitSolver::Solve's diffusion (EP_ExplicitSolver.cc:
+ * 52-85): V += h (-K V) / C_L in strides h = min(dtDiff, dT/2) over dT/2. */
+static void explicit_half(or_sim *s) {
+  const or_csr *c = s->csr;
+  const double TOLER = 1e-6;
+  const double dtd = (s->dt_diff < 0.5 * s->dt) ? s->dt_diff : 0.5 * s->dt;
+  double solve = 0.5 * s->dt, h = dtd;
+  while (solve > TOLER) {
+    spmv(s->n, c->row_ptr, c->col, c->K, s->V, s->rhs);
+    for (int64_t i = 0; i < s->n; i++) s->V[i] += h * (-s->rhs[i]) / c->CL[i];
+    solve -= h;
+    h = (h < solve) ? h : solve;
+  }
+}
+
+/* One half of EP_BackwardEuler::Solve (EP_BackwardEuler.cc:64-67):
+ * (C_L + dT/2 K) V' = C_L V from x0 = V. */
+static void be_half(or_sim *s) {
+  const or_csr *c = s->csr;
+  for (int64_t i = 0; i < s->n; i++) s->rhs[i] = c->CL[i] * s->V[i];
+  s->cg_iters_total += or_pcg(s->n, c->row_ptr, c->col, s->A, s->rhs, s->V, s->cg_tol,
+                              s->cg_maxit, NULL);
+}
+
+void or_diffuse_half(or_sim *s) {
+  if (s->scheme == OR_SCHEME_BE) be_half(s);
+  else if (s->scheme == OR_SCHEME_EXPLICIT) explicit_half(s);
+  else diffusion_half(s);
+}
+
+int or_step(or_sim *s) {
+  if (s->scheme == OR_SCHEME_CN) return or_cn_step(s);
+  if (s->scheme == OR_SCHEME_BE) be_half(s);
+  else explicit_half(s);
+  ionic_adaptive(s);
+  if (s->scheme == OR_SCHEME_BE) be_half(s);
+  else explicit_half(s);
   return 0;
 }
 

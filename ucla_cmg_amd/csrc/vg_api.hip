@@ -155,6 +155,10 @@ struct vg_handle {
   int glaunch[2] = {0, 0};
   bool g_stim = false, g_at = false;
   unsigned long long g_notconv_seen = 0;
+  // Step scheme; the explicit scheme's diffusion strides of a half-step.
+  int scheme = VG_SCHEME_CN;
+  double dt_diff = 0.1;
+  std::vector<double> strides;
   // Learned ionic step: the attached network, its device view, the
   // per-row fallback flags and the usage counters (two per wave of the
   // persistent grid).  rec_*: recording of the plain update for training.
@@ -446,15 +450,16 @@ void launch_ecg_finish(vg_handle *h, double *dst) {
                      (int)h->node_blocks, h->ne, dst);
 }
 
+// V_a = from (+ D^-1/2 x when x is given and valid), activation, ECG frame.
 template <class T>
-void launch_apply(vg_handle *h, bool ecg_frame) {
+void apply_from(vg_handle *h, const T *from, const T *x, bool ecg_frame) {
   ProfScope ps(h, VG_KC_APPLY);
   const T thr = (T)h->at_thr;
   const T *Hm = h->H.p ? h->H.as<T>() : nullptr;
   double *part = h->ecg_part.p ? h->ecg_part.as<double>() : nullptr;
 #define VG_APPLY(AT, ECG)                                                                        \
   hipLaunchKernelGGL((k_apply<T, AT, ECG>), dim3(h->node_blocks), dim3(BLK), 0, h->stream,       \
-                     h->Vb.as<T>(), xbuf<T>(h, 1), h->sinv.as<T>(), h->Va.as<T>(), h->rows,      \
+                     from, x, h->sinv.as<T>(), h->Va.as<T>(), h->rows,                            \
                      h->at.as<T>(), thr, h->step_ctr.as<int64_t>(), h->dt, Hm, h->ne,           \
                      h->n_alloc, part,                                                            \
                      h->res.as<CgResult>(), slice_inst_of(h))
@@ -468,6 +473,11 @@ void launch_apply(vg_handle *h, bool ecg_frame) {
 #undef VG_APPLY
   if (ecg_frame)
     launch_ecg_finish(h, h->ecg_frames.as<double>() + (size_t)h->ecg_count * h->ne);
+}
+
+template <class T>
+void launch_apply(vg_handle *h, bool ecg_frame) {
+  apply_from<T>(h, h->Vb.as<T>(), xbuf<T>(h, 1), ecg_frame);
 }
 
 int upload_stim(vg_handle *h) {
@@ -512,6 +522,41 @@ int step_t(vg_handle *h) {
   const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
                      h->ecg_count < h->ecg_max;
   launch_apply<T>(h, frame);
+  launch_fold(h, 1, false, true);
+  if (frame) h->ecg_count += 1;
+  return VG_OK;
+}
+
+// One step of EP_ExplicitSolver::Solve (EP_ExplicitSolver.cc:23-118):
+// forward-Euler diffusion strides over dT/2, the per-node adaptive ionic
+// update, strides over dT/2 again.  Strides ping-pong between two work
+// vectors (Va keeps V^n for the activation test); k_apply writes Va.
+template <class T>
+int step_explicit(vg_handle *h) {
+  int rc = upload_stim(h);
+  if (rc) return rc;
+  const unsigned grid = (unsigned)std::min<int64_t>(h->cg_blocks ? h->cg_blocks : 2048,
+                                                    (h->n_slices + WAVES - 1) / WAVES);
+  T *W1 = h->p.as<T>(), *W2 = h->s.as<T>();
+  auto half = [&](const T *from) {
+    const T *src = from;
+    T *dst = W1;
+    for (double s : h->strides) {
+      hipLaunchKernelGGL(k_explicit<T>, dim3(grid), dim3(BLK), 0, h->stream, sell_of(h),
+                         h->Krhs.as<T>(), src, dst, (T)s, h->n_slices);
+      src = dst;
+      dst = dst == W1 ? W2 : W1;
+    }
+    return src;
+  };
+  const T *v1 = half(h->Va.as<T>());
+  if ((rc = launch_ionic<T>(h, v1, (const T *)nullptr, h->Vb.as<T>()))) return rc;
+  const T *v2 = half(h->Vb.as<T>());
+  h->steps += 1;
+  h->t = h->steps * h->dt;
+  const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
+                     h->ecg_count < h->ecg_max;
+  apply_from<T>(h, v2, (const T *)nullptr, frame);
   launch_fold(h, 1, false, true);
   if (frame) h->ecg_count += 1;
   return VG_OK;
@@ -900,8 +945,38 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->guess = model->cg_x0 != VG_X0_ZERO ? 1 : 0;
   h->shared_x = model->cg_x0 == VG_X0_LAST;
   h->tsz = h->prec == VG_PREC_FP64 ? 8 : 4;
+  if (model->scheme != VG_SCHEME_CN && model->scheme != VG_SCHEME_BE &&
+      model->scheme != VG_SCHEME_EXPLICIT) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "unknown scheme");
+  }
+  if (model->scheme != VG_SCHEME_CN && (model->smoothing > 0.0 || model->n_instances > 1)) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "smoothing and batching need the Crank-Nicolson scheme");
+  }
+  h->scheme = model->scheme;
+  h->dt_diff = model->dt_diffusion > 0.0 ? model->dt_diffusion : 0.1;
+  if (h->scheme == VG_SCHEME_EXPLICIT) {
+    // Strides of min(dtDiff, dT/2) over dT/2 (EP_ExplicitSolver.cc:35-37, 52-85).
+    const double TOLER = 1e-6;
+    double solve = 0.5 * h->dt, hs = std::min(h->dt_diff, 0.5 * h->dt);
+    while (solve > TOLER) {
+      if (h->strides.size() >= 100000) {
+        *out = h.release();
+        return (*out)->fail(VG_ERR_ARG, "too many explicit strides per half-step");
+      }
+      h->strides.push_back(hs);
+      solve -= hs;
+      hs = std::min(hs, solve);
+    }
+  }
   vg_handle *hp = h.get();
-  if (!substeps(h->dt, h->dt_ion, h->sub)) {
+  if (h->scheme != VG_SCHEME_CN) {
+    h->sub = SubSteps{};
+    h->sub.adaptive = 1;
+    h->sub.dT = h->dt;
+    h->sub.dt_ion = h->dt_ion;
+  } else if (!substeps(h->dt, h->dt_ion, h->sub)) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "too many ionic sub-steps per step");
   }
@@ -960,7 +1035,11 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   // Diffusive smoothing adds alpha (4/dT) K to A and subtracts it from B
   // (EP_CrankNicholson.cc:31-36).
   const double kc = h->dt / 4.0 + (model->smoothing > 0.0 ? 4.0 * model->smoothing / h->dt : 0.0);
-  e = build_sell(csr, kc, h->Cm, model->sell_sigma, base, h->n_myo, sell);
+  if (h->scheme == VG_SCHEME_BE)
+    e = build_sell(csr, h->dt / 2.0, h->dt / 2.0, h->Cm, true, model->sell_sigma, base, h->n_myo,
+                   sell);
+  else
+    e = build_sell(csr, kc, 2.0 * kc, h->Cm, false, model->sell_sigma, base, h->n_myo, sell);
   if (!e.empty()) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, e);
@@ -1012,12 +1091,14 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   std::copy(sell.diag.begin(), sell.diag.begin() + std::min<int64_t>(na, sell.diag.size()), diag_pad.begin());
   if (T == 8) {
     if ((rc = upload_raw<double>(H_, H_->Ahat, sell.A_hat))) return rc;
-    if ((rc = upload_raw<double>(H_, H_->Krhs, sell.K_rhs))) return rc;
+    if ((rc = upload_raw<double>(H_, H_->Krhs, H_->scheme == VG_SCHEME_EXPLICIT ? sell.K_exp : sell.K_rhs)))
+      return rc;
     if ((rc = upload_raw<double>(H_, H_->sinv, sinv_pad))) return rc;
     if ((rc = upload_raw<double>(H_, H_->diag, diag_pad))) return rc;
   } else {
     if ((rc = upload_raw<float>(H_, H_->Ahat, sell.A_hat))) return rc;
-    if ((rc = upload_raw<float>(H_, H_->Krhs, sell.K_rhs))) return rc;
+    if ((rc = upload_raw<float>(H_, H_->Krhs, H_->scheme == VG_SCHEME_EXPLICIT ? sell.K_exp : sell.K_rhs)))
+      return rc;
     if ((rc = upload_raw<float>(H_, H_->sinv, sinv_pad))) return rc;
     if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
   }
@@ -1234,7 +1315,9 @@ int vg_step(vg_handle *h, int nsteps) {
   const bool graph = h->graph && !h->profiling;
   for (int k = 0; k < nsteps; k++) {
     int rc;
-    if (graph) rc = h->tsz == 8 ? step_graph<double>(h) : step_graph<float>(h);
+    if (h->scheme == VG_SCHEME_EXPLICIT)
+      rc = h->tsz == 8 ? step_explicit<double>(h) : step_explicit<float>(h);
+    else if (graph) rc = h->tsz == 8 ? step_graph<double>(h) : step_graph<float>(h);
     else rc = h->tsz == 8 ? step_t<double>(h) : step_t<float>(h);
     if (rc) return rc;
   }

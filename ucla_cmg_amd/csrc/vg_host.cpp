@@ -237,7 +237,7 @@ std::string assemble_csr(const vg_mesh &m, const vg_model &mod, HostCsr &out) {
   return "";
 }
 
-std::string build_sell(const HostCsr &c, double kc, double Cm, int sigma,
+std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool lumped, int sigma,
                        const std::vector<int32_t> &base, int64_t n_first, HostSell &o) {
   const int64_t n = c.n;
   const int64_t rows = (int64_t)base.size();
@@ -248,7 +248,7 @@ std::string build_sell(const HostCsr &c, double kc, double Cm, int sigma,
   std::vector<double> d(n, 0.0);
   for (int64_t i = 0; i < n; i++) {
     for (int64_t k = c.row_ptr[i]; k < c.row_ptr[i + 1]; k++)
-      if (c.col[k] == i) d[i] = kc * c.K[k] + Cm * c.M[k];
+      if (c.col[k] == i) d[i] = kc * c.K[k] + (lumped ? c.CL[i] : Cm * c.M[k]);
     if (!(d[i] > 0.0)) return "non-positive diagonal of the CN operator at node " + std::to_string(i);
   }
   if (rows < n || n_first < 0 || n_first > rows) return "bad row order";
@@ -296,6 +296,7 @@ std::string build_sell(const HostCsr &c, double kc, double Cm, int sigma,
   o.col.assign(o.padded, 0);
   o.A_hat.assign(o.padded, 0.0);
   o.K_rhs.assign(o.padded, 0.0);
+  o.K_exp.assign(o.padded, 0.0);
   o.sinv.assign(n_pad, 0.0);
   o.diag.assign(n_pad, 0.0);
   // Padding rows (no node) carry D = 1 and no entries: they stay zero.
@@ -321,10 +322,12 @@ std::string build_sell(const HostCsr &c, double kc, double Cm, int sigma,
           const int32_t cn = row[j].first;
           const int64_t k = row[j].second;
           const int32_t old_r = o.new2old[r], old_c = c.col[k];
-          const double a = kc * c.K[k] + Cm * c.M[k];
+          const double mass = lumped ? (old_c == old_r ? c.CL[old_r] : 0.0) : Cm * c.M[k];
+          const double a = kc * c.K[k] + mass;
           o.col[slot] = cn;
           o.A_hat[slot] = a / std::sqrt(d[old_r] * d[old_c]);
-          o.K_rhs[slot] = -(2.0 * kc) * c.K[k] / std::sqrt(d[old_r]);
+          o.K_rhs[slot] = -rc * c.K[k] / std::sqrt(d[old_r]);
+          o.K_exp[slot] = -c.K[k] / c.CL[old_r];
         } else {
           o.col[slot] = (int32_t)std::min<int64_t>(r, n - 1);  // in-bounds pad, value 0
         }

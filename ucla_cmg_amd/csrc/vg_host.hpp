@@ -35,22 +35,25 @@ struct HostSell {
   std::vector<int16_t> col16;    // [padded]
   std::vector<int32_t> col32;    // wide slices only, same lane-fastest layout
   std::vector<int64_t> wide;     // [n_slices]
-  std::vector<double> A_hat;     // D^-1/2 (Cm M + kc K) D^-1/2
-  std::vector<double> K_rhs;     // -2 kc D^-1/2 K
+  std::vector<double> A_hat;     // D^-1/2 (mass + kc K) D^-1/2
+  std::vector<double> K_rhs;     // -rc D^-1/2 K
+  std::vector<double> K_exp;     // -C_L^-1 K (unscaled; the explicit scheme)
   std::vector<double> sinv;      // [n_pad] D^-1/2
   std::vector<double> diag;      // [n_pad] D = diag(Cm M + kc K)
   std::vector<int32_t> new2old, old2new;
 };
 
-// Builds the scaled CN operators in SELL-64 form, A = Cm M + kc K and
-// B = Cm M - kc K (kc = dt/4, plus 4 alpha/dt with diffusive smoothing,
-// EP_CrankNicholson.cc:22-36); the solve is for the increment,
-// A x = (B - A) V = -2 kc K V.  `base` lists the mesh
+// Builds the scaled half-step operators in SELL-64 form: A = mass + kc K
+// with the solve for the increment A x = -rc K V; mass = Cm M (consistent)
+// or C_L on the diagonal (lumped).  Crank-Nicolson: kc = dt/4 (plus
+// 4 alpha/dt with diffusive smoothing, EP_CrankNicholson.cc:22-36),
+// rc = 2 kc, consistent; backward Euler: kc = rc = dt/2, lumped
+// (EP_BackwardEuler.cc:14-27).  `base` lists the mesh
 // nodes in the solver's row order, -1 marking a padding row (no node, no
 // entries); rows [0, n_first) and [n_first, rows) form two segments, and rows
 // are reordered by descending length inside windows of `sigma` rows of one
 // segment when sigma > 1 (not with padding rows).
-std::string build_sell(const HostCsr &c, double kc, double Cm, int sigma,
+std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool lumped, int sigma,
                        const std::vector<int32_t> &base, int64_t n_first, HostSell &out);
 
 // Lead-field matrix of the pseudo-ECG: ecg_e = sum_n H[e][n] V_n with

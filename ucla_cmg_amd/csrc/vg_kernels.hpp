@@ -33,7 +33,39 @@ constexpr int XCDS = 8;
 struct SubSteps {
   int n;
   double dt[MAX_SUB];
+  // Per-node sub-steps of CardiacBody::getVoltageFromIonicCurrent
+  // (CardiacBody.cc:819-902) instead of the list: dt_ion, then strides of
+  // dt_ion (5 dt_ion when the first dV/dt truncates to int 0) up to dT.
+  int adaptive;
+  double dT, dt_ion;
 };
+
+// Runs step(dt) (which advances a node by dt and returns its dV/dt) over the
+// sub-steps of one ionic update.  The adaptive form's zero-length final call
+// (strides landing exactly on dT) changes nothing and is skipped.
+template <class F>
+__device__ inline void ionic_substeps(const SubSteps &sub, F &&step) {
+  if (!sub.adaptive) {
+    for (int q = 0; q < sub.n; q++) step(sub.dt[q]);
+    return;
+  }
+  double dt = sub.dt_ion;
+  const double dT = sub.dT;
+  if (!(dT / dt > 1.0)) {
+    step(dt);
+    return;
+  }
+  const double dv = step(dt);
+  double tot = dt;
+  if (fabs(dv) < 1.0) dt *= 5.0;  // int slope = iion; if (!slope) dt *= 5
+  for (;;) {
+    const bool last = tot + dt > dT;
+    const double del = last ? dT - tot : dt;
+    if (del > 0.0) step(del);
+    tot += del;
+    if (last) break;
+  }
+}
 
 // CG scalars carried from one iteration launch to the next; launch j reads
 // carry[j & 1] and block 0 writes carry[(j + 1) & 1].
@@ -255,6 +287,22 @@ __device__ inline bool cg_stopped(const CgResult *res) {
 
 __device__ inline void cg_set_stop(CgResult *res) {
   __hip_atomic_store(&res->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- explicit diffusion stride ------------------------------------------------
+// Vout = Vin + h (-C_L^-1 K) Vin, one row per lane (EP_ExplicitSolver.cc:
+// 52-85; Kexp holds -K / C_L in the SELL layout).
+template <class T>
+__global__ __launch_bounds__(BLK) void k_explicit(Sell S, const T *__restrict__ Kexp,
+                                                  const T *__restrict__ Vin, T *__restrict__ Vout,
+                                                  T h, int64_t n_slices) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t sl = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); sl < n_slices;
+       sl += (int64_t)gridDim.x * WAVES) {
+    const int64_t row = sl * 64 + lane;
+    const T acc = sell_row<T>(S, Kexp, sl, lane, [&](int c) { return Vin[c]; });
+    Vout[row] = Vin[row] + h * acc;
+  }
 }
 
 // ---- diffusion right-hand side ---------------------------------------------
@@ -490,11 +538,12 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       T s[LR_NSTATE];
 #pragma unroll
       for (int k = 0; k < LR_NSTATE - 1; k++) s[k] = st[k * a.n_alloc + i];
-      for (int q = 0; q < sub.n; q++) {
-        const T dt = (T)sub.dt[q];
+      ionic_substeps(sub, [&](double dtd) {
+        const T dt = (T)dtd;
         const T dv = lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3]);
         V += dt * dv;
-      }
+        return (double)dv;
+      });
 #pragma unroll
       for (int k = 0; k < LR_NSTATE - 1; k++) st[k * a.n_alloc + i] = s[k];
     } else if constexpr (MODEL == VG_MODEL_MJ) {
@@ -510,10 +559,12 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
         s[MJ_ACC[k]] = (T)ac[k];
       }
       const T it = stim / (Cm * T(MJ_XI));
-      for (int q = 0; q < sub.n; q++) {
-        const T dt = (T)sub.dt[q];
-        V += dt * mj_step<T, sizeof(T) == 4>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
-      }
+      ionic_substeps(sub, [&](double dtd) {
+        const T dt = (T)dtd;
+        const T dv = mj_step<T, sizeof(T) == 4>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
+        V += dt * dv;
+        return (double)dv;
+      });
 #pragma unroll
       for (int k = 0; k < MJ_NGATE; k++) st[k * a.n_alloc + i] = s[MJ_GATE[k]];
 #pragma unroll
@@ -538,8 +589,8 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
           for (int q = 0; q < VG_TT_NSCALE; q++) rx[20 + q] = (float)sc[q];
         }
       }
-      for (int q = 0; q < sub.n; q++) {
-        const T dt = (T)sub.dt[q];
+      ionic_substeps(sub, [&](double dtd) {
+        const T dt = (T)dtd;
         T dv;
         if constexpr (INTEG == VG_INTEG_REFERENCE) {
           int nit = 0;
@@ -549,7 +600,8 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
           dv = tt_step_rl<T, sizeof(T) == 4>(s, V, it, dt, g);
         }
         V += dt * dv;
-      }
+        return (double)dv;
+      });
       if (ry) {
         // Increments from the unrounded start values (still in memory).
         T V0 = Vin[i];
@@ -591,10 +643,12 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
   T s[PK_NSTATE];
 #pragma unroll
   for (int k = 0; k < PK_NSTATE; k++) s[k] = st[k * n_alloc + i];
-  for (int q = 0; q < sub.n; q++) {
-    const T dt = (T)sub.dt[q];
-    V += dt * pk_step<T, sizeof(T) == 4>(s, V, stim, dt);
-  }
+  ionic_substeps(sub, [&](double dtd) {
+    const T dt = (T)dtd;
+    const T dv = pk_step<T, sizeof(T) == 4>(s, V, stim, dt);
+    V += dt * dv;
+    return (double)dv;
+  });
 #pragma unroll
   for (int k = 0; k < PK_NSTATE; k++) st[k * n_alloc + i] = s[k];
   Vout[row] = V;
@@ -833,7 +887,7 @@ __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T
 #pragma unroll
   for (int e = 0; e < MAX_ECG; e++) v[e] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
-    const bool use_x = cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid != 0;
+    const bool use_x = x && cg[slice_inst ? slice_inst[i >> 6] : 0].x_valid != 0;
     const T Vn = use_x ? Vb[i] + sinv[i] * x[i] : Vb[i];
     if constexpr (AT) {
       const T Vo = Va[i];

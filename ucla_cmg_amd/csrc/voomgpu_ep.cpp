@@ -13,14 +13,19 @@
 // WRITERESTART t, STIMULUS NODAL|CIRC|ZONAL|PLANE ... (EPSimulator.cc:147-266).
 // Mechanics keywords are read and ignored.  Keywords of this driver only:
 // GPUPRECISION FP64|MIXED, IONICINTEGRATOR REFERENCE|RUSHLARSEN, CGX0
-// ZERO|PREVIOUS|LAST, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE.
+// ZERO|PREVIOUS|LAST, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE, and
+// SOLVERTYPE CRANKNICKOLSON|BACKWARDEULER|EXPLICIT (the reference fixes
+// CRANKNICKOLSON, EPSimulator.cc:88, and has getSolverType for the others).
 //
 // Behaviour that differs from the reference on purpose (each would be a
 // defect there): a zonal stimulus is stopped without falling through into
 // the planar case (EPSimulator.cc:552-556); a planar stimulus is stopped with
 // its own centre and normal (:556-558 use the start counter's); LVPURKINJE-
 // DIFFUSION is honoured (the reference parses it and passes Dp, :430);
-// WRITERESTART below one step writes no restart (the reference divides by 0).
+// DIFFUSIONDT sets the explicit scheme's stride (the reference parses it and
+// solves with Solve()'s default 0.1); WRITERESTART below one step writes no
+// restart (the reference divides by 0); ZONAL axis letters are read in
+// either case (the reference's map lookup is case-sensitive).
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -86,6 +91,9 @@ int main(int argc, char **argv) {
   std::string model_name, out_dir = "Output/", cell_model = "MAHAJAN", mesh_type = "MIX3D",
                           initial_states;
   double dt_ionic = 0.1, sim_time = 300.0, dt = 0.1, frequency = 1.0, radius = 0.0275;
+  double dt_diffusion = 0.1;
+  int scheme = VG_SCHEME_CN;
+  bool tol_given = false;
   double D[3] = {0.001, 0.0005, 0.00025}, Dp[3] = {0.0032, 0.0032, 0.0032},
          DpLeft[3] = {0.0032, 0.0032, 0.0032};
   bool read_restart = false, write_restart = false;
@@ -127,7 +135,15 @@ int main(int argc, char **argv) {
       cg_x0 = v == "LAST" ? VG_X0_LAST : v == "PREVIOUS" ? VG_X0_PREVIOUS : VG_X0_ZERO;
     } else if (k == "GRAPH") graph = (int)num(t, 1);
     else if (k == "QUIRKS") quirks = (unsigned)num(t, 1);
-    else if (k == "CGTOLERANCE") cg_tol = num(t, 1);
+    else if (k == "CGTOLERANCE") cg_tol = num(t, 1), tol_given = true;
+    else if (k == "DIFFUSIONDT") dt_diffusion = num(t, 1);
+    else if (k == "SOLVERTYPE") {
+      const std::string v = upper(t.at(1));
+      if (v == "EXPLICIT") scheme = VG_SCHEME_EXPLICIT;
+      else if (v == "BACKWARDEULER") scheme = VG_SCHEME_BE;
+      else if (v == "CRANKNICKOLSON") scheme = VG_SCHEME_CN;
+      else die("Unknown Solver type " + t[1]);
+    }
     else if (k == "STIMULUS") {
       Stim s;
       static const std::map<std::string, int> types = {
@@ -191,6 +207,10 @@ int main(int argc, char **argv) {
   model.cg_x0 = cg_x0;
   model.graph = graph;
   model.smoothing = smoothing;
+  model.scheme = scheme;
+  model.dt_diffusion = dt_diffusion;
+  // EP_BackwardEuler iterates to 1e-10 (EP_BackwardEuler.cc:65), CN to 1e-7.
+  if (scheme == VG_SCHEME_BE && !tol_given) model.cg_tol = 1e-10;
   vg_handle *h = nullptr;
   const int rc = vg_create(mesh, &model, &h);
   if (rc != VG_OK) {

@@ -38,6 +38,7 @@ QUIRK_NEWTON_JT = 8
 QUIRK_MJ_INT_ABS = 16
 TT_NSCALE = 5
 X0_ZERO, X0_PREVIOUS, X0_LAST = 0, 1, 2
+SCHEME_CN, SCHEME_BE, SCHEME_EXPLICIT = 0, 1, 2
 KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY = range(5)
 KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply"]
 
@@ -87,7 +88,8 @@ class VgModel(C.Structure):
         ("n_param_sets", C.c_int), ("param_scales", C.c_void_p),
         ("param_index", C.c_void_p), ("sell_sigma", C.c_int), ("cg_blocks", C.c_int),
         ("cg_x0", C.c_int), ("n_instances", C.c_int), ("instance", C.c_void_p),
-        ("graph", C.c_int), ("smoothing", C.c_double),
+        ("graph", C.c_int), ("smoothing", C.c_double), ("scheme", C.c_int),
+        ("dt_diffusion", C.c_double),
     ]
 
 
@@ -237,7 +239,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
                param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO, instance=None,
-               graph=0, smoothing=0.0):
+               graph=0, smoothing=0.0, scheme=0, dt_diffusion=0.0):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -267,6 +269,8 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
     m.cg_x0 = cg_x0
     m.graph = graph
     m.smoothing = smoothing
+    m.scheme = scheme
+    m.dt_diffusion = dt_diffusion
     if instance is not None:
         ins = np.ascontiguousarray(instance, np.int32)
         keep.append(ins)
