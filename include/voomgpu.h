@@ -52,6 +52,10 @@ extern "C" {
 /* Mahajan 2008 rabbit ventricle: each node's constants and initial state by
  * its tissue type 0-8 (vg_mesh.tissue; epi/myo/endo x apex/center/base). */
 #define VG_MODEL_MJ 3
+/* Mahajan with the failing-heart constant set (CellModelType MAHAJAN_FAIL,
+ * Mahajan_fail.cc + SetUpMahajan_failParameters, SetUpIonicConstants.h:254-315):
+ * same states and initial values, other constants. */
+#define VG_MODEL_MJ_FAIL 4
 /* Corrias rabbit Purkinje (IonicModel/Purkinje.cc); in a tissue the model of
  * every Purkinje node (tissue 11/12), here also selectable for vg_cells_*. */
 #define VG_MODEL_PURKINJE 10

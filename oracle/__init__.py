@@ -464,6 +464,7 @@ def pk_compute_ion(const, state, Cm, dt, V, istim):
 # ---- Mahajan 2008 ------------------------------------------------------------
 def _declare_mj(L):
     L.or_mj_constants.argtypes = [C.c_int, _dp]
+    L.or_mj_fail_constants.argtypes = [C.c_int, _dp]
     L.or_mj_init.argtypes = [C.c_int, _dp]
     L.or_mj_xi.restype = C.c_double
     L.or_mj_rates.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp]
@@ -478,6 +479,22 @@ def mj_constants(tissue_type=2):
     c = np.zeros(MJ_NCONST)
     L.or_mj_constants(int(tissue_type), c)
     return c
+
+
+def mj_fail_constants(tissue_type=2):
+    """MAHAJAN_FAIL's constants (SetUpMahajan_failParameters)."""
+    L = lib()
+    _declare_mj(L)
+    c = np.zeros(MJ_NCONST)
+    L.or_mj_fail_constants(int(tissue_type), c)
+    return c
+
+
+def mj_table(fail=False):
+    """[10, 35] constants by tissue type, row 9 for types beyond 8 (the
+    ``mj_const`` argument of :class:`Simulation`)."""
+    f = mj_fail_constants if fail else mj_constants
+    return np.array([f(t) for t in range(10)])
 
 
 def mj_init(tissue_type=2):

@@ -122,6 +122,7 @@ double or_pk_compute_ion(const double *c, double *s, double Cm, double dt, doubl
 #define OR_MJ_NSTATE 26
 #define OR_MJ_NCONST 35
 void or_mj_constants(int type, double *c);
+void or_mj_fail_constants(int type, double *c); /* MAHAJAN_FAIL's set */
 void or_mj_init(int type, double *s);
 double or_mj_xi(void);
 void or_mj_rates(const double *c, const double *s, double h, double istim_uA_per_uF,
@@ -149,7 +150,8 @@ typedef struct {
   int tt_newton_jt;     /* 1 = solve Newton steps with J^T like the reference */
   const int8_t *pk_flag; /* [n] 1 = Purkinje node (runs the Purkinje model), or NULL */
   double *pk_states;    /* [n*22] AoS, used where pk_flag is set */
-  const double *mj_const; /* [9*35] Mahajan constants by tissue type, or NULL */
+  const double *mj_const; /* [10*35] Mahajan constants by tissue type (row 9:
+                             types beyond 8), or NULL (the normal set) */
   int mj_int_abs;       /* 1 = adaptive threshold read through int abs() */
   int scheme;           /* OR_SCHEME_*: the step of EPSolver::New's Method */
   double dt_diff;       /* explicit diffusion sub-step (Solve's dtDiff, 0.1 ms) */

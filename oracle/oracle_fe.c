@@ -416,14 +416,14 @@ int or_cn_step(or_sim *s) {
     } else if (s->model == OR_MJ) {
       /* CardiacBody::New case 3 (CardiacBody.cc:1045-1067): constants and
        * initial state by the node's tissue type. */
-      double mc[9 * OR_MJ_NCONST];
-      for (int t = 0; t < 9; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
+      double mc[10 * OR_MJ_NCONST]; /* by type; row 9: the fallback of other types */
+      for (int t = 0; t < 10; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
       const double *mt = s->mj_const ? s->mj_const : mc;
 #pragma omp parallel for schedule(dynamic, 256)
       for (int64_t i = 0; i < n; i++) {
         if (pkf && pkf[i]) continue;
         const int t = s->param_idx ? s->param_idx[i] : 0; /* Mesh.cc:186,375 */
-        const double dv = or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 2),
+        const double dv = or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 9),
                                             s->states + OR_MJ_NSTATE * i, s->Cm, dt, s->V[i],
                                             s->istim[i], s->mj_int_abs);
         s->V[i] += dt * dv;
@@ -482,7 +482,7 @@ static double node_ion(or_sim *s, int64_t i, double dt, const double *tc, const 
     return or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i]);
   if (s->model == OR_MJ) {
     const int t = s->param_idx ? s->param_idx[i] : 0;
-    return or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 2),
+    return or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 9),
                              s->states + OR_MJ_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i],
                              s->mj_int_abs);
   }
@@ -498,10 +498,10 @@ static double node_ion(or_sim *s, int64_t i, double dt, const double *tc, const 
 /* CardiacBody::getVoltageFromIonicCurrent (CardiacBody.cc:819-902): each
  * node steps on its own sub-step sequence. */
 static void ionic_adaptive(or_sim *s) {
-  double tc[OR_TT_NCONST], pkc[OR_PK_NCONST], mc[9 * OR_MJ_NCONST];
+  double tc[OR_TT_NCONST], pkc[OR_PK_NCONST], mc[10 * OR_MJ_NCONST];
   or_tt_constants(tc);
   or_pk_constants(pkc);
-  for (int t = 0; t < 9; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
+  for (int t = 0; t < 10; t++) or_mj_constants(t, mc + OR_MJ_NCONST * t);
   const double *mt = s->mj_const ? s->mj_const : mc;
   int64_t newton = 0;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : newton)

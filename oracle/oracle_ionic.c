@@ -638,6 +638,29 @@ void or_mj_constants(int type, double *c) {
   c[34] = 1.0;                         /* tau_j multiplier */
 }
 
+/* The failing-heart constant set, SetUpMahajan_failParameters
+ * (SetUpIonicConstants.h:254-315): the same model with slower uptake,
+ * stronger exchange, weaker I_K1 / I_to / I_Ks, a lower release threshold
+ * and leaky Na gates. */
+void or_mj_fail_constants(int type, double *c) {
+  static const double GTOF[9] = {0.12, 0.12, 0.12, 0.12, 0.12, 0.12, 0.12 * 0.77, 0.12 * 0.77,
+                                 0.12 * 0.77};
+  static const double GKS[9] = {1.5, 1.3, 1.1, 1.0, 0.9, 0.8, 1.1, 1.0, 0.9};
+  or_mj_constants(type, c);
+  c[6] = 0.04 * 0.65;
+  c[7] = (type >= 0 && type < 9 ? GTOF[type] : 0.12) * 0.67;
+  c[10] = (type >= 0 && type < 9 ? GKS[type] : 1.0) * 0.1386 * 0.6;
+  c[8] = 0.84 * 2.0;
+  c[11] = 0.3 * 0.5;
+  c[13] = 0.27;
+  c[14] = 6.0;
+  c[17] = 200.0;
+  c[18] = 22.5;
+  c[19] = 50.0;
+  c[26] = 10.0;
+  c[31] = c[32] = c[33] = 0.01;
+}
+
 /* Initial states by tissue type (Mahajan.cc:27-158; out-of-range types take
  * the reference's default row, equal to type 2). */
 static const double MJ_INIT[9][OR_MJ_NSTATE] = {

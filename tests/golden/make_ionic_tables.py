@@ -82,6 +82,71 @@ def mahajan_tables(const_h, mj_cc):
     }
 
 
+def arith(expr, C):
+    """Value of an arithmetic expression of numbers, Constants[k], + - * /
+    and parentheses (a small parser: the reference text is never executed)."""
+    toks = re.findall(r"Constants\[\s*\d+\s*\]|[0-9]*\.?[0-9]+(?:[eE][-+]?\d+)?|[-+*/()]", expr)
+    pos = [0]
+
+    def peek():
+        return toks[pos[0]] if pos[0] < len(toks) else None
+
+    def take():
+        pos[0] += 1
+        return toks[pos[0] - 1]
+
+    def atom():
+        t = take()
+        if t == "(":
+            v = add()
+            assert take() == ")"
+            return v
+        if t == "-":
+            return -atom()
+        if t.startswith("Constants"):
+            return C[int(re.search(r"\d+", t).group(0))]
+        return float(t)
+
+    def mul():
+        v = atom()
+        while peek() in ("*", "/"):
+            v = v * atom() if take() == "*" else v / atom()
+        return v
+
+    def add():
+        v = mul()
+        while peek() in ("+", "-"):
+            v = v + mul() if take() == "+" else v - mul()
+        return v
+
+    v = add()
+    assert pos[0] == len(toks), expr
+    return v
+
+
+def run_constants(body, pos):
+    """The 35 constants a SetUp*Parameters(constants, pos) body leaves:
+    assignments (= and *=) in order, the switch on pos taking its case or
+    default block."""
+    sw = re.search(r"switch\s*\(\s*pos\s*\)\s*\{", body)
+    i = sw.end()
+    depth, k = 1, i
+    while depth:
+        depth += {"{": 1, "}": -1}.get(body[k], 0)
+        k += 1
+    pre, inside, post = body[:sw.start()], body[i:k - 1], body[k:]
+    cases = {int(m.group(1)): m.group(2)
+             for m in re.finditer(r"case\s+(\d+)\s*:\s*\{([^}]*)\}", inside)}
+    default = re.search(r"default\s*:\s*\{([^}]*)\}", inside).group(1)
+    C = {}
+    for part in (pre, cases.get(pos, default), post):
+        for m in re.finditer(r"Constants\[\s*(\d+)\s*\]\s*(\*?=)\s*([^;]+);", part):
+            idx, op, expr = int(m.group(1)), m.group(2), m.group(3)
+            v = arith(expr, C)
+            C[idx] = C[idx] * v if op == "*=" else v
+    return [C[j] for j in range(35)]
+
+
 def main():
     const_h = strip_comments(open(os.path.join(REF, "SetUpIonicConstants.h")).read())
     tt = indexed_literals(function_body(const_h, "void SetUpTusscherParameters"), "Constants")
@@ -107,6 +172,11 @@ def main():
         "luorudy_constants": lr_consts,
     }
     out.update(mahajan_tables(const_h, mj_cc))
+    fail = function_body(const_h, "void SetUpMahajan_failParameters")
+    out["mahajan_fail_constants_by_type"] = [run_constants(fail, t) for t in range(9)]
+    out["mahajan_fail_constants_default"] = run_constants(fail, 99)
+    normal = function_body(const_h, "void SetUpMahajanParameters")
+    out["mahajan_constants_by_type_evaluated"] = [run_constants(normal, t) for t in range(9)]
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     print(f"wrote {OUT}: {len(tt)} tT constants, {len(tt_init)} tT states, "

@@ -13,16 +13,16 @@ from ucla_cmg_amd import voomgpu as vg
 pytestmark = pytest.mark.gpu
 
 
-def _cells_vs_oracle(prec, quirks=0, steps=3000, dt=0.01):
-    types = np.arange(9)
+def _cells_vs_oracle(prec, quirks=0, steps=3000, dt=0.01, fail=False):
+    types = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 11])  # 11: beyond 8, the fallback set
     n = len(types)
-    cells = vg.IonicCells(vg.MODEL_MJ, n, prec)
+    cells = vg.IonicCells(vg.MODEL_MJ_FAIL if fail else vg.MODEL_MJ, n, prec)
     cells.set_tissue_types(types)
     if quirks:
         cells.set_quirks(quirks)
     so = np.array([oracle.mj_init(t) for t in types])
     cells.set_states(so)
-    consts = [oracle.mj_constants(t) for t in types]
+    consts = [(oracle.mj_fail_constants if fail else oracle.mj_constants)(t) for t in types]
     Vg = so[:, 0].copy()
     Vo = so[:, 0].copy()
     xi = oracle.mj_xi()
@@ -46,6 +46,19 @@ def test_mahajan_cells_fp64_match_oracle(gpu_device):
     assert to.max() > 20.0
     assert err < 1e-8
     np.testing.assert_allclose(sg[:, 1:], so[:, 1:], rtol=1e-9, atol=1e-12)
+
+
+def test_mahajan_fail_cells_fp64_match_oracle(gpu_device):
+    """The failing-heart constant set (MAHAJAN_FAIL), every tissue type and
+    the fallback type, through a whole (longer) action potential."""
+    tg, to, sg, so = _cells_vs_oracle(vg.PREC_FP64, steps=45000, fail=True)
+    err = np.abs(tg - to).max()
+    ref, _, _, _ = _cells_vs_oracle(vg.PREC_FP64, steps=45000)
+    print(f"Mahajan_fail cells fp64: max |dV| {err:.3e} mV; V at 300 ms fail "
+          f"{to[30000].round(1)} normal {ref[30000].round(1)}")
+    assert to.max() > 20.0 and err < 1e-7
+    assert (to[30000] > ref[30000] + 20.0).all()  # still repolarising where normal cells rest
+    np.testing.assert_allclose(sg[:, 1:], so[:, 1:], rtol=1e-8, atol=1e-11)
 
 
 def test_mahajan_cells_int_abs_quirk(gpu_device):
@@ -78,12 +91,13 @@ def _slab(nx=15, ny=9, nz=3):
     return mesh, types, meshgen.slab_corner_nodes(nx, ny, nz, 3)
 
 
-def _tissue_pair(prec, steps, cg_tol, dt=0.01):
+def _tissue_pair(prec, steps, cg_tol, dt=0.01, fail=False):
     mesh, types, stim = _slab()
-    gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_MJ, precision=prec, dt=dt, D=D_ISO,
-                           cg_tol=cg_tol)
+    gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_MJ_FAIL if fail else vg.MODEL_MJ,
+                           precision=prec, dt=dt, D=D_ISO, cg_tol=cg_tol)
     ref = oracle.Simulation(mesh, oracle.MODEL_MJ, dt=dt, D=D_ISO, cg_tol=1e-10,
-                            param_idx=types.astype(np.uint16))
+                            param_idx=types.astype(np.uint16),
+                            mj_const=oracle.mj_table(fail) if fail else None)
     np.testing.assert_allclose(gpu.getVoltage(), ref.V, rtol=0,
                                atol=0.0 if prec == vg.PREC_FP64 else 1e-4)
     gpu.enableActivation(-40.0)
@@ -111,6 +125,13 @@ def test_mahajan_slab_fp64_matches_oracle(gpu_device):
     both = at_o >= 0
     assert np.abs(at_g[both] - at_o[both]).max() < 1e-6
     np.testing.assert_allclose(st[:, 1:], ref.states[:, 1:], rtol=1e-6, atol=1e-9)
+
+
+def test_mahajan_fail_slab_fp64_matches_oracle(gpu_device):
+    gpu, ref, mesh = _tissue_pair(vg.PREC_FP64, 600, 1e-10, fail=True)
+    err = np.abs(gpu.getVoltage() - ref.V).max()
+    print(f"Mahajan_fail slab fp64: max|dV| {err:.3e} mV, activated {(ref.act >= 0).mean():.2f}")
+    assert (ref.act >= 0).mean() > 0.5 and err < 1e-6
 
 
 def test_mahajan_slab_mixed_within_tolerance(gpu_device):

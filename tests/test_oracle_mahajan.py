@@ -36,6 +36,43 @@ def test_constants_and_initial_states_match_reference_tables():
     assert oracle.mj_xi() == pytest.approx(GOLD["mahajan_xi"], rel=1e-15)
 
 
+def test_every_constant_of_both_sets_matches_the_evaluated_reference_functions():
+    """All 35 constants of SetUpMahajanParameters and
+    SetUpMahajan_failParameters per tissue type, evaluated from the
+    reference's assignments (products, *=, the switch on pos)."""
+    for t in range(9):
+        np.testing.assert_array_equal(oracle.mj_constants(t),
+                                      GOLD["mahajan_constants_by_type_evaluated"][t])
+        np.testing.assert_array_equal(oracle.mj_fail_constants(t),
+                                      GOLD["mahajan_fail_constants_by_type"][t])
+    np.testing.assert_array_equal(oracle.mj_fail_constants(12),
+                                  GOLD["mahajan_fail_constants_default"])
+    tab = oracle.mj_table(fail=True)
+    assert tab.shape == (10, 35) and tab[9][10] == GOLD["mahajan_fail_constants_default"][10]
+
+
+def test_failing_set_changes_the_action_potential():
+    """The failing-heart constants (weaker I_K1 and I_Ks, stronger exchange)
+    lengthen the action potential of the same cell."""
+    def apd(c):
+        s = oracle.mj_init(2)
+        V, t, up, down = s[0], 0.0, None, None
+        for k in range(60000):
+            stim = 50.0 * oracle.mj_xi() if t < 2.0 else 0.0
+            dv = oracle.mj_compute_ion(c, s, 1.0, 0.01, V, stim)
+            V += 0.01 * dv
+            t += 0.01
+            if up is None and V > -40.0:
+                up = t
+            if up is not None and down is None and t > up + 5.0 and V < -70.0:
+                down = t
+                break
+        return down - up
+    a, b = apd(oracle.mj_constants(2)), apd(oracle.mj_fail_constants(2))
+    print(f"APD to -70 mV: normal {a:.1f} ms, failing {b:.1f} ms")
+    assert b > a + 10.0
+
+
 @pytest.mark.parametrize("t", range(9))
 def test_reference_initial_states_are_diastolic_rest(t):
     """The reference's per-type initial states are a quiescent diastolic

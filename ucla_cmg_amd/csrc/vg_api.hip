@@ -71,6 +71,22 @@ int mj_type(const int8_t *tissue, int64_t i) {
   const int t = tissue ? tissue[i] : 0;
   return (t >= 0 && t < 9) ? t : 2;
 }
+// Row of the per-type constants (G_tof, G_Ks): the type, or row 9 for types
+// beyond 8 -- the default branch of the constant setup, which for the
+// normal set equals type 2 and for the failing-heart set does not.
+int mj_row(const int8_t *tissue, int64_t i) {
+  const int t = tissue ? tissue[i] : 0;
+  return (t >= 0 && t < 9) ? t : 9;
+}
+// Per-type scaling rows (G_tof, G_Ks in slots 0, 1) of either constant set.
+std::vector<double> mj_scale_rows(bool fail) {
+  std::vector<double> sc((size_t)10 * VG_TT_NSCALE, 0.0);
+  for (int t = 0; t < 10; t++) {
+    sc[(size_t)t * VG_TT_NSCALE + 0] = fail ? MJF_GTOF[t] : MJ_GTOF[t < 9 ? t : 2];
+    sc[(size_t)t * VG_TT_NSCALE + 1] = fail ? MJF_GKS[t] : MJ_GKS[t < 9 ? t : 2];
+  }
+  return sc;
+}
 int ndev_of(int model) { return nstate_of(model) - (vslot_of(model) >= 0 ? 1 : 0); }
 const double *init_table(int model) {
   return model == VG_MODEL_LR      ? LR_INIT
@@ -155,6 +171,7 @@ struct vg_handle {
   int glaunch[2] = {0, 0};
   bool g_stim = false, g_at = false;
   unsigned long long g_notconv_seen = 0;
+  bool mj_fail = false;  // Mahajan with MAHAJAN_FAIL's constants
   // Step scheme; the explicit scheme's diffusion strides of a half-step.
   int scheme = VG_SCHEME_CN;
   double dt_diff = 0.1;
@@ -209,6 +226,7 @@ struct vg_handle {
 
 struct vg_cells {
   int prec, model, integ;
+  bool mj_fail = false;
   int newton_jt = 0, mj_int_abs = 0;
   int64_t n, n_alloc;
   DevBuf states, acc, scales, pidx, V, stim, dvdt;
@@ -426,6 +444,8 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   }
   if (h->n_myo > 0) {
     if (h->model == VG_MODEL_LR) launch_ionic_t<T, VG_MODEL_LR, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
+    else if (h->model == VG_MODEL_MJ && h->mj_fail)
+      launch_ionic_t<T, VG_MODEL_MJ_FAIL, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else if (h->model == VG_MODEL_MJ)
       launch_ionic_t<T, VG_MODEL_MJ, VG_INTEG_REFERENCE>(h, Vin, x, Vout);
     else if (h->integ == VG_INTEG_REFERENCE)
@@ -911,11 +931,12 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   std::unique_ptr<vg_handle> h(new vg_handle());
   if (!mesh || !model) return VG_ERR_ARG;
   if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_TT &&
-      model->cell_model != VG_MODEL_MJ) {
+      model->cell_model != VG_MODEL_MJ && model->cell_model != VG_MODEL_MJ_FAIL) {
     *out = h.release();
     return (*out)->fail(VG_ERR_UNSUPPORTED, "cell model not supported");
   }
-  if (model->cell_model == VG_MODEL_MJ && model->n_param_sets > 0) {
+  const bool mj_fail = model->cell_model == VG_MODEL_MJ_FAIL;
+  if ((model->cell_model == VG_MODEL_MJ || mj_fail) && model->n_param_sets > 0) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "Mahajan takes its constants from the tissue types, not "
                                     "from parameter sets");
@@ -934,7 +955,9 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     return (*out)->fail(VG_ERR_ARG, "dt, dt_ionic, Cm, cg_tol and cg_maxit must be positive");
   }
   h->prec = model->precision;
-  h->model = model->cell_model;
+  // The failing-heart set is the Mahajan model with other constants.
+  h->model = mj_fail ? VG_MODEL_MJ : model->cell_model;
+  h->mj_fail = mj_fail;
   h->integ = model->integrator;
   h->quirks = model->quirks;
   h->dt = model->dt;
@@ -1156,12 +1179,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi;
   // Mahajan: one set per tissue type holding G_tof and G_Ks).
   if (H_->model == VG_MODEL_MJ) {
-    std::vector<double> sc((size_t)9 * VG_TT_NSCALE, 0.0);
-    for (int t = 0; t < 9; t++) {
-      sc[(size_t)t * VG_TT_NSCALE + 0] = MJ_GTOF[t];
-      sc[(size_t)t * VG_TT_NSCALE + 1] = MJ_GKS[t];
-    }
-    H_->nsets = 9;
+    const std::vector<double> sc = mj_scale_rows(H_->mj_fail);
+    H_->nsets = 10;
     if (T == 8) {
       if ((rc = upload_raw<double>(H_, H_->scales, sc))) return rc;
     } else {
@@ -1169,7 +1188,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     }
     std::vector<uint16_t> pi(na, 0);
     for (int64_t r = 0; r < H_->rows; r++)
-      pi[r] = H_->new2old[r] >= 0 ? (uint16_t)mj_type(mesh->tissue, H_->new2old[r]) : 0;
+      pi[r] = H_->new2old[r] >= 0 ? (uint16_t)mj_row(mesh->tissue, H_->new2old[r]) : 0;
     HIPCHK(H_, H_->pidx.alloc(na * 2));
     HIPCHK(H_, hipMemcpy(H_->pidx.p, pi.data(), na * 2, hipMemcpyHostToDevice));
   } else {
@@ -1604,12 +1623,15 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
 int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   if (!out || n < 1) return VG_ERR_ARG;
   if (model != VG_MODEL_LR && model != VG_MODEL_TT && model != VG_MODEL_PURKINJE &&
-      model != VG_MODEL_MJ)
+      model != VG_MODEL_MJ && model != VG_MODEL_MJ_FAIL)
     return VG_ERR_UNSUPPORTED;
+  const bool mj_fail = model == VG_MODEL_MJ_FAIL;
+  if (mj_fail) model = VG_MODEL_MJ;
   if (prec != VG_PREC_FP64 && prec != VG_PREC_MIXED) return VG_ERR_ARG;
   std::unique_ptr<vg_cells> c(new vg_cells());
   c->prec = prec;
   c->model = model;
+  c->mj_fail = mj_fail;
   c->integ = integ;
   c->n = n;
   c->n_alloc = ((n + BLK - 1) / BLK) * BLK;
@@ -1625,12 +1647,8 @@ int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
       c->dvdt.alloc(n * 8) != hipSuccess)
     return VG_ERR_HIP;
   // Mahajan: one scaling set per tissue type carrying G_tof, G_Ks.
-  std::vector<double> one(model == VG_MODEL_MJ ? 9 * VG_TT_NSCALE : VG_TT_NSCALE, 1.0);
-  if (model == VG_MODEL_MJ)
-    for (int t = 0; t < 9; t++) {
-      one[(size_t)t * VG_TT_NSCALE + 0] = MJ_GTOF[t];
-      one[(size_t)t * VG_TT_NSCALE + 1] = MJ_GKS[t];
-    }
+  std::vector<double> one =
+      model == VG_MODEL_MJ ? mj_scale_rows(mj_fail) : std::vector<double>(VG_TT_NSCALE, 1.0);
   if (T == 8) {
     if (c->scales.alloc(one.size() * 8) != hipSuccess) return VG_ERR_HIP;
     if (hipMemcpy(c->scales.p, one.data(), one.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
@@ -1661,7 +1679,7 @@ int vg_cells_set_scales(vg_cells *c, int nsets, const double *sc, const uint16_t
   if (c->model == VG_MODEL_MJ) {  // pidx = tissue types; no scaling sets
     if (nsets != 0 || sc || !pidx) return VG_ERR_ARG;
     std::vector<uint16_t> t(pidx, pidx + c->n);
-    for (auto &v : t) v = (uint16_t)(v < 9 ? v : 2);
+    for (auto &v : t) v = (uint16_t)(v < 9 ? v : 9);
     if (!c->pidx.p && c->pidx.alloc(c->n * 2) != hipSuccess) return VG_ERR_HIP;
     if (hipMemcpy(c->pidx.p, t.data(), c->n * 2, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
     return VG_OK;
@@ -1708,12 +1726,14 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, con
   if (c->prec == VG_PREC_FP64) {
     if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(double, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_MJ && c->mj_fail) VG_CELLS(double, VG_MODEL_MJ_FAIL, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_MJ) VG_CELLS(double, VG_MODEL_MJ, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(double, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(double, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);
   } else {
     if (c->model == VG_MODEL_LR) VG_CELLS(float, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(float, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);
+    else if (c->model == VG_MODEL_MJ && c->mj_fail) VG_CELLS(float, VG_MODEL_MJ_FAIL, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_MJ) VG_CELLS(float, VG_MODEL_MJ, VG_INTEG_REFERENCE);
     else if (c->integ == VG_INTEG_REFERENCE) VG_CELLS(float, VG_MODEL_TT, VG_INTEG_REFERENCE);
     else VG_CELLS(float, VG_MODEL_TT, VG_INTEG_RUSH_LARSEN);

@@ -591,6 +591,28 @@ constexpr double MJ_GTOF[9] = {0.11, 0.11, 0.11, 0.11, 0.11, 0.11, 0.11 * 0.85, 
                                0.11 * 0.85};
 constexpr double MJ_GKS[9] = {1.9 * 0.32,  1.4 * 0.32,  1.0 * 0.32, 0.74 * 0.32, 0.52 * 0.32,
                               0.35 * 0.32, 0.98 * 0.32, 0.7 * 0.32, 0.5 * 0.32};
+// MAHAJAN_FAIL's per-type G_tof, G_Ks (SetUpMahajan_failParameters,
+// SetUpIonicConstants.h:254-315: the case value times 0.67 and 0.6); row 9 is
+// the set of types beyond 8 (its default branch), which differs from type 2.
+constexpr double MJF_GTOF[10] = {0.12 * 0.67, 0.12 * 0.67, 0.12 * 0.67, 0.12 * 0.67,
+                                 0.12 * 0.67, 0.12 * 0.67, 0.12 * 0.77 * 0.67, 0.12 * 0.77 * 0.67,
+                                 0.12 * 0.77 * 0.67, 0.12 * 0.67};
+constexpr double MJF_GKS[10] = {1.5 * 0.1386 * 0.6, 1.3 * 0.1386 * 0.6, 1.1 * 0.1386 * 0.6,
+                                1.0 * 0.1386 * 0.6, 0.9 * 0.1386 * 0.6, 0.8 * 0.1386 * 0.6,
+                                1.1 * 0.1386 * 0.6, 1.0 * 0.1386 * 0.6, 0.9 * 0.1386 * 0.6,
+                                0.1386 * 0.6};
+// The other constants that differ between the two sets.
+template <bool FAIL>
+struct MjK {
+  static constexpr double gtos = 0.04, gk1 = 0.3, gnaca = 0.84, vup = 0.4, taus = 4.0,
+                          taua = 100.0, av = 11.3, cstar = 90.0, sh = 50.0, leak = 0.0;
+};
+template <>
+struct MjK<true> {
+  static constexpr double gtos = 0.04 * 0.65, gk1 = 0.3 * 0.5, gnaca = 0.84 * 2.0, vup = 0.27,
+                          taus = 6.0, taua = 200.0, av = 22.5, cstar = 50.0, sh = 10.0,
+                          leak = 0.01;
+};
 // Initial states by tissue type (Mahajan.cc:27-158).
 constexpr double MJ_INIT[9][MJ_NSTATE] = {
     {-81.4359, 0.002789, 0.96679, 0.97859, 0.014994, 0.044487, 0.091992, 2.1933, 0.23277,
@@ -637,10 +659,11 @@ __device__ inline T mj_buffer(T c) {
 
 // Mahajan::ucla_rhsfun with the comp_* currents (Mahajan.cc:168-603): every
 // rate at gate step h, stimulus istim in uA/uF, tissue-type G_tof and G_Ks.
-template <class T, bool FAST>
+template <class T, bool FAST, bool FAIL = false>
 __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gks,
                                 T R[MJ_NSTATE]) {
   using M = Mth<T, FAST>;
+  using K = MjK<FAIL>;
   const T V = S[0];
   const T frt = T(96.485 / (8.314 * 308.0));
   const T nao = T(136.0), ki = T(140.0), ko = T(5.4), cao = T(1.8);
@@ -653,7 +676,7 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
   const T b_k1 = (T(0.49124) * M::ex(T(0.08032) * (V - ek1 + T(5.476))) +
                   M::ex(T(0.061750) * (V - ek1 - T(594.31)))) /
                  (T(1) + M::ex(T(-0.5143) * (V - ek1 + T(4.753))));
-  const T i_k1 = T(0.3) * sq_ko * a_k1 / (a_k1 + b_k1) * (V - ek1);
+  const T i_k1 = T(K::gk1) * sq_ko * a_k1 / (a_k1 + b_k1) * (V - ek1);
 
   // I_to slow and fast
   const T ek = M::lg(ko / ki) / frt;
@@ -662,7 +685,7 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
   const T y_inf = T(1) / (T(1) + M::ex((V + T(33.5)) / T(10.0)));
   const T tau_xs = T(9.0) / (T(1) + e3) + T(0.5);
   const T tau_ys = T(3000.0) / (T(1) + M::ex((V + T(60.0)) / T(10.0))) + T(30.0);
-  const T i_tos = T(0.04) * S[22] * (S[23] + T(0.5) * y_inf) * (V - ek);
+  const T i_tos = T(K::gtos) * S[22] * (S[23] + T(0.5) * y_inf) * (V - ek);
   R[22] = mj_relax<T, FAST>(x_inf, tau_xs, S[22], h);
   R[23] = mj_relax<T, FAST>(y_inf, tau_ys, S[23], h);
   const T v30 = V / T(30.0);
@@ -690,13 +713,13 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
   const T zw8 = T(1.3) * nai3 + T(87.5 * 87.5 * 87.5) * csm +
                 T(12.3 * 12.3 * 12.3) * cao * (T(1) + csm / T(0.0036)) +
                 T(0.0036) * nao3 * (T(1) + rn * rn * rn) + nai3 * cao + nao3 * csm;
-  const T j_naca = T(0.84) * aloss * zw3 / (zw4 * zw8);
+  const T j_naca = T(K::gnaca) * aloss * zw3 / (zw4 * zw8);
 
   // Ca cycling
-  const T j_d = (cs - ci) / T(4.0);
+  const T j_d = (cs - ci) / T(K::taus);
   R[24] = T(0.0327) * ci * (T(70.0) - S[24]) - T(0.0196) * S[24];
   R[25] = T(0.0327) * cs * (T(70.0) - S[25]) - T(0.0196) * S[25];
-  const T j_up = T(0.4) * ci * ci / (ci * ci + T(0.25));
+  const T j_up = T(K::vup) * ci * ci / (ci * ci + T(0.25));
   const T j_leak = T(2.069e-5) * cj * cj / (cj * cj + T(2500.0)) * (T(16.667) * cj - ci);
 
   // L-type Ca channel Markov chain
@@ -744,15 +767,15 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
   R[8] = mj_buffer(cs) * (T(50.0) * (S[12] - j_d - j_ca + j_naca) - R[25]);
   R[9] = mj_buffer(ci) * (j_d - j_up + j_leak - R[24]);
   R[10] = -S[12] + j_up - j_leak;
-  R[11] = (cj - S[11]) / T(100.0);
+  R[11] = (cj - S[11]) / T(K::taua);
 
   // SR release: load dependence, sparks, dyad
   T q = T(0);
   const T cjp = S[11];
-  if (cjp > T(50.0) && cjp < T(90.0)) q = cjp - T(50.0);
-  else if (cjp >= T(90.0)) q = T(11.3) * cjp + T((90.0 - 50.0) - 11.3 * 90.0);
+  if (cjp > T(K::sh) && cjp < T(K::cstar)) q = cjp - T(K::sh);
+  else if (cjp >= T(K::cstar)) q = T(K::av) * cjp + T((K::cstar - K::sh) - K::av * K::cstar);
   if (q >= T(153.0)) q = T(153.0);
-  const T Qr = cj * q / T(90.0);
+  const T Qr = cj * q / T(K::cstar);
   const T arx = fabs(rxa);
   const T ey = M::ex(T(-0.05) * (V + T(30.0)));
   const T spark = T(2.58079) * po * arx * ey / (T(1) + ey);
@@ -777,7 +800,15 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
                a * (T(0.1212) * M::ex(T(-0.01052) * V) / (T(1) + M::ex(T(-0.1378) * (V + T(40.14)))));
   const T tau_h = T(1) / (ah + bh), tau_j = T(1) / (aj + bj), tau_m = T(1) / (am + bm);
   const T m = S[1];
-  const T i_na = T(12.0) * S[2] * S[3] * m * m * m * (V - ena);
+  T i_na;
+  if constexpr (FAIL) {
+    // Leaky gates, (x + l) / (1 + l), in the reference's order (Mahajan.cc:269).
+    const T l = T(K::leak), ol = T(1) + l;
+    i_na = T(12.0) * (S[2] + l) / ol * (S[3] + l) / ol * (m + l) / ol * (m + l) / ol * (m + l) / ol *
+           (V - ena);
+  } else {
+    i_na = T(12.0) * S[2] * S[3] * m * m * m * (V - ena);
+  }
   R[2] = mj_relax<T, FAST>(ah * tau_h, tau_h, S[2], h);
   R[3] = mj_relax<T, FAST>(aj * tau_j, tau_j, S[3], h);
   R[1] = mj_relax<T, FAST>(am * tau_m, tau_m, S[1], h);
@@ -818,17 +849,17 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
 // the MJ_ACC states, which the updates accumulate.  Forward Euler over dt, or
 // 10 sub-steps of dt/10 when |dV/dt| > 20 (int_abs: when |trunc(dV/dt)| > 20);
 // returns the last dV/dt evaluated.
-template <class T, bool FAST>
+template <class T, bool FAST, bool FAIL = false>
 __device__ inline T mj_step(T S[MJ_NSTATE], double acc[MJ_NACC], T V, T istim, T dt, T gtof,
                             T gks, bool int_abs) {
   T R[MJ_NSTATE];
   S[0] = V;
-  mj_rates<T, FAST>(S, dt, istim, gtof, gks, R);
+  mj_rates<T, FAST, FAIL>(S, dt, istim, gtof, gks, R);
   const bool adapt = int_abs ? fabs(trunc(R[0])) > T(20) : fabs(R[0]) > T(20);
   const int nsub = adapt ? 10 : 1;
   const T h = adapt ? dt / T(10) : dt;
   for (int q = 0; q < nsub; q++) {
-    if (adapt) mj_rates<T, FAST>(S, h, istim, gtof, gks, R);
+    if (adapt) mj_rates<T, FAST, FAIL>(S, h, istim, gtof, gks, R);
     S[0] += R[0] * h;
 #pragma unroll
     for (int k = 0; k < MJ_NGATE; k++) S[MJ_GATE[k]] += R[MJ_GATE[k]] * h;

@@ -546,7 +546,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       });
 #pragma unroll
       for (int k = 0; k < LR_NSTATE - 1; k++) st[k * a.n_alloc + i] = s[k];
-    } else if constexpr (MODEL == VG_MODEL_MJ) {
+    } else if constexpr (MODEL == VG_MODEL_MJ || MODEL == VG_MODEL_MJ_FAIL) {
       // Gates and channel states in T (rows of st), Ca-cycling and Na_i in
       // FP64 (rows of acc); sc[0], sc[1] = G_tof, G_Ks of the node's type.
       T s[MJ_NSTATE];
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       const T it = stim / (Cm * T(MJ_XI));
       ionic_substeps(sub, [&](double dtd) {
         const T dt = (T)dtd;
-        const T dv = mj_step<T, sizeof(T) == 4>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
+        const T dv = mj_step<T, sizeof(T) == 4, MODEL == VG_MODEL_MJ_FAIL>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
         V += dt * dv;
         return (double)dv;
       });
@@ -979,7 +979,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
     dvdt[i] = (double)lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3]);
 #pragma unroll
     for (int k = 0; k < LR_NSTATE - 1; k++) st[k * n_alloc + i] = s[k];
-  } else if constexpr (MODEL == VG_MODEL_MJ) {
+  } else if constexpr (MODEL == VG_MODEL_MJ || MODEL == VG_MODEL_MJ_FAIL) {
     T s[MJ_NSTATE];
     double ac[MJ_NACC];
 #pragma unroll
@@ -989,7 +989,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
       ac[k] = acc[k * n_alloc + i];
       s[MJ_ACC[k]] = (T)ac[k];
     }
-    dvdt[i] = (double)mj_step<T, sizeof(T) == 4>(s, ac, V, stim / (Cm * T(MJ_XI)), dt, sc[0],
+    dvdt[i] = (double)mj_step<T, sizeof(T) == 4, MODEL == VG_MODEL_MJ_FAIL>(s, ac, V, stim / (Cm * T(MJ_XI)), dt, sc[0],
                                                  sc[1], mj_int_abs != 0);
 #pragma unroll
     for (int k = 0; k < MJ_NGATE; k++) st[k * n_alloc + i] = s[MJ_GATE[k]];

@@ -7,7 +7,7 @@
 //
 // Deck keywords (case-insensitive, one per line; a line containing #, $ or *
 // anywhere is a comment, EPSimulator.cc:142-146): MODEL, OUTPUTDIR, CELLMODEL
-// (LUORUDY, TUSSCHER, MAHAJAN), MESHTYPE (MIX3D, TET), QUADORDER, DT, IONICDT,
+// (LUORUDY, TUSSCHER, MAHAJAN, MAHAJAN_FAIL), MESHTYPE (MIX3D, TET), QUADORDER, DT, IONICDT,
 // SIMULATIONTIME, FREQUENCY, TWODRADIUS, TISSUEDIFFUSION, PURKINJEDIFFUSION,
 // LVPURKINJEDIFFUSION, SMOOTHING [alpha], INITIALSTATES, READRESTART t,
 // WRITERESTART t, STIMULUS NODAL|CIRC|ZONAL|PLANE ... (EPSimulator.cc:147-266).
@@ -183,6 +183,7 @@ int main(int argc, char **argv) {
   if (cell_model == "LUORUDY") cm = VG_MODEL_LR;
   else if (cell_model == "TUSSCHER") cm = VG_MODEL_TT;
   else if (cell_model == "MAHAJAN") cm = VG_MODEL_MJ;
+  else if (cell_model == "MAHAJAN_FAIL") cm = VG_MODEL_MJ_FAIL;
   else die("CELLMODEL " + cell_model + " is not supported");
 
   char err[1024] = {0};
@@ -262,7 +263,8 @@ int main(int argc, char **argv) {
       if ((t == 11 || t == 12) && !pk.empty()) {
         if ((int)pk.size() != nv[i]) die("PURKINJE block has the wrong number of states");
         src = pk.data();
-      } else if (cm == VG_MODEL_MJ && t != 11 && t != 12 && !cell.empty()) {
+      } else if ((cm == VG_MODEL_MJ || cm == VG_MODEL_MJ_FAIL) && t != 11 && t != 12 &&
+                 !cell.empty()) {
         if (cell.size() == (size_t)nv[i]) src = cell.data();
         else if (cell.size() == (size_t)9 * nv[i] && t >= 0 && t < 9) src = cell.data() + (size_t)t * nv[i];
         else die("CELL block has the wrong number of states");

@@ -26,6 +26,7 @@ VG_OK = 0
 MODEL_LR = 0
 MODEL_TT = 2
 MODEL_MJ = 3
+MODEL_MJ_FAIL = 4  # Mahajan with the failing-heart constants (MAHAJAN_FAIL)
 MODEL_PURKINJE = 10
 PREC_FP64 = 0
 PREC_MIXED = 1
