@@ -48,6 +48,10 @@ extern "C" {
 /* Cell models, numbered like CardiacBody::New's CellModelType switch
  * (CardiacBody.cc:988-1093). */
 #define VG_MODEL_LR 0
+/* Luo-Rudy with the 14 gate and background values interpolated from
+ * LuoRudyGateTable's 0.001 mV table over (-90, 200) mV (CellModelType
+ * LUORUDYWITHGATE; LRGateTable.h, LR.cc:71-79). */
+#define VG_MODEL_LR_GATE 1
 #define VG_MODEL_TT 2
 /* Mahajan 2008 rabbit ventricle: each node's constants and initial state by
  * its tissue type 0-8 (vg_mesh.tissue; epi/myo/endo x apex/center/base). */

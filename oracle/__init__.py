@@ -87,7 +87,7 @@ class _Sim(C.Structure):
         ("nthreads", C.c_int), ("tt_newton_jt", C.c_int),
         ("pk_flag", C.c_void_p), ("pk_states", C.c_void_p),
         ("mj_const", C.c_void_p), ("mj_int_abs", C.c_int),
-        ("scheme", C.c_int), ("dt_diff", C.c_double),
+        ("scheme", C.c_int), ("lr_gate", C.c_int), ("dt_diff", C.c_double),
     ]
 
 
@@ -106,6 +106,9 @@ def _declare(L):
     L.or_lr_xi.restype = C.c_double
     L.or_lr_compute_ion.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double]
     L.or_lr_compute_ion.restype = C.c_double
+    L.or_lr_gate_compute_ion.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.or_lr_gate_compute_ion.restype = C.c_double
+    L.or_lr_tab.argtypes = [C.c_double, C.c_int, _dp]
     L.or_tt_constants.argtypes = [_dp]
     L.or_tt_init.argtypes = [_dp]
     L.or_tt_rates.argtypes = [_dp, _dp, _dp, _dp]
@@ -238,6 +241,17 @@ def lr_compute_ion(state, Cm, dt, V, istim):
     return lib().or_lr_compute_ion(state, Cm, dt, V, istim)
 
 
+def lr_tab(V, gate=False):
+    t = np.zeros(14)
+    lib().or_lr_tab(V, int(bool(gate)), t)
+    return t
+
+
+def lr_gate_compute_ion(state, Cm, dt, V, istim):
+    """LUORUDYWITHGATE's Compute_Ion (the LRGateTable interpolation)."""
+    return lib().or_lr_gate_compute_ion(state, Cm, dt, V, istim)
+
+
 def tt_constants():
     c = np.zeros(TT_NCONST)
     lib().or_tt_constants(c)
@@ -307,7 +321,8 @@ class Simulation:
     def __init__(self, mesh, model, dt, dt_ion=None, Cm=1.0, cg_tol=1e-10, cg_maxit=1000,
                  D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, DpLeft=None, V0=None, states=None,
                  tt_const=None, param_idx=None, nthreads=0, pmj_sticky=False, csr=None,
-                 newton_jt=False, mj_const=None, mj_int_abs=False, scheme=0, dt_diff=0.1):
+                 newton_jt=False, mj_const=None, mj_int_abs=False, scheme=0, dt_diff=0.1,
+                 lr_gate=False):
         """Mahajan (model 3) takes each node's tissue type 0-8 from
         ``param_idx`` (default 0, Mesh.cc:186,375; types beyond 8 run the
         reference's fallback, type 2) and its initial state from that type's
@@ -363,6 +378,7 @@ class Simulation:
         s.mj_int_abs = int(bool(mj_int_abs))
         s.scheme = scheme
         s.dt_diff = dt_diff
+        s.lr_gate = int(bool(lr_gate))
         tissue = getattr(mesh, "tissue", None)
         self.pk_flag = None
         self.pk_states = None

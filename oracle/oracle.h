@@ -97,6 +97,12 @@ enum { OR_LR = 0, OR_TT = 2, OR_MJ = 3 };
 void or_lr_init(double *s);
 double or_lr_xi(void);
 double or_lr_compute_ion(double *s, double Cm, double dt, double V, double istim);
+/* LUORUDYWITHGATE: the same model with the 14 gate/background values taken
+ * from LuoRudyGateTable's 0.001 mV table (LRGateTable.h, LR.cc:71-79). */
+double or_lr_gate_compute_ion(double *in, double Cm, double dt, double volt, double istim);
+/* The 14 make_parameter1 values at v: m,h,j,d,f,X (inf, alpha+beta), x1 and
+ * I_K1 + I_Kp + I_b; gate = 1 through the table (tests). */
+void or_lr_tab(double v, int gate, double tab[14]);
 
 void or_tt_constants(double *c);
 void or_tt_init(double *s);
@@ -154,6 +160,7 @@ typedef struct {
                              types beyond 8), or NULL (the normal set) */
   int mj_int_abs;       /* 1 = adaptive threshold read through int abs() */
   int scheme;           /* OR_SCHEME_*: the step of EPSolver::New's Method */
+  int lr_gate;          /* LR: 1 = LUORUDYWITHGATE (table of LRGateTable.h) */
   double dt_diff;       /* explicit diffusion sub-step (Solve's dtDiff, 0.1 ms) */
 } or_sim;
 

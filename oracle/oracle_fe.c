@@ -409,8 +409,9 @@ int or_cn_step(or_sim *s) {
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < n; i++) {
         if (pkf && pkf[i]) continue;
-        const double dv = or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm,
-                                            dt, s->V[i], s->istim[i]);
+        double *st = s->states + OR_LR_NSTATE * i;
+        const double dv = s->lr_gate ? or_lr_gate_compute_ion(st, s->Cm, dt, s->V[i], s->istim[i])
+                                     : or_lr_compute_ion(st, s->Cm, dt, s->V[i], s->istim[i]);
         s->V[i] += dt * dv;
       }
     } else if (s->model == OR_MJ) {
@@ -479,7 +480,10 @@ static double node_ion(or_sim *s, int64_t i, double dt, const double *tc, const 
   if (s->pk_flag && s->pk_flag[i])
     return or_pk_compute_ion(pkc, s->pk_states + OR_PK_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i]);
   if (s->model == OR_LR)
-    return or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm, dt, s->V[i], s->istim[i]);
+    return s->lr_gate ? or_lr_gate_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm, dt, s->V[i],
+                                               s->istim[i])
+                      : or_lr_compute_ion(s->states + OR_LR_NSTATE * i, s->Cm, dt, s->V[i],
+                                          s->istim[i]);
   if (s->model == OR_MJ) {
     const int t = s->param_idx ? s->param_idx[i] : 0;
     return or_mj_compute_ion(mt + (int64_t)OR_MJ_NCONST * (t < 9 ? t : 9),

@@ -105,12 +105,26 @@ static void lr_table(const lr_const *C, double v, double tab[14]) {
 }
 
 /* LuoRudy::Compute_Ion (LR.cc:65-100) without the gate table. */
-double or_lr_compute_ion(double *in, double Cm, double dt, double volt,
-                         double istim) {
+/* LuoRudyGateTable::get_parameters (LRGateTable.h:93-110, the branch without
+ * _USE_GATE_SLOPE): the 14 values linearly interpolated between the
+ * make_parameter1 values of the 0.001 mV grid -90 + 0.001 i.  The grid values
+ * are computed where needed (they are what makeGateTable stores). */
+static void lr_gate_table(const lr_const *C, double v, double tab[14]) {
+  const int index = (int)((v + 90) * 1000);
+  double lo[14], hi[14];
+  lr_table(C, -90 + 0.001 * index, lo);
+  lr_table(C, -90 + 0.001 * (index + 1), hi);
+  const double delVbydT = 1000 * (v - (-90 + 0.001 * index));
+  for (int i = 0; i < 14; i++) tab[i] = (hi[i] - lo[i]) * delVbydT + lo[i];
+}
+
+static double lr_compute(double *in, double Cm, double dt, double volt, double istim, int gate) {
   const lr_const C = lr_constants();
   double tabz[14];
   in[7] = volt;
-  lr_table(&C, in[7], tabz);
+  /* LR.cc:71-79: the table inside (-90, 200) mV, the formulas outside. */
+  if (gate && volt > -90 && volt < 200) lr_gate_table(&C, in[7], tabz);
+  else lr_table(&C, in[7], tabz);
   for (int i = 0; i < 6; i++)
     in[i] = tabz[2 * i] - (tabz[2 * i] - in[i]) * exp(-dt * tabz[2 * i + 1]);
   const double Es = 7.7 - 13.0287 * log(in[6]);
@@ -120,6 +134,20 @@ double or_lr_compute_ion(double *in, double Cm, double dt, double volt,
   in[6] = in[6] + (-.0001 * i_si + 0.07 * (0.0001 - in[6])) * dt;
   const double i_tot = -(tabz[13] + i_k + i_na + i_si);
   return (i_tot + istim / (or_lr_xi() * Cm));
+}
+
+double or_lr_compute_ion(double *in, double Cm, double dt, double volt, double istim) {
+  return lr_compute(in, Cm, dt, volt, istim, 0);
+}
+
+void or_lr_tab(double v, int gate, double tab[14]) {
+  const lr_const C = lr_constants();
+  if (gate && v > -90 && v < 200) lr_gate_table(&C, v, tab);
+  else lr_table(&C, v, tab);
+}
+
+double or_lr_gate_compute_ion(double *in, double Cm, double dt, double volt, double istim) {
+  return lr_compute(in, Cm, dt, volt, istim, 1);
 }
 
 /* =================== ten Tusscher 2006 (IonicModel/Tusscher.cc) ============ */

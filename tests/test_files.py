@@ -137,7 +137,7 @@ def test_ep_driver_rejects_bad_decks_before_touching_a_gpu(tmp_path):
     deck.write_text("# no model\nCellModel Tusscher\n")
     r = subprocess.run([EP_TOOL, str(deck)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "no MODEL" in r.stderr
-    deck.write_text("Model m\nCellModel LuoRudyWithGate\n")
+    deck.write_text("Model m\nCellModel FitzHughNagumo\n")
     r = subprocess.run([EP_TOOL, str(deck)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "not supported" in r.stderr
     deck.write_text(f"Model {tmp_path}/nothing\nCellModel Tusscher\n")

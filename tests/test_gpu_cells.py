@@ -49,6 +49,27 @@ def test_lr_fp64_matches_oracle(gpu_device):
     np.testing.assert_array_equal(tg[:, 0], tg[:, -1])  # lanes agree
 
 
+def test_lr_gate_table_fp64_matches_oracle(gpu_device):
+    """LUORUDYWITHGATE: the device's LuoRudyGateTable (built on the host,
+    interpolated on the GPU) against the oracle's on-the-fly grid values,
+    through a paced action potential."""
+    n, dt = 4, 0.01
+    cells = vg.IonicCells(vg.MODEL_LR_GATE, n, vg.PREC_FP64)
+    st = oracle.lr_init()
+    V = -84.0
+    for _ in range(50000):
+        V += dt * oracle.lr_gate_compute_ion(st, 1.0, dt, V, 0.0)
+    st[7] = V
+    cells.set_states(np.tile(st, (n, 1)))
+    so = st.copy()
+    tg, to = _pace(lambda V, s: cells.compute_ion(1.0, dt, V, s),
+                   lambda V, s: oracle.lr_gate_compute_ion(so, 1.0, dt, V, s),
+                   3000, dt, 200, 50.0 * oracle.lr_xi(), np.full(n, V))
+    err = np.abs(tg[:, 0] - to).max()
+    print(f"LR gate table fp64: peak {to.max():.2f} mV, max |dV| {err:.3e}")
+    assert to.max() > 20.0 and err < 1e-8
+
+
 def test_tt_reference_integrator_fp64_matches_oracle(gpu_device):
     n = 3
     dt = 0.02

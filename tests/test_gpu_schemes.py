@@ -52,6 +52,28 @@ def test_scheme_fp64_matches_oracle(gpu_device, scheme, model):
         assert st["cg_solves"] == 2 * steps and st["cg_not_converged"] == 0
 
 
+def test_lr_gate_table_slab_fp64_matches_oracle(gpu_device):
+    """LUORUDYWITHGATE in tissue (Crank-Nicolson) against the oracle."""
+    nx, ny, nz = 13, 11, 4
+    mesh = meshgen.slab(nx, ny, nz, 0.025)
+    stim = meshgen.slab_corner_nodes(nx, ny, nz, 3)
+    gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_LR_GATE, dt=0.01, D=D_SLAB,
+                           precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10)
+    ref = oracle.Simulation(mesh, oracle.MODEL_LR, dt=0.01, D=D_SLAB, cg_tol=1e-10,
+                            V0=gpu.getVoltage(), states=gpu.getStateVariables(), lr_gate=True)
+    gpu.setStimulusCurrent(stim, 50.0 * oracle.lr_xi())
+    ref.istim[stim] = 50.0 * oracle.lr_xi()
+    for k in range(600):
+        if k == 200:
+            gpu.clearStimulus()
+            ref.istim[:] = 0.0
+        gpu.Solve(1)
+        ref.step()
+    err = np.abs(gpu.getVoltage() - ref.V).max()
+    print(f"LR gate slab: max|dV| {err:.2e} mV")
+    assert ref.V.max() > 0.0 and err < 1e-6
+
+
 def test_explicit_mixed_precision_tracks_fp64(gpu_device):
     nx, ny, nz = 13, 11, 4
     mesh = meshgen.slab(nx, ny, nz, 0.025)

@@ -78,6 +78,14 @@ int mj_row(const int8_t *tissue, int64_t i) {
   const int t = tissue ? tissue[i] : 0;
   return (t >= 0 && t < 9) ? t : 9;
 }
+// LuoRudyGateTable (LRGateTable.h:67-76): make_parameter1 at -90 + 0.001 i,
+// 14 values per row, in FP64 on the host.
+std::vector<double> lr_gate_table() {
+  std::vector<double> t((size_t)LR_TABLE_ROWS * 14);
+  for (int i = 0; i < LR_TABLE_ROWS; i++) lr_tab<double, false>(-90 + 0.001 * i, &t[(size_t)i * 14]);
+  return t;
+}
+
 // Per-type scaling rows (G_tof, G_Ks in slots 0, 1) of either constant set.
 std::vector<double> mj_scale_rows(bool fail) {
   std::vector<double> sc((size_t)10 * VG_TT_NSCALE, 0.0);
@@ -172,6 +180,7 @@ struct vg_handle {
   bool g_stim = false, g_at = false;
   unsigned long long g_notconv_seen = 0;
   bool mj_fail = false;  // Mahajan with MAHAJAN_FAIL's constants
+  DevBuf lr_table;       // LUORUDYWITHGATE's table in T, or empty
   // Step scheme; the explicit scheme's diffusion strides of a half-step.
   int scheme = VG_SCHEME_CN;
   double dt_diff = 0.1;
@@ -227,6 +236,7 @@ struct vg_handle {
 struct vg_cells {
   int prec, model, integ;
   bool mj_fail = false;
+  DevBuf lr_table;  // LUORUDYWITHGATE's table, or empty
   int newton_jt = 0, mj_int_abs = 0;
   int64_t n, n_alloc;
   DevBuf states, acc, scales, pidx, V, stim, dvdt;
@@ -402,6 +412,7 @@ void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
   a.rec_stride = h->rec_stride;
   a.log_dt = (float)std::log(h->dt);
   a.only = nullptr;
+  a.lr_table = h->lr_table.p;
   if constexpr (MODEL == VG_MODEL_TT) if (h->surr && !h->rec_x) {
     // The network advances the rows inside its envelope and flags the rest,
     // which the plain kernel below then updates.
@@ -930,8 +941,9 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   *out = nullptr;
   std::unique_ptr<vg_handle> h(new vg_handle());
   if (!mesh || !model) return VG_ERR_ARG;
-  if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_TT &&
-      model->cell_model != VG_MODEL_MJ && model->cell_model != VG_MODEL_MJ_FAIL) {
+  if (model->cell_model != VG_MODEL_LR && model->cell_model != VG_MODEL_LR_GATE &&
+      model->cell_model != VG_MODEL_TT && model->cell_model != VG_MODEL_MJ &&
+      model->cell_model != VG_MODEL_MJ_FAIL) {
     *out = h.release();
     return (*out)->fail(VG_ERR_UNSUPPORTED, "cell model not supported");
   }
@@ -955,9 +967,13 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     return (*out)->fail(VG_ERR_ARG, "dt, dt_ionic, Cm, cg_tol and cg_maxit must be positive");
   }
   h->prec = model->precision;
-  // The failing-heart set is the Mahajan model with other constants.
-  h->model = mj_fail ? VG_MODEL_MJ : model->cell_model;
+  // The failing-heart set is the Mahajan model with other constants; the
+  // gate-table Luo-Rudy is Luo-Rudy with a table.
+  h->model = mj_fail ? VG_MODEL_MJ
+             : model->cell_model == VG_MODEL_LR_GATE ? VG_MODEL_LR
+                                                      : model->cell_model;
   h->mj_fail = mj_fail;
+  const bool lr_gate = model->cell_model == VG_MODEL_LR_GATE;
   h->integ = model->integrator;
   h->quirks = model->quirks;
   h->dt = model->dt;
@@ -1176,6 +1192,14 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, hipHostMalloc((void **)&H_->h_tot, sizeof(CgTotals), hipHostMallocDefault));
   H_->graph = model->graph ? 1 : 0;
   HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult) * H_->n_inst, hipHostMallocDefault));
+  if (lr_gate) {
+    const std::vector<double> tab = lr_gate_table();
+    if (T == 8) {
+      if ((rc = upload_raw<double>(H_, H_->lr_table, tab))) return rc;
+    } else {
+      if ((rc = upload_raw<float>(H_, H_->lr_table, tab))) return rc;
+    }
+  }
   // Parameter sets (tT scalings; LR uses gNa and gCaL as its Gna and Gsi;
   // Mahajan: one set per tissue type holding G_tof and G_Ks).
   if (H_->model == VG_MODEL_MJ) {
@@ -1622,16 +1646,28 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
 // ---- independent cells -------------------------------------------------------
 int vg_cells_create(int model, int prec, int integ, int64_t n, vg_cells **out) {
   if (!out || n < 1) return VG_ERR_ARG;
-  if (model != VG_MODEL_LR && model != VG_MODEL_TT && model != VG_MODEL_PURKINJE &&
-      model != VG_MODEL_MJ && model != VG_MODEL_MJ_FAIL)
+  if (model != VG_MODEL_LR && model != VG_MODEL_LR_GATE && model != VG_MODEL_TT &&
+      model != VG_MODEL_PURKINJE && model != VG_MODEL_MJ && model != VG_MODEL_MJ_FAIL)
     return VG_ERR_UNSUPPORTED;
-  const bool mj_fail = model == VG_MODEL_MJ_FAIL;
+  const bool mj_fail = model == VG_MODEL_MJ_FAIL, lr_gate = model == VG_MODEL_LR_GATE;
   if (mj_fail) model = VG_MODEL_MJ;
+  if (lr_gate) model = VG_MODEL_LR;
   if (prec != VG_PREC_FP64 && prec != VG_PREC_MIXED) return VG_ERR_ARG;
   std::unique_ptr<vg_cells> c(new vg_cells());
   c->prec = prec;
   c->model = model;
   c->mj_fail = mj_fail;
+  if (lr_gate) {
+    const std::vector<double> tab = lr_gate_table();
+    const size_t bytes = tab.size() * (prec == VG_PREC_FP64 ? 8 : 4);
+    if (c->lr_table.alloc(bytes) != hipSuccess) return VG_ERR_HIP;
+    if (prec == VG_PREC_FP64) {
+      if (hipMemcpy(c->lr_table.p, tab.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+    } else {
+      const std::vector<float> f(tab.begin(), tab.end());
+      if (hipMemcpy(c->lr_table.p, f.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
+    }
+  }
   c->integ = integ;
   c->n = n;
   c->n_alloc = ((n + BLK - 1) / BLK) * BLK;
@@ -1722,7 +1758,8 @@ int vg_cells_compute_ion(vg_cells *c, double Cm, double dt, const double *V, con
   hipLaunchKernelGGL((k_cells<T, M, I>), dim3(nb), dim3(BLK), 0, 0, c->V.as<double>(), sp,      \
                      c->dvdt.as<double>(), c->states.as<T>(),                                   \
                      c->acc.p ? c->acc.as<double>() : (double *)nullptr, pi, c->scales.as<T>(), \
-                     n, c->n_alloc, Cm, dt, 500, c->newton_jt, c->mj_int_abs)
+                     n, c->n_alloc, Cm, dt, 500, c->newton_jt, c->mj_int_abs,              \
+                     c->lr_table.p ? c->lr_table.as<T>() : (const T *)nullptr)
   if (c->prec == VG_PREC_FP64) {
     if (c->model == VG_MODEL_LR) VG_CELLS(double, VG_MODEL_LR, VG_INTEG_REFERENCE);
     else if (c->model == VG_MODEL_PURKINJE) VG_CELLS(double, VG_MODEL_PURKINJE, VG_INTEG_REFERENCE);

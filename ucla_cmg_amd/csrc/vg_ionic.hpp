@@ -13,9 +13,9 @@ namespace vg {
 // ---- math shims --------------------------------------------------------------
 template <class T, bool FAST> struct Mth;
 template <bool FAST> struct Mth<double, FAST> {
-  __device__ static double ex(double x) { return exp(x); }
-  __device__ static double lg(double x) { return log(x); }
-  __device__ static double sq(double x) { return sqrt(x); }
+  __host__ __device__ static double ex(double x) { return exp(x); }
+  __host__ __device__ static double lg(double x) { return log(x); }
+  __host__ __device__ static double sq(double x) { return sqrt(x); }
   __device__ static double pw(double x, double y) { return pow(x, y); }
   __device__ static double em1(double x) { return expm1(x); }
 };
@@ -40,20 +40,15 @@ template <> struct Mth<float, false> {
 constexpr int LR_NSTATE = 8;
 constexpr double LR_INIT[LR_NSTATE] = {.001, 0.0001, 0.99, .001, 0.9, .001, 0.0000001, 0};
 
-// LuoRudy::Compute_Ion + make_parameter1 (LR.cc:65-183), evaluated in T.
+// make_parameter1 (LR.cc:102-183; LRGateTable.h:114-190): at v, the 14
+// values (inf, alpha + beta) of m, h, j, d, f, X, then x1 and
+// I_K1 + I_Kp + I_b.
 template <class T, bool FAST>
-__device__ inline T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T gna_scale,
-                                   T gsi_scale) {
+__host__ __device__ inline void lr_tab(T v, T tab[14]) {
   using M = Mth<T, FAST>;
   const T c_K0 = T(5.4), c_Ki = T(145.0), RT_F = T(26.73);
-  const T Gna = T(12.00) * gna_scale;
-  const T Gsi = T(0.07) * gsi_scale;
-  const T GK = T(0.705) * M::sq(c_K0 / T(5.4));
   const T E_K1 = RT_F * M::lg(c_K0 / c_Ki);
   const T G_K1 = T(.6047) * M::sq(c_K0 / T(5.4));
-  const T r = T(11e-4), L = T(100E-4);
-  const T Xi = T(2.) * (r + L) / (r * L);
-
   T alpha_m, beta_m, alpha_h, beta_h, alpha_j, beta_j;
   if (fabs(v + T(47.13)) < T(0.001)) alpha_m = T(3.2);
   else alpha_m = T(0.32) * (v + T(47.13)) / (T(1.0) - M::ex(T(-0.1) * (v + T(47.13))));
@@ -93,22 +88,53 @@ __device__ inline T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T g
   const T kp = T(1.) / (T(1) + M::ex((T(7.488) - v) / T(5.98)));
   const T I_kp = T(0.0183) * kp * (v - E_K1);
   const T I_b = T(0.0392) * (v + T(59.87));
+  const T a[6] = {alpha_m, alpha_h, alpha_j, alpha_d, alpha_f, alpha_x};
+  const T b[6] = {beta_m, beta_h, beta_j, beta_d, beta_f, beta_x};
+  for (int i = 0; i < 6; i++) {
+    tab[2 * i] = a[i] / (a[i] + b[i]);
+    tab[2 * i + 1] = a[i] + b[i];
+  }
+  tab[12] = x1;
+  tab[13] = I_k1 + I_kp + I_b;
+}
 
-  // Rush-Larsen with the rate sums alpha+beta (LR.cc:83-85).
-  const T inf[6] = {alpha_m / (alpha_m + beta_m), alpha_h / (alpha_h + beta_h),
-                    alpha_j / (alpha_j + beta_j), alpha_d / (alpha_d + beta_d),
-                    alpha_f / (alpha_f + beta_f), alpha_x / (alpha_x + beta_x)};
-  const T sum[6] = {alpha_m + beta_m, alpha_h + beta_h, alpha_j + beta_j,
-                    alpha_d + beta_d, alpha_f + beta_f, alpha_x + beta_x};
+// Rows of LuoRudyGateTable (LRGateTable.h:67-76): make_parameter1 at
+// -90 + 0.001 i, i < LR_TABLE_ROWS.
+constexpr int LR_TABLE_ROWS = 290001;
+
+// LuoRudy::Compute_Ion (LR.cc:65-99), evaluated in T.  With a gate table
+// (LUORUDYWITHGATE) the 14 values inside (-90, 200) mV are interpolated
+// linearly from it (LuoRudyGateTable::get_parameters, LRGateTable.h:93-110).
+template <class T, bool FAST>
+__device__ inline T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T gna_scale,
+                                   T gsi_scale, const T *__restrict__ table = nullptr) {
+  using M = Mth<T, FAST>;
+  const T c_K0 = T(5.4);
+  const T Gna = T(12.00) * gna_scale;
+  const T Gsi = T(0.07) * gsi_scale;
+  const T GK = T(0.705) * M::sq(c_K0 / T(5.4));
+  const T r = T(11e-4), L = T(100E-4);
+  const T Xi = T(2.) * (r + L) / (r * L);
+  T tab[14];
+  if (table && v > T(-90) && v < T(200)) {
+    const int index = (int)((v + T(90)) * T(1000));
+    const T d = T(1000) * (v - (T(-90) + T(0.001) * T(index)));
+    const T *lo = table + (int64_t)index * 14;
 #pragma unroll
-  for (int i = 0; i < 6; i++) s[i] = inf[i] - (inf[i] - s[i]) * M::ex(-dt * sum[i]);
+    for (int i = 0; i < 14; i++) tab[i] = (lo[14 + i] - lo[i]) * d + lo[i];
+  } else {
+    lr_tab<T, FAST>(v, tab);
+  }
+  // Rush-Larsen with the rate sums alpha+beta (LR.cc:83-85).
+#pragma unroll
+  for (int i = 0; i < 6; i++) s[i] = tab[2 * i] - (tab[2 * i] - s[i]) * M::ex(-dt * tab[2 * i + 1]);
   s[7] = v;
   const T Es = T(7.7) - T(13.0287) * M::lg(s[6]);
   const T i_na = Gna * s[0] * s[0] * s[0] * s[1] * s[2] * (v - T(54.4));
   const T i_si = Gsi * s[3] * s[4] * (v - Es);
-  const T i_k = GK * x1 * s[5] * (v + T(77.0));
+  const T i_k = GK * tab[12] * s[5] * (v + T(77.0));
   s[6] = s[6] + (T(-.0001) * i_si + T(0.07) * (T(0.0001) - s[6])) * dt;
-  const T i_tot = -((I_k1 + I_kp + I_b) + i_k + i_na + i_si);
+  const T i_tot = -(tab[13] + i_k + i_na + i_si);
   return i_tot + istim / (Xi * Cm);
 }
 

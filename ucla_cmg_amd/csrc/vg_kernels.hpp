@@ -511,6 +511,8 @@ struct IonicArgs {
   float log_dt;
   // Update only the rows flagged here (the learned step's fallback), or all.
   const unsigned char *only;
+  // Luo-Rudy: LuoRudyGateTable rows in T (LUORUDYWITHGATE), or null.
+  const void *lr_table;
 };
 
 template <class T, int MODEL, int INTEG>
@@ -540,7 +542,8 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       for (int k = 0; k < LR_NSTATE - 1; k++) s[k] = st[k * a.n_alloc + i];
       ionic_substeps(sub, [&](double dtd) {
         const T dt = (T)dtd;
-        const T dv = lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3]);
+        const T dv = lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3],
+                                                       (const T *)a.lr_table);
         V += dt * dv;
         return (double)dv;
       });
@@ -966,7 +969,8 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
                                                const uint16_t *__restrict__ pidx,
                                                const T *__restrict__ scales, int64_t n,
                                                int64_t n_alloc, double Cm_d, double dt_d,
-                                               int max_newton, int newton_jt, int mj_int_abs) {
+                                               int max_newton, int newton_jt, int mj_int_abs,
+                                               const T *__restrict__ lr_table) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   const T V = (T)Vd[i], stim = stim_d ? (T)stim_d[i] : T(0), Cm = (T)Cm_d, dt = (T)dt_d;
@@ -976,7 +980,7 @@ __global__ __launch_bounds__(BLK) void k_cells(const double *__restrict__ Vd,
     T s[LR_NSTATE];
 #pragma unroll
     for (int k = 0; k < LR_NSTATE - 1; k++) s[k] = st[k * n_alloc + i];
-    dvdt[i] = (double)lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3]);
+    dvdt[i] = (double)lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3], lr_table);
 #pragma unroll
     for (int k = 0; k < LR_NSTATE - 1; k++) st[k * n_alloc + i] = s[k];
   } else if constexpr (MODEL == VG_MODEL_MJ || MODEL == VG_MODEL_MJ_FAIL) {

@@ -7,7 +7,7 @@
 //
 // Deck keywords (case-insensitive, one per line; a line containing #, $ or *
 // anywhere is a comment, EPSimulator.cc:142-146): MODEL, OUTPUTDIR, CELLMODEL
-// (LUORUDY, TUSSCHER, MAHAJAN, MAHAJAN_FAIL), MESHTYPE (MIX3D, TET), QUADORDER, DT, IONICDT,
+// (LUORUDY, LUORUDYWITHGATE, TUSSCHER, MAHAJAN, MAHAJAN_FAIL), MESHTYPE (MIX3D, TET), QUADORDER, DT, IONICDT,
 // SIMULATIONTIME, FREQUENCY, TWODRADIUS, TISSUEDIFFUSION, PURKINJEDIFFUSION,
 // LVPURKINJEDIFFUSION, SMOOTHING [alpha], INITIALSTATES, READRESTART t,
 // WRITERESTART t, STIMULUS NODAL|CIRC|ZONAL|PLANE ... (EPSimulator.cc:147-266).
@@ -181,6 +181,7 @@ int main(int argc, char **argv) {
   if (mesh_type != "MIX3D" && mesh_type != "TET") die("MESHTYPE " + mesh_type + " is not supported");
   int cm;
   if (cell_model == "LUORUDY") cm = VG_MODEL_LR;
+  else if (cell_model == "LUORUDYWITHGATE") cm = VG_MODEL_LR_GATE;
   else if (cell_model == "TUSSCHER") cm = VG_MODEL_TT;
   else if (cell_model == "MAHAJAN") cm = VG_MODEL_MJ;
   else if (cell_model == "MAHAJAN_FAIL") cm = VG_MODEL_MJ_FAIL;

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libvoomgpu.so")
 
 VG_OK = 0
 MODEL_LR = 0
+MODEL_LR_GATE = 1  # Luo-Rudy through LuoRudyGateTable (LUORUDYWITHGATE)
 MODEL_TT = 2
 MODEL_MJ = 3
 MODEL_MJ_FAIL = 4  # Mahajan with the failing-heart constants (MAHAJAN_FAIL)
