@@ -367,7 +367,8 @@ void or_tt_newton_system(const double *C, double *S, const double *stateN,
   memset(rates, 0, sizeof(rates));
   alg[12] = alg12;
   for (int i = 0; i < 6; i++) {
-    const double eps = 1e-3 * S[TT_NR[i]];
+    /* Tusscher.cc:158 takes 1e-3 x; at x = 0 that is 0/0 -- step 1e-12 then. */
+    const double eps = S[TT_NR[i]] != 0.0 ? 1e-3 * S[TT_NR[i]] : 1e-12;
     const double twoeps = 2. * eps;
     S[TT_NR[i]] += eps;
     or_tt_rates(C, S, alg, rates);
@@ -412,7 +413,8 @@ double or_tt_compute_ion(const double *C, double *S, double Cm, double dt,
   int it = 0;
   while (resid > 1e-6 && it < 500) {
     for (int i = 0; i < 6; i++) {
-      const double eps = 1e-3 * S[TT_NR[i]];
+      /* Tusscher.cc:158 takes 1e-3 x; at x = 0 that is 0/0 -- step 1e-12 then. */
+    const double eps = S[TT_NR[i]] != 0.0 ? 1e-3 * S[TT_NR[i]] : 1e-12;
       const double twoeps = 2. * eps;
       S[TT_NR[i]] += eps;
       or_tt_rates(C, S, alg, rates);

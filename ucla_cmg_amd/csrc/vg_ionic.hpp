@@ -415,7 +415,8 @@ __device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
   while (resid > T(1e-6) && it < max_newton) {
     for (int i = 0; i < 6; i++) {
       const int ki = TT_NR_STATE[i];
-      const T eps = T(1e-3) * S[ki];
+      // 1e-3 x (Tusscher.cc:158); at x = 0 (0/0 there) a step of 1e-12.
+      const T eps = S[ki] != T(0) ? T(1e-3) * S[ki] : T(1e-12);
       const T twoeps = T(2.) * eps;
       S[ki] += eps;
       tt_currents<T, false, false>(S, istim_term, g, R7);
