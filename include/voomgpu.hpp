@@ -46,6 +46,29 @@ struct Mesh {
   }
 };
 
+// Mesh::New's files (<model>.node/.ele[/.fibers/.purk/.cell/.angle],
+// Mesh/Mesh.cc:142-416) into a Mesh; ids become 0-based.
+inline Mesh readMesh(const std::string &model, double radius = 0.0275) {
+  char err[1024] = {0};
+  vg_meshfile *mf = nullptr;
+  const int rc = vg_mesh_read(model.c_str(), radius, &mf, err, sizeof err);
+  if (rc != VG_OK) throw Error(rc, std::string("readMesh: ") + err);
+  const vg_mesh *v = vg_meshfile_mesh(mf);
+  Mesh m;
+  m.xyz.assign(v->xyz, v->xyz + 3 * v->n_nodes);
+  m.elem_nn.assign(v->elem_nn, v->elem_nn + v->n_elems);
+  m.conn.assign(v->conn, v->conn + 4 * v->n_elems);
+  if (v->fibers) m.fibers.assign(v->fibers, v->fibers + 9 * v->n_elems);
+  if (v->tissue) m.tissue.assign(v->tissue, v->tissue + v->n_nodes);
+  if (v->n_pmj) {
+    m.pmj_node.assign(v->pmj_node, v->pmj_node + v->n_pmj);
+    m.pmj_data.assign(v->pmj_data, v->pmj_data + 4 * v->n_pmj);
+  }
+  m.radius = v->radius;
+  vg_meshfile_free(mf);
+  return m;
+}
+
 inline vg_model default_model(int cell_model) {
   vg_model m{};
   m.cell_model = cell_model;
@@ -149,6 +172,18 @@ class GpuCNSolver {
     std::vector<double> e(ne_);
     check(vg_ecg(h_, e.data()), "computeECG");
     return e;
+  }
+
+  // EPSolver::WriteOutput / writeRestart / readRestart (EPSolver.cc:116-211):
+  // Voltage_time_%08.2f.%02.0f and InterVariables_time_* under path.
+  void WriteOutput(double time, const std::string &path, int rank = 0) {
+    check(vg_write_voltage(h_, path.c_str(), time, rank), "WriteOutput");
+  }
+  void writeRestart(double time, const std::string &path, int rank = 0) {
+    check(vg_write_restart(h_, path.c_str(), time, rank), "writeRestart");
+  }
+  void readRestart(double time, const std::string &path, int rank = 0) {
+    check(vg_read_restart(h_, path.c_str(), time, rank), "readRestart");
   }
 
   vg_stats stats() {
