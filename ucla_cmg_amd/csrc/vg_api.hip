@@ -141,9 +141,11 @@ struct vg_handle {
   SubSteps sub{};
   // device
   DevBuf off, width, wide, col16, col32, Ahat, Krhs, sinv, diag;
-  DevBuf Va, Vb, r, w, p, s, x, states, istim, pidx, scales, at;
+  DevBuf Va, Vb, r, x, states, istim, pidx, scales, at;
+  // CG rows {r, w, s, p} interleaved, one array per parity of the iteration
+  // (the explicit scheme uses q0 as two plain work vectors).
+  DevBuf q0, q1;
   DevBuf acc;  // Mahajan: FP64 Ca-cycling and Na_i states [MJ_NACC][n_alloc]
-  DevBuf r2, w2, s2;  // ping-pong partners of r, w, s for the fused CG
   // Increment of the second half-step (x is the first's) unless both
   // half-steps share x (cg_x0 = VG_X0_LAST); with guess = 1 each solve starts
   // from what its increment buffer holds.
@@ -307,7 +309,8 @@ CgBufs cgbufs_of(vg_handle *h) {
                 h->blk_inst.as<int>(),
                 h->inst_blk.as<int>(),
                 h->inst_slice.as<int64_t>(),
-                h->n_inst};
+                h->n_inst,
+                h->n_slices};
 }
 
 // Slice -> instance table for the node-wise kernels (none with one instance).
@@ -341,7 +344,8 @@ void launch_cg_start(vg_handle *h) {
   ProfScope ps(h, VG_KC_CG_START);
   hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
-                     h->r.as<T>(), h->w.as<T>(), h->diag.as<T>(), cgbufs_of(h));
+                     h->r.as<T>(), h->q0.as<typename Vec4Of<T>::type>(), h->diag.as<T>(),
+                     cgbufs_of(h));
 }
 
 template <class T>
@@ -349,8 +353,8 @@ void launch_cg_iter(vg_handle *h, int j, int which) {
   ProfScope ps(h, VG_KC_CG_ITER);
   hipLaunchKernelGGL(k_cg_iter<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
-                     h->r.as<T>(), h->r2.as<T>(), h->w.as<T>(), h->w2.as<T>(), h->s.as<T>(),
-                     h->s2.as<T>(), h->p.as<T>(), xbuf<T>(h, which), h->diag.as<T>(),
+                     h->q0.as<typename Vec4Of<T>::type>(), h->q1.as<typename Vec4Of<T>::type>(),
+                     xbuf<T>(h, which), h->diag.as<T>(),
                      cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
 }
 
@@ -568,7 +572,7 @@ int step_explicit(vg_handle *h) {
   if (rc) return rc;
   const unsigned grid = (unsigned)std::min<int64_t>(h->cg_blocks ? h->cg_blocks : 2048,
                                                     (h->n_slices + WAVES - 1) / WAVES);
-  T *W1 = h->p.as<T>(), *W2 = h->s.as<T>();
+  T *W1 = h->q0.as<T>(), *W2 = h->q0.as<T>() + h->n_alloc;
   auto half = [&](const T *from) {
     const T *src = from;
     T *dst = W1;
@@ -1160,10 +1164,13 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     }
   }
   // Vectors.
-  for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->w, &H_->p, &H_->s, &H_->x, &H_->istim, &H_->at,
-                    &H_->r2, &H_->w2, &H_->s2, &H_->x2}) {
+  for (DevBuf *b : {&H_->Va, &H_->Vb, &H_->r, &H_->x, &H_->istim, &H_->at, &H_->x2}) {
     HIPCHK(H_, b->alloc(na * T));
     HIPCHK(H_, hipMemset(b->p, 0, na * T));
+  }
+  for (DevBuf *b : {&H_->q0, &H_->q1}) {
+    HIPCHK(H_, b->alloc(4 * na * T));
+    HIPCHK(H_, hipMemset(b->p, 0, 4 * na * T));
   }
   if (H_->model == VG_MODEL_MJ) {
     HIPCHK(H_, H_->states.alloc((size_t)MJ_NGATE * na * T));

@@ -317,7 +317,7 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
         std::sort(row.begin(), row.end());
       }
       for (int j = 0; j < o.width[s]; j++) {
-        const int64_t slot = o.offset[s] + (int64_t)j * SELL_C + l;
+        const int64_t slot = sell_slot(o.offset[s], o.width[s], j, l);
         if (j < (int)row.size()) {
           const int32_t cn = row[j].first;
           const int64_t k = row[j].second;
@@ -342,7 +342,7 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
     bool narrow = true;
     for (int j = 0; j < o.width[s] && narrow; j++)
       for (int l = 0; l < SELL_C; l++) {
-        const int64_t d = (int64_t)o.col[o.offset[s] + (int64_t)j * SELL_C + l] - (s * SELL_C + l);
+        const int64_t d = (int64_t)o.col[sell_slot(o.offset[s], o.width[s], j, l)] - (s * SELL_C + l);
         if (d < -32767 || d > 32767) {
           narrow = false;
           break;
@@ -350,10 +350,11 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
       }
     const int64_t a = o.offset[s], b = o.offset[s + 1];
     if (narrow) {
-      for (int64_t k = a; k < b; k++) {
-        const int64_t row = s * SELL_C + (k - a) % SELL_C;
-        o.col16[k] = (int16_t)(o.col[k] - row);
-      }
+      for (int j = 0; j < o.width[s]; j++)
+        for (int l = 0; l < SELL_C; l++) {
+          const int64_t k = sell_slot(a, o.width[s], j, l);
+          o.col16[k] = (int16_t)(o.col[k] - (s * SELL_C + l));
+        }
     } else {
       o.wide[s] = (int64_t)o.col32.size();
       o.col32.insert(o.col32.end(), o.col.begin() + a, o.col.begin() + b);

@@ -22,11 +22,21 @@ std::string assemble_csr(const vg_mesh &mesh, const vg_model &model, HostCsr &ou
 std::string validate_mesh(const vg_mesh &mesh);
 
 // SELL-64 layout: rows in slices of 64 lanes, one lane per row; slice s holds
-// width[s] (its longest row) entries per lane stored lane-fastest from
-// offset[s].  Columns are stored as 16-bit offsets from the row (col16) in
-// every slice whose columns all lie within +-32767 rows of their row, else as
-// int32 in col32 from wide[s] (wide[s] = -1 for a 16-bit slice).
+// width[s] (its longest row) entries per lane from offset[s]: the first
+// 4*(width/4) entries of a lane in groups of four consecutive entries
+// ([group][lane][4], so one 16-byte load per lane fetches a group), the 0-3
+// left over lane-fastest ([entry][lane]).  Columns are stored as 16-bit
+// offsets from the row (col16) in every slice whose columns all lie within
+// +-32767 rows of their row, else as int32 in col32 from wide[s] (wide[s] = -1
+// for a 16-bit slice), in the same layout.
 constexpr int SELL_C = 64;
+
+// Storage position of entry j of lane l in a slice of width w at offset off.
+inline int64_t sell_slot(int64_t off, int w, int j, int l) {
+  const int w4 = w & ~3;
+  return j < w4 ? off + (int64_t)(j >> 2) * (4 * SELL_C) + 4 * l + (j & 3)
+                : off + (int64_t)w4 * SELL_C + (int64_t)(j - w4) * SELL_C + l;
+}
 struct HostSell {
   int64_t n = 0, rows = 0, n_slices = 0, padded = 0;  // rows: n plus padding rows
   std::vector<int64_t> offset;   // [n_slices + 1]

@@ -98,12 +98,27 @@ struct CgBufs {
   const int *inst_blk;        // [n_inst + 1]
   const int64_t *inst_slice;  // [n_inst + 1]
   int n_inst;
+  int64_t n_slices;           // all slices (the block split needs no table with one instance)
 };
 
-__device__ inline double wave_sum(double v) {
+// NV wave sums, the shuffles of the NV values interleaved so their latencies
+// overlap; every lane ends with the totals.
+template <int NV>
+__device__ inline void wave_sum_n(double (&v)[NV]) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int o = 32; o > 0; o >>= 1) {
+    double t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) t[k] = __shfl_xor(v[k], o, 64);
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += t[k];
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+  double u[1] = {v};
+  wave_sum_n<1>(u);
+  return u[0];
 }
 
 // Block sum of NV values, stored as part[k * gridDim.x + blockIdx.x].
@@ -111,10 +126,13 @@ template <int NV>
 __device__ inline void put_partials(const double (&v)[NV], double *__restrict__ part) {
   __shared__ double sm[WAVES][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double u[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) {
-    const double s = wave_sum(v[k]);
-    if (lane == 0) sm[wv][k] = s;
+  for (int k = 0; k < NV; k++) u[k] = v[k];
+  wave_sum_n<NV>(u);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) sm[wv][k] = u[k];
   }
   __syncthreads();
   if (threadIdx.x < NV) {
@@ -132,12 +150,19 @@ __device__ inline void sum_partials(const double *__restrict__ part, int nblk, i
                                     double (&tot)[NV]) {
   __shared__ double sm[WAVES][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double s[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += BLK) s += part[(size_t)k * nblk + b];
-    s = wave_sum(s);
-    if (lane == 0) sm[wv][k] = s;
+  for (int k = 0; k < NV; k++) s[k] = 0.0;
+  // Block-outer, so the loads of several blocks' partials are in flight at once.
+#pragma unroll 4
+  for (int b = b0 + threadIdx.x; b < b1; b += BLK) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] += part[(size_t)k * nblk + b];
+  }
+  wave_sum_n<NV>(s);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) sm[wv][k] = s[k];
   }
   __syncthreads();
 #pragma unroll
@@ -154,6 +179,15 @@ __device__ inline void sum_partials(const double *__restrict__ part, int nblk, i
 // instance over a multiple of XCDS blocks the chunks of the blocks one XCD
 // runs are adjacent, so the rows an XCD gathers from mostly sit in its own L2.
 __device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int &b0, int &b1) {
+  if (cb.n_inst == 1) {  // from kernel arguments alone: no load before the first stream
+    const int b = blockIdx.x, g = gridDim.x;
+    const int l = g % XCDS == 0 ? (b % XCDS) * (g / XCDS) + b / XCDS : b;
+    b0 = 0;
+    b1 = g;
+    lo = cb.n_slices * l / g;
+    hi = cb.n_slices * (l + 1) / g;
+    return 0;
+  }
   const int b = blockIdx.x, k = cb.blk_inst[b];
   b0 = cb.inst_blk[k];
   b1 = cb.inst_blk[k + 1];
@@ -166,10 +200,12 @@ __device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int
 }
 
 // SELL-64 operator pattern (values are separate arrays on the same layout).
-// Slice s: width[s] entries per lane, lane-fastest from off[s]; columns as
-// 16-bit offsets from the row in col16 (same layout), or, for a slice with a
-// column further than 32767 rows away (wide[s] >= 0), as int32 in col32 from
-// wide[s].
+// Slice s: width[s] entries per lane from off[s]; the first 4*(width/4) in
+// groups of four consecutive entries per lane ([group][lane][4]: one 16-byte
+// value load and one 8-byte column load per lane fetch a group), the 0-3 left
+// over lane-fastest ([entry][lane]).  Columns are 16-bit offsets from the row
+// in col16 (same layout), or, for a slice with a column further than 32767
+// rows away (wide[s] >= 0), int32 in col32 from wide[s].
 struct Sell {
   const int64_t *off;
   const int *width;
@@ -178,39 +214,79 @@ struct Sell {
   const int *col32;
 };
 
-// sum over j < wd of v(j) * gather(col(j)) in order j = 0, 1, ...; loads go in
-// groups of 4 (then the 0-3 left over), all column and value loads of a
-// group issued before its gathers, so each wave keeps a dozen independent
-// loads in flight instead of one dependent chain.
-template <class T, class CF, class VF, class G>
-__device__ inline T sell_dot(int wd, CF colat, VF valat, G gather) {
+template <class T>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+  using type = float4;
+};
+template <>
+struct Vec4Of<double> {
+  using type = double4;
+};
+template <class C>
+struct Col4Of;
+template <>
+struct Col4Of<int16_t> {
+  using type = short4;
+};
+template <>
+struct Col4Of<int> {
+  using type = int4;
+};
+
+// sum over j < wd of v(j) * gather(col(j)) in order j = 0, 1, ... for one
+// lane of a slice whose values start at vs and columns at cs; colmap turns a
+// stored column into a row index.  Two groups (8 entries) are loaded before
+// their gathers, so each wave keeps two wide loads and 8 gathers in flight.
+template <class T, class C, class CM, class G>
+__device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restrict__ cs, int lane,
+                             CM colmap, G gather) {
+  using V4 = typename Vec4Of<T>::type;
+  using C4 = typename Col4Of<C>::type;
+  const int n4 = wd >> 2;
+  const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(vs) + lane;
+  const C4 *__restrict__ cp = reinterpret_cast<const C4 *>(cs) + lane;
   T acc = T(0);
-  int j = 0;
-  for (; j + 4 <= wd; j += 4) {
-    int c[4];
-    T v[4];
+  int g = 0;
+  for (; g + 2 <= n4; g += 2) {
+    V4 v[2];
+    C4 c[2];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      c[q] = colat(j + q);
-      v[q] = valat(j + q);
+    for (int q = 0; q < 2; q++) {
+      v[q] = vp[(g + q) * 64];
+      c[q] = cp[(g + q) * 64];
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc += v[q] * gather(c[q]);
+    for (int q = 0; q < 2; q++) {
+      acc += v[q].x * gather(colmap(c[q].x));
+      acc += v[q].y * gather(colmap(c[q].y));
+      acc += v[q].z * gather(colmap(c[q].z));
+      acc += v[q].w * gather(colmap(c[q].w));
+    }
   }
-  const int k = wd - j;
-  if (k > 0) {
-    int c[3];
-    T v[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-      if (q < k) {
-        c[q] = colat(j + q);
-        v[q] = valat(j + q);
-      }
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-      if (q < k) acc += v[q] * gather(c[q]);
+  if (g < n4) {
+    const V4 v = vp[g * 64];
+    const C4 c = cp[g * 64];
+    acc += v.x * gather(colmap(c.x));
+    acc += v.y * gather(colmap(c.y));
+    acc += v.z * gather(colmap(c.z));
+    acc += v.w * gather(colmap(c.w));
   }
+  const int rem = wd & 3;
+  const T *__restrict__ vt = vs + (int64_t)n4 * 256 + lane;
+  const C *__restrict__ ct = cs + (int64_t)n4 * 256 + lane;
+  int cc[3];
+  T vv[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    if (q < rem) {
+      cc[q] = colmap(ct[q * 64]);
+      vv[q] = vt[q * 64];
+    }
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    if (q < rem) acc += vv[q] * gather(cc[q]);
   return acc;
 }
 
@@ -218,67 +294,75 @@ __device__ inline T sell_dot(int wd, CF colat, VF valat, G gather) {
 template <class T, class G>
 __device__ inline T sell_row(const Sell &S, const T *__restrict__ val, int64_t slice, int lane,
                              G gather) {
-  const int64_t o = S.off[slice] + lane;
+  const int64_t o = S.off[slice];
   const int wd = S.width[slice];
   const int64_t wo = S.wide[slice];
-  const T *__restrict__ vp = val + o;
-  auto valat = [&](int j) { return vp[j * 64]; };
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
-    const int16_t *__restrict__ cp = S.col16 + o;
-    return sell_dot<T>(wd, [&](int j) { return row + (int)cp[j * 64]; }, valat, gather);
+    return sell_dot<T>(wd, val + o, S.col16 + o, lane, [&](int c) { return row + c; }, gather);
   }
-  const int *__restrict__ cp = S.col32 + wo + lane;
-  return sell_dot<T>(wd, [&](int j) { return cp[j * 64]; }, valat, gather);
+  return sell_dot<T>(wd, val + o, S.col32 + wo, lane, [](int c) { return c; }, gather);
 }
 
 // Two operators on one pattern in one pass (columns loaded once):
 // acc_a = sum va[j] ga(col[j]), acc_b = sum vb[j] gb(col[j]).
+template <class T, class C, class CM, class GA, class GB>
+__device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__restrict__ bs,
+                                 const C *__restrict__ cs, int lane, CM colmap, GA ga, GB gb,
+                                 T &acc_a, T &acc_b) {
+  using V4 = typename Vec4Of<T>::type;
+  using C4 = typename Col4Of<C>::type;
+  const int n4 = wd >> 2;
+  const V4 *__restrict__ ap = reinterpret_cast<const V4 *>(as) + lane;
+  const V4 *__restrict__ bp = reinterpret_cast<const V4 *>(bs) + lane;
+  const C4 *__restrict__ cp = reinterpret_cast<const C4 *>(cs) + lane;
+  T sa = T(0), sb = T(0);
+  for (int g = 0; g < n4; g++) {
+    const V4 x = ap[g * 64], y = bp[g * 64];
+    const C4 c4 = cp[g * 64];
+    const int c[4] = {colmap(c4.x), colmap(c4.y), colmap(c4.z), colmap(c4.w)};
+    const T xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sa += xs[q] * ga(c[q]);
+      sb += ys[q] * gb(c[q]);
+    }
+  }
+  const int rem = wd & 3;
+  const int64_t t = (int64_t)n4 * 256 + lane;
+  int c[3];
+  T x[3], y[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    if (q < rem) {
+      c[q] = colmap(cs[t + q * 64]);
+      x[q] = as[t + q * 64];
+      y[q] = bs[t + q * 64];
+    }
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    if (q < rem) {
+      sa += x[q] * ga(c[q]);
+      sb += y[q] * gb(c[q]);
+    }
+  acc_a = sa;
+  acc_b = sb;
+}
+
 template <class T, class GA, class GB>
 __device__ inline void sell_row2(const Sell &S, const T *__restrict__ va, const T *__restrict__ vb,
                                  int64_t slice, int lane, GA ga, GB gb, T &acc_a, T &acc_b) {
-  const int64_t o = S.off[slice] + lane;
+  const int64_t o = S.off[slice];
   const int wd = S.width[slice];
   const int64_t wo = S.wide[slice];
-  const T *__restrict__ ap = va + o;
-  const T *__restrict__ bp = vb + o;
-  // Both sums share the column loads: gather a pair, accumulate both.
-  struct P {
-    T a, b;
-  };
-  auto pairdot = [&](auto colat) {
-    T sa = T(0), sb = T(0);
-    int j = 0;
-    for (; j < wd; j += 4) {
-      int c[4];
-      T x[4], y[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (j + q < wd) {
-          c[q] = colat(j + q);
-          x[q] = ap[(j + q) * 64];
-          y[q] = bp[(j + q) * 64];
-        }
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (j + q < wd) {
-          sa += x[q] * ga(c[q]);
-          sb += y[q] * gb(c[q]);
-        }
-    }
-    return P{sa, sb};
-  };
-  P r;
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
-    const int16_t *__restrict__ cp = S.col16 + o;
-    r = pairdot([&](int j) { return row + (int)cp[j * 64]; });
+    sell_dot2<T>(wd, va + o, vb + o, S.col16 + o, lane, [&](int c) { return row + c; }, ga, gb,
+                 acc_a, acc_b);
   } else {
-    const int *__restrict__ cp = S.col32 + wo + lane;
-    r = pairdot([&](int j) { return cp[j * 64]; });
+    sell_dot2<T>(wd, va + o, vb + o, S.col32 + wo, lane, [](int c) { return c; }, ga, gb, acc_a,
+                 acc_b);
   }
-  acc_a = r.a;
-  acc_b = r.b;
 }
 
 __device__ inline bool cg_stopped(const CgResult *res) {
@@ -354,10 +438,12 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
 
 // ---- CG (Chronopoulos-Gear form, Jacobi scaling folded into A_hat) ---------
 // Start: ||b||_D^2 from the right-hand side's partials (b = 0: the increment
-// is zero); w_0 = A_hat r_0 and partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.
+// is zero); w_0 = A_hat r_0, the records {r_0, w_0, 0, 0} of parity 0, and
+// partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.
 template <class T>
 __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ val,
-                                                  const T *__restrict__ r, T *__restrict__ w,
+                                                  const T *__restrict__ r,
+                                                  typename Vec4Of<T>::type *__restrict__ q,
                                                   const T *__restrict__ diag, CgBufs cb) {
   int64_t lo, hi;
   int b0, b1;
@@ -381,8 +467,9 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
     const T acc = sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
     const int64_t row = sl * 64 + lane;
-    w[row] = acc;
-    const double rr = (double)r[row];
+    const T ri = r[row];
+    q[row] = {ri, acc, T(0), T(0)};
+    const double rr = (double)ri;
     v[0] += rr * rr;
     v[1] += rr * (double)acc;
     v[2] += (double)diag[row] * rr * rr;
@@ -401,34 +488,42 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
 //   p_j = r_j + beta_j p_{j-1},  s_j = w_j + beta_j s_{j-1},
 //   x_{j+1} = x_j + alpha_j p_j,  r_{j+1} = r_j - alpha_j s_j,
 // and w_{j+1} = A_hat r_{j+1}, rebuilding the neighbours' r_{j+1} from their
-// r_j, w_j, s_{j-1} (ping-pong buffers by parity of j), so no other block's
-// writes of this launch are read.
+// r_j, w_j, s_{j-1}.  Rows are {r, w, s, p} in one 16-byte (FP64: 32-byte)
+// record, so a neighbour costs one wide gather; launch j reads the records
+// {r_j, w_j, s_{j-1}, p_{j-1}} of parity j and writes {r_{j+1}, w_{j+1}, s_j,
+// p_j} to the other array, so no other block's writes of this launch are read.
 template <class T, bool FIRST>
 __device__ inline T cg_iter_row(const Sell &S, const T *__restrict__ val,
-                                const T *__restrict__ r0, const T *__restrict__ w0,
-                                const T *__restrict__ s0, int64_t slice, int lane, T alpha,
-                                T beta) {
+                                const typename Vec4Of<T>::type *__restrict__ q0, int64_t slice,
+                                int lane, T alpha, T beta) {
   return sell_row<T>(S, val, slice, lane, [&](int c) {
-    const T sc = FIRST ? w0[c] : w0[c] + beta * s0[c];
-    return r0[c] - alpha * sc;
+    const typename Vec4Of<T>::type g = q0[c];
+    const T sc = FIRST ? g.y : g.y + beta * g.z;
+    return g.x - alpha * sc;
   });
 }
 
 template <class T>
-__global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ val, T *rA, T *rB, T *wA,
-                                                 T *wB, T *sA, T *sB, T *__restrict__ p,
+__global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ val,
+                                                 typename Vec4Of<T>::type *qA,
+                                                 typename Vec4Of<T>::type *qB,
                                                  T *__restrict__ x, const T *__restrict__ diag,
                                                  CgBufs cb, int j, double tol2, int maxit,
                                                  int guess) {
-  CgResult *res = cb.res + cb.blk_inst[blockIdx.x];
-  if (cg_stopped(res)) return;
+  using V4 = typename Vec4Of<T>::type;
   int64_t lo, hi;
   int b0, b1;
   const int inst = block_work(cb, lo, hi, b0, b1);
+  CgResult *res = cb.res + inst;
+  // The stop flag, the carried scalars and the partials are loaded together;
+  // a launch queued past the end of its solve reads partials nobody wrote
+  // for it and discards them.
+  const bool stopped = cg_stopped(res);
   const int par = j & 1;
+  const CgCarry c = cb.carry[par * cb.n_inst + inst];
   double t[3];
   sum_partials<3>(par ? cb.part1 : cb.part0, gridDim.x, b0, b1, t);
-  const CgCarry c = cb.carry[par * cb.n_inst + inst];
+  if (stopped) return;
   const double gamma = t[0], delta = t[1], rho = t[2];
   int end = 0;  // 1: converged, 2: not converged
   double alpha = 0.0, beta = 0.0;
@@ -456,31 +551,29 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
     }
     return;
   }
-  const T *r0 = par ? rB : rA, *w0 = par ? wB : wA, *s0 = par ? sB : sA;
-  T *r1 = par ? rA : rB, *w1 = par ? wA : wB, *s1 = par ? sA : sB;
+  const V4 *__restrict__ q0 = par ? qB : qA;
+  V4 *__restrict__ q1 = par ? qA : qB;
   const T a = (T)alpha, bt = (T)beta;
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
     const int64_t row = sl * 64 + lane;
-    const T ri = r0[row], wi = w0[row];
+    const V4 own = q0[row];
+    const T ri = own.x, wi = own.y;
     T pi, si, xi, acc;
     if (j == 0) {
       pi = ri;
       si = wi;
       xi = guess ? x[row] + a * pi : a * pi;
-      acc = cg_iter_row<T, true>(S, val, r0, w0, s0, sl, lane, a, bt);
+      acc = cg_iter_row<T, true>(S, val, q0, sl, lane, a, bt);
     } else {
-      pi = ri + bt * p[row];
-      si = wi + bt * s0[row];
+      pi = ri + bt * own.w;
+      si = wi + bt * own.z;
       xi = x[row] + a * pi;
-      acc = cg_iter_row<T, false>(S, val, r0, w0, s0, sl, lane, a, bt);
+      acc = cg_iter_row<T, false>(S, val, q0, sl, lane, a, bt);
     }
     const T rn = ri - a * si;
-    r1[row] = rn;
-    w1[row] = acc;
-    s1[row] = si;
-    p[row] = pi;
+    q1[row] = {rn, acc, si, pi};
     x[row] = xi;
     const double rd = (double)rn;
     v[0] += rd * rd;
