@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -168,6 +169,8 @@ struct vg_handle {
   std::vector<int64_t> inst_slice_host;               // [n_inst + 1]
   DevBuf H, ecg_part, ecg_out, ecg_frames;
   CgResult *h_res = nullptr;  // pinned
+  CgResult *h_pub = nullptr;  // coherent host memory the finishing CG launch writes
+  CgResult *d_pub = nullptr;  // its device address
   int nsets = 1;
   int cg_guess[2] = {8, 8};  // launches expected per solve, by half-step
   int seen_max[2] = {0, 0};  // most iterations seen per half-step
@@ -226,6 +229,7 @@ struct vg_handle {
       }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_res) (void)hipHostFree(h_res);
+    if (h_pub) (void)hipHostFree(h_pub);
     if (h_tot) (void)hipHostFree(h_tot);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -310,7 +314,8 @@ CgBufs cgbufs_of(vg_handle *h) {
                 h->inst_blk.as<int>(),
                 h->inst_slice.as<int64_t>(),
                 h->n_inst,
-                h->n_slices};
+                h->n_slices,
+                h->d_pub};
 }
 
 // Slice -> instance table for the node-wise kernels (none with one instance).
@@ -385,8 +390,38 @@ int cg_account(vg_handle *h, int which) {
 // expected number of iteration launches (from the previous solve of this
 // half-step) is queued, then the result record is read back; launches after
 // the solve has ended return at once, and a solve still running gets more.
+//
+// The launch that ends a solve also writes its result to coherent host memory
+// (h_pub), so the host goes on as soon as that record appears instead of
+// waiting for the stream; the device record, copied once the stream has
+// drained, decides whenever the published one has not appeared by then.
+template <class T>
+int cg_wait(vg_handle *h, bool &all) {
+  using clock = std::chrono::steady_clock;
+  auto next_query = clock::now() + std::chrono::microseconds(20);
+  for (;;) {
+    all = true;
+    for (int k = 0; k < h->n_inst && all; k++)
+      all = __atomic_load_n(&h->h_pub[k].done, __ATOMIC_ACQUIRE) != 0;
+    if (all) {
+      std::memcpy(h->h_res, h->h_pub, sizeof(CgResult) * h->n_inst);
+      return VG_OK;
+    }
+    if (clock::now() < next_query) continue;
+    const hipError_t q = hipStreamQuery(h->stream);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) HIPCHK(h, q);
+    next_query = clock::now() + std::chrono::microseconds(20);
+  }
+  HIPCHK(h, hipMemcpy(h->h_res, h->res.p, sizeof(CgResult) * h->n_inst, hipMemcpyDeviceToHost));
+  all = true;
+  for (int k = 0; k < h->n_inst; k++) all = all && h->h_res[k].done;
+  return VG_OK;
+}
+
 template <class T>
 int cg_solve(vg_handle *h, const T *V, int which) {
+  for (int k = 0; k < h->n_inst; k++) __atomic_store_n(&h->h_pub[k].done, 0, __ATOMIC_SEQ_CST);
   launch_rhs<T>(h, V, which);
   launch_cg_start<T>(h);
   int j = 0;
@@ -394,11 +429,9 @@ int cg_solve(vg_handle *h, const T *V, int which) {
   for (;;) {
     for (int k = 0; k < batch && j <= h->maxit; k++) launch_cg_iter<T>(h, j++, which);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemcpyAsync(h->h_res, h->res.p, sizeof(CgResult) * h->n_inst,
-                             hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    bool all = true;
-    for (int k = 0; k < h->n_inst; k++) all = all && h->h_res[k].done;
+    bool all = false;
+    const int rc = cg_wait<T>(h, all);
+    if (rc) return rc;
     if (all || j > h->maxit) break;
     batch = 4;
   }
@@ -1199,6 +1232,10 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, hipHostMalloc((void **)&H_->h_tot, sizeof(CgTotals), hipHostMallocDefault));
   H_->graph = model->graph ? 1 : 0;
   HIPCHK(H_, hipHostMalloc((void **)&H_->h_res, sizeof(CgResult) * H_->n_inst, hipHostMallocDefault));
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_pub, sizeof(CgResult) * H_->n_inst,
+                           hipHostMallocCoherent | hipHostMallocMapped));
+  std::memset(H_->h_pub, 0, sizeof(CgResult) * H_->n_inst);
+  HIPCHK(H_, hipHostGetDevicePointer((void **)&H_->d_pub, H_->h_pub, 0));
   if (lr_gate) {
     const std::vector<double> tab = lr_gate_table();
     if (T == 8) {

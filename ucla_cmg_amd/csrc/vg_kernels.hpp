@@ -99,6 +99,7 @@ struct CgBufs {
   const int64_t *inst_slice;  // [n_inst + 1]
   int n_inst;
   int64_t n_slices;           // all slices (the block split needs no table with one instance)
+  CgResult *pub;              // [n_inst] host-visible copies of the results, or null
 };
 
 // NV wave sums, the shuffles of the NV values interleaved so their latencies
@@ -373,6 +374,21 @@ __device__ inline void cg_set_stop(CgResult *res) {
   __hip_atomic_store(&res->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Copies a finished solve's result into host-coherent memory, `done` last
+// with a system-scope release, so the host sees the outcome while the
+// stream's remaining launches still run.
+__device__ inline void cg_publish(CgResult *pub, int iter, int notconv, int x_valid, double rho,
+                                  double rho_ref) {
+  if (!pub) return;
+  pub->iter = iter;
+  pub->notconv = notconv;
+  pub->stop = 1;
+  pub->x_valid = x_valid;
+  pub->rho = rho;
+  pub->rho_ref = rho_ref;
+  __hip_atomic_store(&pub->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // ---- explicit diffusion stride ------------------------------------------------
 // Vout = Vin + h (-C_L^-1 K) Vin, one row per lane (EP_ExplicitSolver.cc:
 // 52-85; Kexp holds -K / C_L in the SELL layout).
@@ -459,6 +475,7 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
       res->done = 1;
       res->notconv = rho_ref == 0.0 ? 0 : 1;
       cg_set_stop(res);
+      cg_publish(cb.pub ? cb.pub + inst : nullptr, 0, res->notconv, 0, rho_ref, rho_ref);
     }
     return;
   }
@@ -548,6 +565,8 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
       res->x_valid = (j > 0 || guess) ? 1 : 0;
       res->rho = rho;
       cg_set_stop(res);
+      cg_publish(cb.pub ? cb.pub + inst : nullptr, j, end == 2 ? 1 : 0, res->x_valid, rho,
+                 c.rho_ref);
     }
     return;
   }
