@@ -462,7 +462,8 @@ void launch_ionic_t(vg_handle *h, const T *Vin, const T *x, T *Vout) {
                        h->res.as<CgResult>(), slice_inst_of(h));
     a.only = h->surr_fb.as<unsigned char>();
   }
-  hipLaunchKernelGGL((k_ionic<T, MODEL, INTEG>), dim3((unsigned)((h->n_myo + BLK - 1) / BLK)),
+  auto kern = h->sub.adaptive ? k_ionic<T, MODEL, INTEG, true> : k_ionic<T, MODEL, INTEG, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)((h->n_myo + BLK - 1) / BLK)),
                      dim3(BLK), 0, h->stream, Vin, x, h->sinv.as<T>(), Vout, h->states.as<T>(),
                      h->acc.p ? h->acc.as<double>() : (double *)nullptr,
                      h->stim_active ? h->istim.as<T>() : (const T *)nullptr,
@@ -484,7 +485,8 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
       HIPCHK(h, hipStreamWaitEvent(h->stream_pk, h->ev_fork, 0));
       sp = h->stream_pk;
     }
-    hipLaunchKernelGGL(k_ionic_pk<T>, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, sp, Vin,
+    auto kpk = h->sub.adaptive ? k_ionic_pk<T, true> : k_ionic_pk<T, false>;
+    hipLaunchKernelGGL(kpk, dim3((unsigned)(h->n_pk_alloc / BLK)), dim3(BLK), 0, sp, Vin,
                        x, h->sinv.as<T>(), Vout, h->pk_states.as<T>(),
                        h->stim_active ? h->istim.as<T>() : (const T *)nullptr, h->n_myo, h->n_pk,
                        h->n_pk_alloc, h->sub, h->res.as<CgResult>(), slice_inst_of(h));

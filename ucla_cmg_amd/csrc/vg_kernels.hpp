@@ -41,11 +41,14 @@ struct SubSteps {
 };
 
 // Runs step(dt) (which advances a node by dt and returns its dV/dt) over the
-// sub-steps of one ionic update.  The adaptive form's zero-length final call
-// (strides landing exactly on dT) changes nothing and is skipped.
-template <class F>
+// sub-steps of one ionic update: the fixed list, or (ADAPT, for sub.adaptive)
+// the per-node strides.  The adaptive form's zero-length final call (strides
+// landing exactly on dT) changes nothing and is skipped.  The two forms are
+// separate instantiations: one kernel holding both inlines the model's step
+// twice and needs half again the registers.
+template <bool ADAPT, class F>
 __device__ inline void ionic_substeps(const SubSteps &sub, F &&step) {
-  if (!sub.adaptive) {
+  if constexpr (!ADAPT) {
     for (int q = 0; q < sub.n; q++) step(sub.dt[q]);
     return;
   }
@@ -627,7 +630,7 @@ struct IonicArgs {
   const void *lr_table;
 };
 
-template <class T, int MODEL, int INTEG>
+template <class T, int MODEL, int INTEG, bool ADAPT>
 __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const T *__restrict__ x,
                                                const T *__restrict__ sinv, T *__restrict__ Vout,
                                                T *__restrict__ st, double *__restrict__ acc,
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
       T s[LR_NSTATE];
 #pragma unroll
       for (int k = 0; k < LR_NSTATE - 1; k++) s[k] = st[k * a.n_alloc + i];
-      ionic_substeps(sub, [&](double dtd) {
+      ionic_substeps<ADAPT>(sub, [&](double dtd) {
         const T dt = (T)dtd;
         const T dv = lr_compute_ion<T, sizeof(T) == 4>(s, Cm, dt, V, stim, sc[0], sc[3],
                                                        (const T *)a.lr_table);
@@ -674,7 +677,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
         s[MJ_ACC[k]] = (T)ac[k];
       }
       const T it = stim / (Cm * T(MJ_XI));
-      ionic_substeps(sub, [&](double dtd) {
+      ionic_substeps<ADAPT>(sub, [&](double dtd) {
         const T dt = (T)dtd;
         const T dv = mj_step<T, sizeof(T) == 4, MODEL == VG_MODEL_MJ_FAIL>(s, ac, V, it, dt, sc[0], sc[1], a.mj_int_abs != 0);
         V += dt * dv;
@@ -704,7 +707,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
           for (int q = 0; q < VG_TT_NSCALE; q++) rx[20 + q] = (float)sc[q];
         }
       }
-      ionic_substeps(sub, [&](double dtd) {
+      ionic_substeps<ADAPT>(sub, [&](double dtd) {
         const T dt = (T)dtd;
         T dv;
         if constexpr (INTEG == VG_INTEG_REFERENCE) {
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(BLK) void k_ionic(const T *__restrict__ Vin, const 
 
 // Purkinje segment: rows [row0, row0 + n) of the solver order, states SoA with
 // n_alloc lanes per state (all 22, the model keeps its own V copy).
-template <class T>
+template <class T, bool ADAPT>
 __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, const T *__restrict__ x,
                                                   const T *__restrict__ sinv, T *__restrict__ Vout,
                                                   T *__restrict__ st, const T *__restrict__ istim,
@@ -758,7 +761,7 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
   T s[PK_NSTATE];
 #pragma unroll
   for (int k = 0; k < PK_NSTATE; k++) s[k] = st[k * n_alloc + i];
-  ionic_substeps(sub, [&](double dtd) {
+  ionic_substeps<ADAPT>(sub, [&](double dtd) {
     const T dt = (T)dtd;
     const T dv = pk_step<T, sizeof(T) == 4>(s, V, stim, dt);
     V += dt * dv;
