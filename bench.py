@@ -86,17 +86,23 @@ def electrodes10():
 MODEL_KW = dict(D=(1.2 / 1400.0, 0.2 / 1400.0, 0.2 / 1400.0), dt=0.02, cg_tol=1e-7)
 
 
+# Operator and CG-vector layout of this build (SELL-64 with entries in groups
+# of four per lane; CG rows as {r, w, s, p} records): a traffic measurement
+# taken on another layout does not describe this kernel.
+LAYOUT = "sell64-groups4/cg-records4"
+
+
 def measured_traffic(nnz, stored):
     """HBM bytes per k_cg_iter launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when it was
-    measured on this operator layout; else None."""
+    measured on this operator and layout; else None."""
     path = os.path.join(ROOT, "profiles", "k_cg_iter_lv1m_purkinje_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except OSError:
         return None
-    if t.get("nnz") != nnz or t.get("sell_stored_entries") != stored:
+    if t.get("nnz") != nnz or t.get("sell_stored_entries") != stored or t.get("layout") != LAYOUT:
         return None
     return t["hbm_read_bytes"] + t["hbm_write_bytes"]
 
