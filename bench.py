@@ -493,7 +493,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cg-blocks", type=int, default=0,
                     help="workgroups per CG launch (0 = library default)")
-    ap.add_argument("--cg-x0", choices=["zero", "previous", "last"], default="last",
+    ap.add_argument("--cg-x0", choices=["zero", "previous", "last", "extrapolate"], default="last",
                     help="initial guess of each half-step solve (same stopping bound)")
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
                     help="replay each step as a captured hipGraph (no host syncs)")
@@ -560,7 +560,8 @@ def main():
                               integrator=vg.INTEG_RUSH_LARSEN, param_scales=scales,
                               param_index=pidx, cg_blocks=args.cg_blocks,
                               cg_x0={"zero": vg.X0_ZERO, "previous": vg.X0_PREVIOUS,
-                                     "last": vg.X0_LAST}[args.cg_x0],
+                                     "last": vg.X0_LAST,
+                                     "extrapolate": vg.X0_EXTRAPOLATE}[args.cg_x0],
                               graph=args.graph,
                               **MODEL_KW)
     solver.enableActivation(-40.0)

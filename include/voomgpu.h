@@ -182,6 +182,10 @@ typedef struct {
                             reference's Krylov iterates */
 #define VG_X0_PREVIOUS 1 /* the previous increment of the same half-step */
 #define VG_X0_LAST 2     /* the increment of the last solve (either half-step) */
+#define VG_X0_EXTRAPOLATE 3 /* each half-step's increment extrapolated from its
+                               own last three, quadratic in time (the first
+                               solves use as many as they have) */
+#define VG_X0_EXTRAPOLATE_LINEAR 4 /* the same from its last two, linear */
 
 typedef struct {
   int64_t steps;
@@ -255,7 +259,8 @@ int vg_profile_get(vg_handle *h, int kernel_class, double *total_ms, int64_t *la
 #define VG_KC_CG_START 2 /* w_0 = A r_0 */
 #define VG_KC_IONIC 3    /* ionic update, Purkinje kernel included */
 #define VG_KC_APPLY 4
-#define VG_KC_COUNT 5
+#define VG_KC_PK_PRESOLVE 5 /* Purkinje rows of the initial guess, solved locally */
+#define VG_KC_COUNT 6
 
 int vg_stats_get(vg_handle *h, vg_stats *out);
 

@@ -39,7 +39,7 @@ SCALES = np.array([[1.0, 1.0, 1.0, 1.0, 1.0], [0.6, 1.4, 1.0, 1.2, 1.0],
                    [1.4, 0.5, 1.0, 0.7, 1.0]])
 
 
-@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_LAST])
+@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_LAST, vg.X0_EXTRAPOLATE])
 def test_batch_fp64_instances_match_their_own_oracle_runs(gpu_device, cg_x0):
     parts, mesh, inst, offs, stim = _batch(DIMS, RATIOS, SCALES)
     dt, steps = 0.02, 300
@@ -143,7 +143,8 @@ def _run(mesh, kw, stim, amp, steps, stim_off, ecg=None):
     return out
 
 
-def test_captured_step_equals_host_driven_step(gpu_device):
+@pytest.mark.parametrize("cg_x0", [vg.X0_LAST, vg.X0_EXTRAPOLATE])
+def test_captured_step_equals_host_driven_step(gpu_device, cg_x0):
     """The hipGraph replay runs the same kernels with the same launch indices
     (spare launches return at once), so V, activation times, ECG frames and
     iteration counts agree bit for bit with the host-driven step, across a
@@ -151,7 +152,7 @@ def test_captured_step_equals_host_driven_step(gpu_device):
     parts, mesh, inst, offs, stim = _batch(DIMS, RATIOS, SCALES)
     kw = dict(cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN,
               dt=0.02, D=(DL, DL, DL), cg_tol=1e-7, param_scales=SCALES,
-              param_index=inst.astype(np.uint16), instance=inst, cg_x0=vg.X0_LAST)
+              param_index=inst.astype(np.uint16), instance=inst, cg_x0=cg_x0)
     el = np.array([[0.2, 0.1, 2.0], [3.0, 0.5, -1.0]])
     a = _run(mesh, dict(kw, graph=0), stim, 1e5, 400, 100, el)
     b = _run(mesh, dict(kw, graph=1), stim, 1e5, 400, 100, el)
@@ -162,12 +163,14 @@ def test_captured_step_equals_host_driven_step(gpu_device):
     assert b[2]["cg_not_converged"] == 0
 
 
-def test_captured_step_with_purkinje_stream(gpu_device):
-    """The Purkinje ionic kernel runs on its own stream inside the capture."""
+@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_LAST])
+def test_captured_step_with_purkinje_stream(gpu_device, cg_x0):
+    """The Purkinje ionic kernel runs on its own stream inside the capture
+    (with a guess, so does the local solve of the guess's Purkinje rows)."""
     from test_gpu_purkinje import cable_slab
     mesh, n, target = cable_slab()
     kw = dict(cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN,
-              dt=0.02, D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, cg_tol=1e-7)
+              dt=0.02, D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, cg_tol=1e-7, cg_x0=cg_x0)
     root = np.arange(n, n + 3, dtype=np.int32)
     a = _run(mesh, dict(kw, graph=0), root, 2.5e5, 500, 100)
     b = _run(mesh, dict(kw, graph=1), root, 2.5e5, 500, 100)

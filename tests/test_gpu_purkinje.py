@@ -38,12 +38,16 @@ def cable_slab(ncable=20, L=0.05, pmj_len=0.2):
     return mesh, n, target
 
 
-def test_purkinje_cable_drives_slab_fp64(gpu_device):
+@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_LAST, vg.X0_EXTRAPOLATE])
+def test_purkinje_cable_drives_slab_fp64(gpu_device, cg_x0):
+    """FP64 against the oracle.  Solves that start from a guess (which also
+    solve the Purkinje rows of the guess locally first) stop at the same
+    residual bound from another start: V agrees to the CG tolerance's reach."""
     mesh, n, target = cable_slab()
     dt = 0.02
     kw = dict(D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, dt=dt)
     gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_FP64,
-                           integrator=vg.INTEG_REFERENCE, cg_tol=1e-10, **kw)
+                           integrator=vg.INTEG_REFERENCE, cg_tol=1e-10, cg_x0=cg_x0, **kw)
     nv, st = gpu.getStateVariablesRagged()
     assert (nv[:n] == 19).all() and (nv[n:] == 22).all()
     V0 = gpu.getVoltage()
@@ -62,15 +66,18 @@ def test_purkinje_cable_drives_slab_fp64(gpu_device):
     Vg = gpu.getVoltage()
     err = np.abs(Vg - ref.V).max()
     at_g, at_o = gpu.activationTimes(), ref.act
-    print(f"Purkinje->slab: max|dV| {err:.3e} mV, AT root {at_o[n]:.3f}, target {at_o[target]:.3f}, "
-          f"slab activated {(at_o[:n] >= 0).mean():.2f}")
+    st = gpu.stats()
+    print(f"Purkinje->slab x0 {cg_x0}: max|dV| {err:.3e} mV, AT root {at_o[n]:.3f}, target "
+          f"{at_o[target]:.3f}, slab activated {(at_o[:n] >= 0).mean():.2f}, CG its/solve "
+          f"{st['cg_iterations'] / st['cg_solves']:.2f}")
+    tol, rtol, atol = (1e-6, 1e-9, 1e-12) if cg_x0 == vg.X0_ZERO else (1e-4, 1e-5, 1e-8)
     # The junction fires the slab near 10.8 ms; by 16 ms the wave has crossed
     # about half of it (0.46 of the nodes in the oracle).
     assert at_o[target] > 0 and (at_o[:n] >= 0).mean() > 0.25
-    assert err < 1e-6
+    assert err < tol
     both = at_o >= 0
     np.testing.assert_array_equal(at_g >= 0, both)
-    assert np.abs(at_g[both] - at_o[both]).max() < 1e-6
+    assert np.abs(at_g[both] - at_o[both]).max() < tol
     nv2, st2 = gpu.getStateVariablesRagged()
-    np.testing.assert_allclose(st2[n + 5], ref.pk_states[n + 5], rtol=1e-9, atol=1e-12)
-    np.testing.assert_allclose(st2[target][1:], ref.states[target][1:], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(st2[n + 5], ref.pk_states[n + 5], rtol=rtol, atol=atol)
+    np.testing.assert_allclose(st2[target][1:], ref.states[target][1:], rtol=rtol, atol=atol)

@@ -37,7 +37,7 @@ def _run_pair(mesh, model, steps, dt, stim_nodes, istim, stim_steps, gpu_kw, ora
 GRIDS = [0, 16]
 
 
-@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_PREVIOUS, vg.X0_LAST])
+@pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_PREVIOUS, vg.X0_LAST, vg.X0_EXTRAPOLATE])
 @pytest.mark.parametrize("cg_blocks", GRIDS)
 @pytest.mark.parametrize("model", [vg.MODEL_LR, vg.MODEL_TT])
 def test_fp64_step_matches_oracle(gpu_device, model, cg_blocks, cg_x0):
@@ -141,7 +141,7 @@ def test_previous_increment_guess_saves_iterations(gpu_device):
     mesh = meshgen.slab(nx, ny, nz, 0.025)
     stim = meshgen.slab_corner_nodes(nx, ny, nz, 3)
     res = {}
-    for x0 in (vg.X0_ZERO, vg.X0_PREVIOUS, vg.X0_LAST):
+    for x0 in (vg.X0_ZERO, vg.X0_PREVIOUS, vg.X0_LAST, vg.X0_EXTRAPOLATE):
         gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, dt=0.02, D=D_SLAB,
                                precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE,
                                cg_tol=1e-10, cg_x0=x0)
@@ -152,15 +152,19 @@ def test_previous_increment_guess_saves_iterations(gpu_device):
         st = gpu.stats()
         res[x0] = (gpu.getVoltage(), st["cg_iterations"] / st["cg_solves"], st["cg_not_converged"])
         gpu.close()
-    (V0, it0, nc0), (V1, it1, nc1), (V2, it2, nc2) = (res[vg.X0_ZERO], res[vg.X0_PREVIOUS],
-                                                      res[vg.X0_LAST])
+    (V0, it0, nc0), (V1, it1, nc1), (V2, it2, nc2), (V3, it3, nc3) = (
+        res[vg.X0_ZERO], res[vg.X0_PREVIOUS], res[vg.X0_LAST], res[vg.X0_EXTRAPOLATE])
     print(f"CG iterations per solve: zero guess {it0:.2f}, previous increment {it1:.2f}, "
-          f"last increment {it2:.2f}; max|dV| {np.abs(V0 - V1).max():.2e}, "
-          f"{np.abs(V0 - V2).max():.2e} mV")
-    assert nc0 == 0 and nc1 == 0 and nc2 == 0
+          f"last increment {it2:.2f}, extrapolated {it3:.2f}; max|dV| "
+          f"{np.abs(V0 - V1).max():.2e}, {np.abs(V0 - V2).max():.2e}, "
+          f"{np.abs(V0 - V3).max():.2e} mV")
+    assert nc0 == 0 and nc1 == 0 and nc2 == 0 and nc3 == 0
     assert it1 < 0.9 * it0 and it2 < 0.9 * it0
+    # The quadratic extrapolation in time beats both one-increment guesses.
+    assert it3 < 0.95 * min(it1, it2)
     assert V0.max() > 0.0
-    assert np.abs(V0 - V1).max() < 1e-4 and np.abs(V0 - V2).max() < 1e-4
+    for V in (V1, V2, V3):
+        assert np.abs(V0 - V).max() < 1e-4
 
 
 def test_far_columns_int32_slices(gpu_device):

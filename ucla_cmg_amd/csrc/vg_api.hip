@@ -153,6 +153,13 @@ struct vg_handle {
   DevBuf x2;
   int guess = 0;
   bool shared_x = false;
+  // cg_x0 = VG_X0_EXTRAPOLATE: each half-step's two earlier increments
+  // (xhist: d_{n-1} and d_{n-2} of half-step 0, then of half-step 1) and the
+  // number of its solves so far (capped at 3).
+  bool extrap = false;
+  int extrap_order = 3;
+  DevBuf xhist;
+  int solves_done[2] = {0, 0};
   // Rows [0, n_myo) run the body's cell model, rows [n_myo, n) the Purkinje
   // model (their states in pk_states, SoA with n_pk_alloc lanes per state).
   int64_t n_myo = 0, n_pk = 0, n_pk_alloc = 0;
@@ -329,6 +336,19 @@ T *xbuf(vg_handle *h, int which) {
   return (which && !h->shared_x) ? h->x2.as<T>() : h->x.as<T>();
 }
 
+// Local solve of the Purkinje rows of the guess (solves that start from a
+// guess, one body, a tree small enough for one workgroup's LDS).
+template <class T>
+void launch_pk_presolve(vg_handle *h, const T *V, int which) {
+  if (!h->guess || h->n_pk == 0 || h->n_inst != 1) return;
+  const int64_t cap = std::min<int64_t>((int64_t)PK_BLK * PK_ROWS, 61440 / (int64_t)sizeof(T));
+  if (h->n_pk > cap) return;
+  ProfScope ps(h, VG_KC_PK_PRESOLVE);
+  hipLaunchKernelGGL(k_pk_presolve<T>, dim3(1), dim3(PK_BLK), (unsigned)(h->n_pk * sizeof(T)),
+                     h->stream, sell_of(h), h->Ahat.as<T>(), h->Krhs.as<T>(), V, xbuf<T>(h, which),
+                     h->n_myo, (int)h->n_pk);
+}
+
 template <class T>
 void launch_rhs(vg_handle *h, const T *V, int which) {
   ProfScope ps(h, VG_KC_RHS);
@@ -344,13 +364,22 @@ void launch_rhs(vg_handle *h, const T *V, int which) {
                        cgbufs_of(h));
 }
 
+// The solve of half-step `which` turns the other half-step's history into
+// that one's next guess (its increment has been used by then).
 template <class T>
-void launch_cg_start(vg_handle *h) {
+void launch_cg_start(vg_handle *h, int which) {
   ProfScope ps(h, VG_KC_CG_START);
+  XHist<T> hist{nullptr, nullptr, nullptr, 0};
+  if (h->extrap) {
+    const int o = 1 - which;
+    T *base = h->xhist.as<T>() + (size_t)(2 * o) * h->n_alloc;
+    hist = XHist<T>{xbuf<T>(h, o), base, base + h->n_alloc,
+                    std::min(h->extrap_order, h->solves_done[o])};
+  }
   hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
                      h->r.as<T>(), h->q0.as<typename Vec4Of<T>::type>(), h->diag.as<T>(),
-                     cgbufs_of(h));
+                     cgbufs_of(h), hist);
 }
 
 template <class T>
@@ -422,8 +451,9 @@ int cg_wait(vg_handle *h, bool &all) {
 template <class T>
 int cg_solve(vg_handle *h, const T *V, int which) {
   for (int k = 0; k < h->n_inst; k++) __atomic_store_n(&h->h_pub[k].done, 0, __ATOMIC_SEQ_CST);
+  launch_pk_presolve<T>(h, V, which);
   launch_rhs<T>(h, V, which);
-  launch_cg_start<T>(h);
+  launch_cg_start<T>(h, which);
   int j = 0;
   int batch = h->cg_guess[which];
   for (;;) {
@@ -435,6 +465,7 @@ int cg_solve(vg_handle *h, const T *V, int which) {
     if (all || j > h->maxit) break;
     batch = 4;
   }
+  h->solves_done[which] = std::min(3, h->solves_done[which] + 1);
   return cg_account(h, which);
 }
 
@@ -644,8 +675,9 @@ int capture_step(vg_handle *h) {
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   const T *V[2] = {h->Va.as<T>(), h->Vb.as<T>()};
   for (int w = 0; w < 2; w++) {
+    launch_pk_presolve<T>(h, V[w], w);
     launch_rhs<T>(h, V[w], w);
-    launch_cg_start<T>(h);
+    launch_cg_start<T>(h, w);
     for (int j = 0; j < h->glaunch[w]; j++) launch_cg_iter<T>(h, j, w);
     launch_fold(h, w, true, false);
     if (w == 0) {
@@ -996,7 +1028,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown precision");
   }
-  if (model->cg_x0 != VG_X0_ZERO && model->cg_x0 != VG_X0_PREVIOUS && model->cg_x0 != VG_X0_LAST) {
+  if (model->cg_x0 != VG_X0_ZERO && model->cg_x0 != VG_X0_PREVIOUS && model->cg_x0 != VG_X0_LAST &&
+      model->cg_x0 != VG_X0_EXTRAPOLATE && model->cg_x0 != VG_X0_EXTRAPOLATE_LINEAR) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown cg_x0");
   }
@@ -1022,6 +1055,8 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->maxit = model->cg_maxit;
   h->guess = model->cg_x0 != VG_X0_ZERO ? 1 : 0;
   h->shared_x = model->cg_x0 == VG_X0_LAST;
+  h->extrap = model->cg_x0 == VG_X0_EXTRAPOLATE || model->cg_x0 == VG_X0_EXTRAPOLATE_LINEAR;
+  h->extrap_order = model->cg_x0 == VG_X0_EXTRAPOLATE_LINEAR ? 2 : 3;
   h->tsz = h->prec == VG_PREC_FP64 ? 8 : 4;
   if (model->scheme != VG_SCHEME_CN && model->scheme != VG_SCHEME_BE &&
       model->scheme != VG_SCHEME_EXPLICIT) {
@@ -1206,6 +1241,10 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   for (DevBuf *b : {&H_->q0, &H_->q1}) {
     HIPCHK(H_, b->alloc(4 * na * T));
     HIPCHK(H_, hipMemset(b->p, 0, 4 * na * T));
+  }
+  if (H_->extrap) {
+    HIPCHK(H_, H_->xhist.alloc(4 * na * T));
+    HIPCHK(H_, hipMemset(H_->xhist.p, 0, 4 * na * T));
   }
   if (H_->model == VG_MODEL_MJ) {
     HIPCHK(H_, H_->states.alloc((size_t)MJ_NGATE * na * T));

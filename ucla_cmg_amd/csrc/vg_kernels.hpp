@@ -408,11 +408,118 @@ __global__ __launch_bounds__(BLK) void k_explicit(Sell S, const T *__restrict__ 
   }
 }
 
+// ---- Purkinje rows of the initial guess ----------------------------------------
+// Before the right-hand side of a solve that starts from a guess, one
+// workgroup improves the guess on the Purkinje rows [r0, r0 + npk) with the
+// myocardium rows held: PK_ITERS CG iterations on A_pp e = (b - A_hat x)_p,
+// A_pp the Purkinje rows' block of A_hat, then x_p += e.  The Purkinje tree's
+// fronts are sharp in time, so any guess carried from earlier solves is poor
+// there, and those ~0.4 % of the rows set the initial residual; A_pp is well
+// conditioned (Jacobi-scaled mass-dominated cable, eigenvalues ~[0.5, 1.5] on
+// the LV tree), so a few local iterations remove that part.  The solve that
+// follows stops on the same test as before: only its starting point moves.
+// The search direction lives in LDS (dynamic, npk values); each thread owns
+// rows tid + k PK_BLK.
+constexpr int PK_BLK = 1024;
+constexpr int PK_ROWS = 6;  // rows per thread: at most PK_BLK * PK_ROWS Purkinje rows
+constexpr int PK_ITERS = 8;
+
+__device__ inline double pk_block_sum(double v, double *sm) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();  // earlier readers of sm are done
+  if (lane == 0) sm[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < PK_BLK / 64; w++) s += sm[w];
+  return s;
+}
+
+template <class T>
+__global__ __launch_bounds__(PK_BLK) void k_pk_presolve(Sell S, const T *__restrict__ Ahat,
+                                                        const T *__restrict__ Krhs,
+                                                        const T *__restrict__ V, T *__restrict__ x,
+                                                        int64_t r0, int npk) {
+  extern __shared__ double pk_lds[];
+  T *p = reinterpret_cast<T *>(pk_lds);
+  __shared__ double sm[PK_BLK / 64];
+  const int tid = threadIdx.x;
+  T e[PK_ROWS], rr[PK_ROWS];
+  double rz = 0.0;
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; k++) {
+    const int i = tid + k * PK_BLK;
+    e[k] = rr[k] = T(0);
+    if (i < npk) {
+      const int64_t row = r0 + i;
+      const T vi = V[row];
+      T bk, ax;
+      sell_row2<T>(S, Krhs, Ahat, row >> 6, (int)(row & 63), [&](int c) { return V[c] - vi; },
+                   [&](int c) { return x[c]; }, bk, ax);
+      rr[k] = bk - ax;
+      p[i] = rr[k];
+      rz += (double)rr[k] * (double)rr[k];
+    }
+  }
+  rz = pk_block_sum(rz, sm);  // its second barrier also publishes p
+  for (int it = 0; it < PK_ITERS && rz > 0.0; it++) {
+    T q[PK_ROWS];
+    double pq = 0.0;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      const int i = tid + k * PK_BLK;
+      q[k] = T(0);
+      if (i < npk) {
+        const int64_t row = r0 + i;
+        q[k] = sell_row<T>(S, Ahat, row >> 6, (int)(row & 63), [&](int c) {
+          const int64_t j = (int64_t)c - r0;
+          return (j >= 0 && j < npk) ? p[j] : T(0);
+        });
+        pq += (double)p[i] * (double)q[k];
+      }
+    }
+    pq = pk_block_sum(pq, sm);  // every read of p in this iteration is done
+    if (!(pq > 0.0) || !isfinite(pq)) break;
+    const T a = (T)(rz / pq);
+    double rz2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      const int i = tid + k * PK_BLK;
+      if (i < npk) {
+        e[k] += a * p[i];
+        rr[k] -= a * q[k];
+        rz2 += (double)rr[k] * (double)rr[k];
+      }
+    }
+    rz2 = pk_block_sum(rz2, sm);
+    const T b = (T)(rz2 / rz);
+    rz = rz2;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      const int i = tid + k * PK_BLK;
+      if (i < npk) p[i] = rr[k] + b * p[i];
+    }
+    __syncthreads();  // the new p before the next product
+  }
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; k++) {
+    const int i = tid + k * PK_BLK;
+    if (i < npk) x[r0 + i] += e[k];
+  }
+}
+
 // ---- diffusion right-hand side ---------------------------------------------
 // b = K_rhs V and r_0 = b - A_hat x_0 (x_0 = 0 unless GUESS: then x holds the
-// increment of the previous solve of this half-step); partials of ||b||_D^2
-// and ||r_0||_D^2.  Opens a solve: clears the result record (no CG launch of
-// the previous solve runs concurrently: launches on one stream are ordered).
+// guess of this solve); partials of ||b||_D^2 and ||r_0||_D^2.  Opens a solve:
+// clears the result record (no CG launch of the previous solve runs
+// concurrently: launches on one stream are ordered).
+// Every row of the stiffness sums to zero (K 1 = 0), so b_i is formed as
+// sum_j K_ij (V_j - V_i): the same value, without summing terms of ~|K_ij| 85
+// mV that cancel to a small b_i.  In FP32 that cancellation leaves rounding
+// noise of ~1e-7 |K_ii| 85 mV in every row, and in a tissue mostly at rest the
+// noise of 1M rows outweighs the wavefront's b, so a solve to 1e-7 ||b|| had
+// to resolve the noise.
 template <class T, bool GUESS>
 __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
                                              const T *__restrict__ Ahat,
@@ -436,16 +543,17 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
     const int64_t row = sl * 64 + lane;
     const double d = (double)diag[row];
+    const T vi = V[row];
     if constexpr (GUESS) {
       T bk, ax;
-      sell_row2<T>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c]; },
+      sell_row2<T>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c] - vi; },
                    [&](int c) { return x[c]; }, bk, ax);
       const T rr = bk - ax;
       r[row] = rr;
       v[0] += d * (double)bk * (double)bk;
       v[1] += d * (double)rr * (double)rr;
     } else {
-      const T bk = sell_row<T>(S, Krhs, sl, lane, [&](int c) { return V[c]; });
+      const T bk = sell_row<T>(S, Krhs, sl, lane, [&](int c) { return V[c] - vi; });
       r[row] = bk;
       const double e = d * (double)bk * (double)bk;
       v[0] += e;
@@ -455,19 +563,47 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
   put_partials<2>(v, cb.part1);
 }
 
+// Increment history of one half-step for the extrapolated initial guess: x
+// holds that half-step's last increment d_n (its CG unknown), y = d_{n-1},
+// z = d_{n-2}.  order = how many of them are valid (0: no history kept).
+template <class T>
+struct XHist {
+  T *x, *y, *z;
+  int order;
+};
+
+// Turns a half-step's history into the guess of its next solve, row by row:
+// x <- d_n (order 1), 2 d_n - d_{n-1} (2) or d_{n-2} + 3 (d_n - d_{n-1}) (3),
+// and shifts the history.  Runs inside the other half-step's solve, after the
+// consumers of d_n (the ionic update or the apply) and before that
+// half-step's next right-hand side, which gathers the guess.
+template <class T>
+__device__ inline void rotate_history(const XHist<T> &h, int64_t row) {
+  const T xn = h.x[row], y = h.y[row], z = h.z[row];
+  const T g = h.order == 1 ? xn : h.order == 2 ? xn + (xn - y) : z + T(3) * (xn - y);
+  h.z[row] = y;
+  h.y[row] = xn;
+  h.x[row] = g;
+}
+
 // ---- CG (Chronopoulos-Gear form, Jacobi scaling folded into A_hat) ---------
 // Start: ||b||_D^2 from the right-hand side's partials (b = 0: the increment
 // is zero); w_0 = A_hat r_0, the records {r_0, w_0, 0, 0} of parity 0, and
-// partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.
+// partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.  With hist.order > 0 it
+// also rotates the other half-step's increment history (row-local).
 template <class T>
 __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ val,
                                                   const T *__restrict__ r,
                                                   typename Vec4Of<T>::type *__restrict__ q,
-                                                  const T *__restrict__ diag, CgBufs cb) {
+                                                  const T *__restrict__ diag, CgBufs cb,
+                                                  XHist<T> hist) {
   int64_t lo, hi;
   int b0, b1;
   const int inst = block_work(cb, lo, hi, b0, b1);
   CgResult *res = cb.res + inst;
+  if (hist.order > 0)
+    for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES)
+      rotate_history(hist, sl * 64 + (threadIdx.x & 63));
   double t0[2];
   sum_partials<2>(cb.part1, gridDim.x, b0, b1, t0);
   const double rho_ref = t0[0];

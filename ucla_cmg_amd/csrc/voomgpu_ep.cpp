@@ -13,7 +13,7 @@
 // WRITERESTART t, STIMULUS NODAL|CIRC|ZONAL|PLANE ... (EPSimulator.cc:147-266).
 // Mechanics keywords are read and ignored.  Keywords of this driver only:
 // GPUPRECISION FP64|MIXED, IONICINTEGRATOR REFERENCE|RUSHLARSEN, CGX0
-// ZERO|PREVIOUS|LAST, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE, and
+// ZERO|PREVIOUS|LAST|EXTRAPOLATE, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE, and
 // SOLVERTYPE CRANKNICKOLSON|BACKWARDEULER|EXPLICIT (the reference fixes
 // CRANKNICKOLSON, EPSimulator.cc:88, and has getSolverType for the others).
 //
@@ -132,7 +132,10 @@ int main(int argc, char **argv) {
       integ = upper(t.at(1)) == "RUSHLARSEN" ? VG_INTEG_RUSH_LARSEN : VG_INTEG_REFERENCE;
     else if (k == "CGX0") {
       const std::string v = upper(t.at(1));
-      cg_x0 = v == "LAST" ? VG_X0_LAST : v == "PREVIOUS" ? VG_X0_PREVIOUS : VG_X0_ZERO;
+      cg_x0 = v == "LAST"          ? VG_X0_LAST
+              : v == "PREVIOUS"    ? VG_X0_PREVIOUS
+              : v == "EXTRAPOLATE" ? VG_X0_EXTRAPOLATE
+                                   : VG_X0_ZERO;
     } else if (k == "GRAPH") graph = (int)num(t, 1);
     else if (k == "QUIRKS") quirks = (unsigned)num(t, 1);
     else if (k == "CGTOLERANCE") cg_tol = num(t, 1), tol_given = true;

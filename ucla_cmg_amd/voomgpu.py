@@ -39,10 +39,10 @@ QUIRK_BAR_1X_VOLUME = 4
 QUIRK_NEWTON_JT = 8
 QUIRK_MJ_INT_ABS = 16
 TT_NSCALE = 5
-X0_ZERO, X0_PREVIOUS, X0_LAST = 0, 1, 2
+X0_ZERO, X0_PREVIOUS, X0_LAST, X0_EXTRAPOLATE, X0_EXTRAPOLATE_LINEAR = 0, 1, 2, 3, 4
 SCHEME_CN, SCHEME_BE, SCHEME_EXPLICIT = 0, 1, 2
-KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY = range(5)
-KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply"]
+KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY, KC_PK_PRESOLVE = range(6)
+KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply", "pk_presolve"]
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = [
