@@ -7,9 +7,13 @@ bar nodes, 256 PMJs, Corrias Purkinje cells), endo/M/epi layers (30/40/30 %)
 and an apicobasal G_Ks gradient, fibers +60 -> -60 degrees, FP32 ionic + SpMV
 with FP64 CG dots, Rush-Larsen ionic integrator, dt = 0.02 ms, CG stopping at
 ||r|| <= 1e-7 ||b - A V^n|| (the reference's r0, EP_CrankNicholson.cc:104),
-each solve started from the last increment (--cg-x0), activation times and a
-10-electrode pseudo-ECG recorded every 1 ms.  A step is one
-EP_CrankNicholson::Solve over all nodes; the metric is node-steps per second.
+each solve started from the last increment (--cg-x0) with the Purkinje rows
+of that guess solved locally first, activation times and a 10-electrode
+pseudo-ECG recorded every 1 ms.  A step is one EP_CrankNicholson::Solve over
+all nodes; the metric is node-steps per second over the whole configured beat
+(400 ms: 100 warm-up steps with the His stimulus and the autotuner, then
+19,900 timed steps), since the CG iterations per step change with the
+activity (fewest while the tree conducts, most in the plateau).
 
 Prints ONE JSON line (rank 0).  --gpus N with N > 1 is refused: this build
 measures one GPU.
@@ -87,9 +91,10 @@ MODEL_KW = dict(D=(1.2 / 1400.0, 0.2 / 1400.0, 0.2 / 1400.0), dt=0.02, cg_tol=1e
 
 
 # Operator and CG-vector layout of this build (SELL-64 with entries in groups
-# of four per lane; CG rows as {r, w, s, p} records): a traffic measurement
-# taken on another layout does not describe this kernel.
-LAYOUT = "sell64-groups4/cg-records4"
+# of four per lane, off-diagonal entries only; CG rows as {r, w, s, p}
+# records): a traffic measurement taken on another layout does not describe
+# this kernel.
+LAYOUT = "sell64-groups4-offdiag/cg-records4"
 
 
 def measured_traffic(nnz, stored):
@@ -486,7 +491,8 @@ def surrogate_bench(args, vg, torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=19900,
+                    help="timed steps (default: the rest of the 400 ms beat after the warm-up)")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh size factor (1.0 = 1.03M nodes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -606,9 +612,23 @@ def main():
     stream = torch.cuda.ExternalStream(solver.stream())
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    # The timed run in windows of ~2000 steps, each closed by a stream
+    # synchronisation (one per ~2 ms of GPU work), so the per-phase cost of
+    # the beat (CG iterations vary with the activity) is reported too.
+    win = max(1, min(2000, args.steps))
+    phases = []
     ev0.record(stream)
     t0 = time.perf_counter()
-    run(k0, k0 + args.steps)
+    k = k0
+    while k < k0 + args.steps:
+        k1 = min(k0 + args.steps, k + win)
+        tw, sw = time.perf_counter(), solver.stats()["cg_iterations"]
+        run(k, k1)
+        solver.synchronize()
+        phases.append({"t_ms": round(k * 0.02, 2), "steps": k1 - k,
+                       "ms_per_step": (time.perf_counter() - tw) * 1e3 / (k1 - k),
+                       "cg_iters_per_step": (solver.stats()["cg_iterations"] - sw) / (k1 - k)})
+        k = k1
     ev1.record(stream)
     solver.synchronize()
     torch.cuda.synchronize()
@@ -642,8 +662,11 @@ def main():
     it_ms, it_launches = prof["cg_iter"]
     nnz = st1["nnz"]
     it_bytes = 8 * nnz + 44 * n
-    # Launches after convergence exit at once; their few microseconds are
-    # charged to the launches that did an iteration (conservative).
+    # The library times each solve's iteration launches as one interval (an
+    # event pair around all of them, so the launches run back to back as in
+    # the timed region, launch boundaries included).  Launches after
+    # convergence exit at once; their few microseconds are charged to the
+    # launches that did an iteration (conservative).
     it_avg_s = (it_ms / 1e3) / max(1, prof_its) if it_launches else float("nan")
     achieved = it_bytes / it_avg_s / 1e9
     traffic = measured_traffic(nnz, st1["sell_padded_nnz"])
@@ -666,7 +689,9 @@ def main():
         "config": {
             "workload": f"tT2006 LV shell + Purkinje tree ({args.purkinje} PMJ resistance), "
                         f"{n} nodes, {mesh.n_elems} elements, plain path (Rush-Larsen ionic), "
-                        "dt 0.02 ms, CG tol 1e-7, AT + 10-lead ECG/ms",
+                        "dt 0.02 ms, CG tol 1e-7, AT + 10-lead ECG/ms; timed: steps "
+                        f"{k0}-{k0 + args.steps} ({k0 * 0.02:g}-{(k0 + args.steps) * 0.02:g} ms "
+                        "of the beat)",
             "nodes": n, "tets": mesh.n_elems, "nnz": nnz, "cg_iters_per_step": k_per_step,
             "cg_blocks": args.cg_blocks,
             "cg_x0": args.cg_x0,
@@ -687,6 +712,7 @@ def main():
             "avg_launch_us": (it_ms * 1e3) / max(1, it_launches),
             "launches_per_iteration": it_launches / max(1, prof_its),
         },
+        "phases": phases,
         "kernels_ms": {k: v[0] for k, v in prof.items()},
         "kernel_launches": {k: v[1] for k, v in prof.items()},
         "ionic_GBps": ionic_bytes / ((ionic_ms / 1e3) / max(1, ionic_launches)) / 1e9 if ionic_launches else None,

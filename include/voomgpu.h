@@ -194,8 +194,8 @@ typedef struct {
   int64_t cg_not_converged;   /* solves that hit cg_maxit */
   int64_t newton_iterations;  /* tT reference integrator only (device sum) */
   int64_t n_nodes;
-  int64_t nnz;
-  int64_t sell_padded_nnz;
+  int64_t nnz;                /* of the assembled operator, diagonal included */
+  int64_t sell_padded_nnz;    /* entries the device stores: off-diagonal, slice-padded */
   double last_cg_relres;      /* ||r||_D / ||K_rhs V||_D of the last solve */
   double sim_time;            /* ms */
 } vg_stats;
@@ -255,7 +255,9 @@ int vg_ecg_frames(vg_handle *h, double *out, int *n_frames);
 int vg_profile_enable(vg_handle *h, int enable);
 int vg_profile_get(vg_handle *h, int kernel_class, double *total_ms, int64_t *launches);
 #define VG_KC_RHS 0      /* r_0 = K_rhs V */
-#define VG_KC_CG_ITER 1  /* fused CG iteration (update + SpMV), one per launch */
+#define VG_KC_CG_ITER 1  /* fused CG iteration (update + SpMV), one per launch;
+                            timed per solve over its launches back to back
+                            (launch boundaries included), counted per launch */
 #define VG_KC_CG_START 2 /* w_0 = A r_0 */
 #define VG_KC_IONIC 3    /* ionic update, Purkinje kernel included */
 #define VG_KC_APPLY 4

@@ -48,8 +48,13 @@ struct DevBuf {
   template <class X> X *as() const { return static_cast<X *>(p); }
 };
 
+struct ProfileInterval {
+  hipEvent_t a, b;
+  int launches;
+};
+
 struct ProfileClass {
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<ProfileInterval> pending;
   double total_ms = 0.0;
   int64_t launches = 0;
 };
@@ -158,6 +163,12 @@ struct vg_handle {
   // number of its solves so far (capped at 3).
   bool extrap = false;
   int extrap_order = 3;
+  // Local solve of the Purkinje rows of a guess (k_pk_presolve): the rows'
+  // block and the myocardium rows coupled to it, built at set-up when the
+  // tree fits (pk_ok).
+  bool pk_ok = false;
+  int pk_ncpl = 0;
+  DevBuf pk_blk_v, pk_blk_j, pk_cpl_row, pk_cpl_v, pk_cpl_j;
   DevBuf xhist;
   int solves_done[2] = {0, 0};
   // Rows [0, n_myo) run the body's cell model, rows [n_myo, n) the Purkinje
@@ -231,8 +242,8 @@ struct vg_handle {
   ~vg_handle() {
     for (auto &c : prof)
       for (auto &pr : c.pending) {
-        (void)hipEventDestroy(pr.first);
-        (void)hipEventDestroy(pr.second);
+        (void)hipEventDestroy(pr.a);
+        (void)hipEventDestroy(pr.b);
       }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_res) (void)hipHostFree(h_res);
@@ -264,10 +275,12 @@ struct vg_cells {
 
 namespace {
 
-// Launch bracketing for vg_profile_*: a pair of events per launch.
+// Launch bracketing for vg_profile_*: a pair of events around one launch, or
+// around the `launches` back-to-back launches recorded in between.
 struct ProfScope {
   vg_handle *h;
   int kc;
+  int launches = 1;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(vg_handle *hh, int k) : h(hh), kc(k) {
     if (!h->profiling) return;
@@ -287,7 +300,7 @@ struct ProfScope {
   ~ProfScope() {
     if (!a) return;
     (void)hipEventRecord(b, h->stream);
-    h->prof[kc].pending.emplace_back(a, b);
+    h->prof[kc].pending.push_back(ProfileInterval{a, b, launches});
   }
 };
 
@@ -295,12 +308,12 @@ void drain_profile(vg_handle *h) {
   for (auto &c : h->prof) {
     for (auto &pr : c.pending) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      if (hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
         c.total_ms += ms;
-        c.launches += 1;
+        c.launches += pr.launches;
       }
-      h->ev_pool.push_back(pr.first);
-      h->ev_pool.push_back(pr.second);
+      h->ev_pool.push_back(pr.a);
+      h->ev_pool.push_back(pr.b);
     }
     c.pending.clear();
   }
@@ -336,17 +349,16 @@ T *xbuf(vg_handle *h, int which) {
   return (which && !h->shared_x) ? h->x2.as<T>() : h->x.as<T>();
 }
 
-// Local solve of the Purkinje rows of the guess (solves that start from a
-// guess, one body, a tree small enough for one workgroup's LDS).
+// Local solve of the Purkinje rows of the guess, between the right-hand
+// side and the start of a solve that starts from a guess.
 template <class T>
-void launch_pk_presolve(vg_handle *h, const T *V, int which) {
-  if (!h->guess || h->n_pk == 0 || h->n_inst != 1) return;
-  const int64_t cap = std::min<int64_t>((int64_t)PK_BLK * PK_ROWS, 61440 / (int64_t)sizeof(T));
-  if (h->n_pk > cap) return;
+void launch_pk_presolve(vg_handle *h, int which) {
+  if (!h->guess || !h->pk_ok) return;
   ProfScope ps(h, VG_KC_PK_PRESOLVE);
   hipLaunchKernelGGL(k_pk_presolve<T>, dim3(1), dim3(PK_BLK), (unsigned)(h->n_pk * sizeof(T)),
-                     h->stream, sell_of(h), h->Ahat.as<T>(), h->Krhs.as<T>(), V, xbuf<T>(h, which),
-                     h->n_myo, (int)h->n_pk);
+                     h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
+                     h->pk_blk_j.as<int>(), h->n_myo, (int)h->n_pk, h->pk_cpl_row.as<int64_t>(),
+                     h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(), h->pk_ncpl);
 }
 
 template <class T>
@@ -382,9 +394,9 @@ void launch_cg_start(vg_handle *h, int which) {
                      cgbufs_of(h), hist);
 }
 
+// (Profiled per solve, around all its iteration launches: see cg_solve.)
 template <class T>
 void launch_cg_iter(vg_handle *h, int j, int which) {
-  ProfScope ps(h, VG_KC_CG_ITER);
   hipLaunchKernelGGL(k_cg_iter<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
                      h->q0.as<typename Vec4Of<T>::type>(), h->q1.as<typename Vec4Of<T>::type>(),
@@ -451,11 +463,14 @@ int cg_wait(vg_handle *h, bool &all) {
 template <class T>
 int cg_solve(vg_handle *h, const T *V, int which) {
   for (int k = 0; k < h->n_inst; k++) __atomic_store_n(&h->h_pub[k].done, 0, __ATOMIC_SEQ_CST);
-  launch_pk_presolve<T>(h, V, which);
   launch_rhs<T>(h, V, which);
+  launch_pk_presolve<T>(h, which);
   launch_cg_start<T>(h, which);
   int j = 0;
   int batch = h->cg_guess[which];
+  // One event pair around all iteration launches of the solve: an event pair
+  // around each launch would add its own markers to every launch boundary.
+  ProfScope ps(h, VG_KC_CG_ITER);
   for (;;) {
     for (int k = 0; k < batch && j <= h->maxit; k++) launch_cg_iter<T>(h, j++, which);
     HIPCHK(h, hipGetLastError());
@@ -465,6 +480,7 @@ int cg_solve(vg_handle *h, const T *V, int which) {
     if (all || j > h->maxit) break;
     batch = 4;
   }
+  ps.launches = j;
   h->solves_done[which] = std::min(3, h->solves_done[which] + 1);
   return cg_account(h, which);
 }
@@ -675,8 +691,8 @@ int capture_step(vg_handle *h) {
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   const T *V[2] = {h->Va.as<T>(), h->Vb.as<T>()};
   for (int w = 0; w < 2; w++) {
-    launch_pk_presolve<T>(h, V[w], w);
     launch_rhs<T>(h, V[w], w);
+    launch_pk_presolve<T>(h, w);
     launch_cg_start<T>(h, w);
     for (int j = 0; j < h->glaunch[w]; j++) launch_cg_iter<T>(h, j, w);
     launch_fold(h, w, true, false);
@@ -771,6 +787,84 @@ int upload_raw(vg_handle *h, DevBuf &b, const std::vector<double> &v) {
   std::vector<T> buf(v.begin(), v.end());
   HIPCHK(h, b.alloc(buf.size() * sizeof(T)));
   HIPCHK(h, hipMemcpy(b.p, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VG_OK;
+}
+
+// The Purkinje rows' block of A_hat and the myocardium rows coupled to it,
+// in the entry-major form k_pk_presolve reads; pk_ok stays false when there
+// is no tree, several instances, a tree too large for one workgroup, or a
+// row with more entries than the kernel keeps.
+int setup_pk_presolve(vg_handle *h, const HostSell &sell) {
+  h->pk_ok = false;
+  const int64_t r0 = h->n_myo, npk = h->n_pk;
+  const int64_t cap = std::min<int64_t>(
+      (int64_t)PK_BLK * (h->tsz == 8 ? pk_rows<double>() : pk_rows<float>()), 61440 / (int64_t)h->tsz);
+  if (npk == 0 || npk > cap || h->n_inst != 1) return VG_OK;
+  // Stored entries of one row.
+  auto visit = [&](int64_t row, auto f) {
+    const int64_t s = row / SELL_C;
+    const int lane = (int)(row % SELL_C);
+    for (int j = 0; j < sell.width[s]; j++) {
+      const int64_t e = sell_slot(sell.offset[s], sell.width[s], j, lane);
+      f((int64_t)sell.col[e], sell.A_hat[e]);
+    }
+  };
+  auto in_tree = [&](int64_t c) { return c >= r0 && c < r0 + npk; };
+  std::vector<double> bv(PK_MAXE * npk, 0.0);
+  std::vector<int> bj(PK_MAXE * npk, 0);
+  std::vector<int64_t> myo;
+  for (int64_t i = 0; i < npk; i++) {
+    // Slot 0: the unit diagonal of A_hat (not stored in the SELL operator).
+    bv[i] = 1.0;
+    bj[i] = (int)i;
+    int m = 1;
+    visit(r0 + i, [&](int64_t c, double v) {
+      if (v == 0.0) return;
+      if (!in_tree(c)) {
+        myo.push_back(c);
+        return;
+      }
+      if (m < PK_MAXE) {
+        bv[m * npk + i] = v;
+        bj[m * npk + i] = (int)(c - r0);
+      }
+      m++;
+    });
+    if (m > PK_MAXE) return VG_OK;
+  }
+  std::sort(myo.begin(), myo.end());
+  myo.erase(std::unique(myo.begin(), myo.end()), myo.end());
+  const int64_t nc = (int64_t)myo.size();
+  std::vector<double> cv(CPL_MAXE * std::max<int64_t>(nc, 1), 0.0);
+  std::vector<int> cj(CPL_MAXE * std::max<int64_t>(nc, 1), 0);
+  for (int64_t c = 0; c < nc; c++) {
+    int m = 0;
+    visit(myo[c], [&](int64_t col, double v) {
+      if (v == 0.0 || !in_tree(col)) return;
+      if (m < CPL_MAXE) {
+        cv[m * nc + c] = v;
+        cj[m * nc + c] = (int)(col - r0);
+      }
+      m++;
+    });
+    if (m > CPL_MAXE) return VG_OK;
+  }
+  int rc;
+  if (h->tsz == 8) {
+    if ((rc = upload_raw<double>(h, h->pk_blk_v, bv))) return rc;
+    if ((rc = upload_raw<double>(h, h->pk_cpl_v, cv))) return rc;
+  } else {
+    if ((rc = upload_raw<float>(h, h->pk_blk_v, bv))) return rc;
+    if ((rc = upload_raw<float>(h, h->pk_cpl_v, cv))) return rc;
+  }
+  HIPCHK(h, h->pk_blk_j.alloc(bj.size() * 4));
+  HIPCHK(h, hipMemcpy(h->pk_blk_j.p, bj.data(), bj.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, h->pk_cpl_j.alloc(cj.size() * 4));
+  HIPCHK(h, hipMemcpy(h->pk_cpl_j.p, cj.data(), cj.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, h->pk_cpl_row.alloc(std::max<int64_t>(nc, 1) * 8));
+  if (nc) HIPCHK(h, hipMemcpy(h->pk_cpl_row.p, myo.data(), nc * 8, hipMemcpyHostToDevice));
+  h->pk_ncpl = (int)nc;
+  h->pk_ok = true;
   return VG_OK;
 }
 
@@ -1215,6 +1309,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     if ((rc = upload_raw<float>(H_, H_->sinv, sinv_pad))) return rc;
     if ((rc = upload_raw<float>(H_, H_->diag, diag_pad))) return rc;
   }
+  if ((rc = setup_pk_presolve(H_, sell))) return rc;
   // Grids: the CG block tables (set_cg_blocks) and the node-wise grid.
   {
     H_->inst_slice_host.assign(H_->n_inst + 1, 0);

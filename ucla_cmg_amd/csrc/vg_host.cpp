@@ -286,7 +286,8 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
       const int64_t r = s * SELL_C + l;
       if (r < rows && o.new2old[r] >= 0) {
         const int32_t old = o.new2old[r];
-        w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]));
+        // The diagonal is not stored (see below).
+        w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]) - 1);
       }
     }
     o.width[s] = w;
@@ -305,6 +306,10 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
     o.sinv[r] = old >= 0 ? 1.0 / std::sqrt(d[old]) : 1.0;
     o.diag[r] = old >= 0 ? d[old] : 1.0;
   }
+  // Off-diagonal entries only.  The diagonal of A_hat = D^-1/2 A D^-1/2 is 1
+  // (D = diag A), and K, K_exp have zero row sums (K 1 = 0), so the kernels
+  // add x_i for A_hat and form the stiffness products as sum_j K_ij (V_j - V_i):
+  // one stored entry and one gather fewer per row.
   std::vector<std::pair<int32_t, int64_t>> row;
   for (int64_t s = 0; s < o.n_slices; s++) {
     for (int l = 0; l < SELL_C; l++) {
@@ -313,7 +318,7 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
       if (r < rows && o.new2old[r] >= 0) {
         const int32_t old = o.new2old[r];
         for (int64_t k = c.row_ptr[old]; k < c.row_ptr[old + 1]; k++)
-          row.emplace_back(o.old2new[c.col[k]], k);
+          if (c.col[k] != old) row.emplace_back(o.old2new[c.col[k]], k);
         std::sort(row.begin(), row.end());
       }
       for (int j = 0; j < o.width[s]; j++) {

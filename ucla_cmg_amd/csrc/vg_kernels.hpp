@@ -394,7 +394,8 @@ __device__ inline void cg_publish(CgResult *pub, int iter, int notconv, int x_va
 
 // ---- explicit diffusion stride ------------------------------------------------
 // Vout = Vin + h (-C_L^-1 K) Vin, one row per lane (EP_ExplicitSolver.cc:
-// 52-85; Kexp holds -K / C_L in the SELL layout).
+// 52-85; Kexp holds the off-diagonal entries of -K / C_L in the SELL layout,
+// whose rows sum to zero, so the product is sum_j Kexp_ij (Vin_j - Vin_i)).
 template <class T>
 __global__ __launch_bounds__(BLK) void k_explicit(Sell S, const T *__restrict__ Kexp,
                                                   const T *__restrict__ Vin, T *__restrict__ Vout,
@@ -403,109 +404,149 @@ __global__ __launch_bounds__(BLK) void k_explicit(Sell S, const T *__restrict__ 
   for (int64_t sl = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); sl < n_slices;
        sl += (int64_t)gridDim.x * WAVES) {
     const int64_t row = sl * 64 + lane;
-    const T acc = sell_row<T>(S, Kexp, sl, lane, [&](int c) { return Vin[c]; });
-    Vout[row] = Vin[row] + h * acc;
+    const T vi = Vin[row];
+    const T acc = sell_row<T>(S, Kexp, sl, lane, [&](int c) { return Vin[c] - vi; });
+    Vout[row] = vi + h * acc;
   }
 }
 
 // ---- Purkinje rows of the initial guess ----------------------------------------
-// Before the right-hand side of a solve that starts from a guess, one
+// After the right-hand side of a solve that starts from a guess, one
 // workgroup improves the guess on the Purkinje rows [r0, r0 + npk) with the
-// myocardium rows held: PK_ITERS CG iterations on A_pp e = (b - A_hat x)_p,
-// A_pp the Purkinje rows' block of A_hat, then x_p += e.  The Purkinje tree's
-// fronts are sharp in time, so any guess carried from earlier solves is poor
-// there, and those ~0.4 % of the rows set the initial residual; A_pp is well
-// conditioned (Jacobi-scaled mass-dominated cable, eigenvalues ~[0.5, 1.5] on
-// the LV tree), so a few local iterations remove that part.  The solve that
-// follows stops on the same test as before: only its starting point moves.
-// The search direction lives in LDS (dynamic, npk values); each thread owns
-// rows tid + k PK_BLK.
+// myocardium rows held: PK_ITERS CG iterations on A_pp e = r0_p, A_pp the
+// Purkinje rows' block of A_hat, then x_p += e and r0 -= A_hat e (on the
+// Purkinje rows that is the local residual; on the myocardium rows coupled to
+// the tree, the PMJ targets, it is their coupling entries times e).  The
+// tree's fronts are sharp in time, so any guess carried from earlier solves
+// is poor there, and while the tree conducts those ~0.4 % of the rows set the
+// initial residual; A_pp is well conditioned (Jacobi-scaled mass-dominated
+// cable, eigenvalues ~[0.5, 1.5] on the LV tree), so a few local iterations
+// remove that part.  The solve that follows stops on the same test as
+// before: only its starting point moves.
+// The block (blk_v/blk_j: at most PK_MAXE entries per row, local column
+// indices, entry-major) and the coupling rows (cpl_*: a myocardium row, its
+// CPL_MAXE entries in Purkinje columns) are built on the host; unused slots
+// hold value 0 and index 0.  The search direction lives in LDS; each thread
+// owns rows tid + k PK_BLK.
 constexpr int PK_BLK = 1024;
-constexpr int PK_ROWS = 6;  // rows per thread: at most PK_BLK * PK_ROWS Purkinje rows
+// Rows per thread (registers hold each row's block entries): at most
+// PK_BLK * pk_rows<T>() Purkinje rows.
+template <class T>
+constexpr int pk_rows() {
+  return sizeof(T) == 8 ? 4 : 5;
+}
+constexpr int PK_MAXE = 4;  // block entries per row (a tree node: itself + 3 neighbours)
+constexpr int CPL_MAXE = 4;
 constexpr int PK_ITERS = 8;
 
-__device__ inline double pk_block_sum(double v, double *sm) {
-  v = wave_sum(v);
+// Block sums of two values; sm holds two slots per wave and alternates
+// between calls (buf), so one barrier suffices: a slot is written again only
+// two calls later, after another barrier.
+__device__ inline void pk_block_sum2(double &a, double &b, double (*sm)[PK_BLK / 64][2], int buf) {
+  double v[2] = {a, b};
+  wave_sum_n<2>(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();  // earlier readers of sm are done
-  if (lane == 0) sm[wv] = v;
+  if (lane == 0) {
+    sm[buf][wv][0] = v[0];
+    sm[buf][wv][1] = v[1];
+  }
   __syncthreads();
-  double s = 0.0;
+  double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-  for (int w = 0; w < PK_BLK / 64; w++) s += sm[w];
-  return s;
+  for (int w = 0; w < PK_BLK / 64; w++) {
+    s0 += sm[buf][w][0];
+    s1 += sm[buf][w][1];
+  }
+  a = s0;
+  b = s1;
 }
 
+// Local solve in the Chronopoulos-Gear form (one fused reduction and one
+// product per iteration): the residual is the vector other rows read, in LDS.
 template <class T>
-__global__ __launch_bounds__(PK_BLK) void k_pk_presolve(Sell S, const T *__restrict__ Ahat,
-                                                        const T *__restrict__ Krhs,
-                                                        const T *__restrict__ V, T *__restrict__ x,
-                                                        int64_t r0, int npk) {
+__global__ __launch_bounds__(PK_BLK) void k_pk_presolve(T *__restrict__ r, T *__restrict__ x,
+                                                        const T *__restrict__ blk_v,
+                                                        const int *__restrict__ blk_j, int64_t r0,
+                                                        int npk, const int64_t *__restrict__ cpl_row,
+                                                        const T *__restrict__ cpl_v,
+                                                        const int *__restrict__ cpl_j, int ncpl) {
   extern __shared__ double pk_lds[];
-  T *p = reinterpret_cast<T *>(pk_lds);
-  __shared__ double sm[PK_BLK / 64];
+  T *rl = reinterpret_cast<T *>(pk_lds);
+  __shared__ double sm[2][PK_BLK / 64][2];
   const int tid = threadIdx.x;
-  T e[PK_ROWS], rr[PK_ROWS];
-  double rz = 0.0;
+  constexpr int PK_ROWS = pk_rows<T>();
+  T e[PK_ROWS], rr[PK_ROWS], p[PK_ROWS], sv[PK_ROWS], ev[PK_ROWS][PK_MAXE];
+  int ec[PK_ROWS][PK_MAXE];
 #pragma unroll
   for (int k = 0; k < PK_ROWS; k++) {
     const int i = tid + k * PK_BLK;
-    e[k] = rr[k] = T(0);
-    if (i < npk) {
-      const int64_t row = r0 + i;
-      const T vi = V[row];
-      T bk, ax;
-      sell_row2<T>(S, Krhs, Ahat, row >> 6, (int)(row & 63), [&](int c) { return V[c] - vi; },
-                   [&](int c) { return x[c]; }, bk, ax);
-      rr[k] = bk - ax;
-      p[i] = rr[k];
-      rz += (double)rr[k] * (double)rr[k];
+    e[k] = rr[k] = p[k] = sv[k] = T(0);
+    const bool on = i < npk;
+#pragma unroll
+    for (int m = 0; m < PK_MAXE; m++) {
+      ev[k][m] = on ? blk_v[(int64_t)m * npk + i] : T(0);
+      ec[k][m] = on ? blk_j[(int64_t)m * npk + i] : 0;
+    }
+    if (on) {
+      rr[k] = r[r0 + i];
+      rl[i] = rr[k];
     }
   }
-  rz = pk_block_sum(rz, sm);  // its second barrier also publishes p
-  for (int it = 0; it < PK_ITERS && rz > 0.0; it++) {
+  __syncthreads();
+  double gamma_old = 0.0, alpha_old = 0.0;
+  for (int it = 0; it < PK_ITERS; it++) {
     T q[PK_ROWS];
-    double pq = 0.0;
+    double gamma = 0.0, delta = 0.0;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      T a = T(0);
+#pragma unroll
+      for (int m = 0; m < PK_MAXE; m++) a += ev[k][m] * rl[ec[k][m]];
+      q[k] = a;
+      gamma += (double)rr[k] * (double)rr[k];
+      delta += (double)rr[k] * (double)a;
+    }
+    pk_block_sum2(gamma, delta, sm, it & 1);  // also: every read of rl is done
+    if (!(gamma > 0.0) || !isfinite(gamma) || !isfinite(delta)) break;
+    double beta = 0.0, alpha;
+    if (it == 0) {
+      alpha = gamma / delta;
+    } else {
+      beta = gamma / gamma_old;
+      alpha = gamma / (delta - beta * gamma / alpha_old);
+    }
+    if (!(alpha > 0.0) || !isfinite(alpha)) break;
+    gamma_old = gamma;
+    alpha_old = alpha;
+    const T a = (T)alpha, b = (T)beta;
 #pragma unroll
     for (int k = 0; k < PK_ROWS; k++) {
       const int i = tid + k * PK_BLK;
-      q[k] = T(0);
-      if (i < npk) {
-        const int64_t row = r0 + i;
-        q[k] = sell_row<T>(S, Ahat, row >> 6, (int)(row & 63), [&](int c) {
-          const int64_t j = (int64_t)c - r0;
-          return (j >= 0 && j < npk) ? p[j] : T(0);
-        });
-        pq += (double)p[i] * (double)q[k];
-      }
+      p[k] = rr[k] + b * p[k];
+      sv[k] = q[k] + b * sv[k];
+      e[k] += a * p[k];
+      rr[k] -= a * sv[k];
+      if (i < npk) rl[i] = rr[k];
     }
-    pq = pk_block_sum(pq, sm);  // every read of p in this iteration is done
-    if (!(pq > 0.0) || !isfinite(pq)) break;
-    const T a = (T)(rz / pq);
-    double rz2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < PK_ROWS; k++) {
-      const int i = tid + k * PK_BLK;
-      if (i < npk) {
-        e[k] += a * p[i];
-        rr[k] -= a * q[k];
-        rz2 += (double)rr[k] * (double)rr[k];
-      }
-    }
-    rz2 = pk_block_sum(rz2, sm);
-    const T b = (T)(rz2 / rz);
-    rz = rz2;
-#pragma unroll
-    for (int k = 0; k < PK_ROWS; k++) {
-      const int i = tid + k * PK_BLK;
-      if (i < npk) p[i] = rr[k] + b * p[i];
-    }
-    __syncthreads();  // the new p before the next product
+    __syncthreads();  // the new residual before the next product
   }
+  // Every read of rl is behind a barrier (the one ending an iteration, or
+  // the reduction's before a break), so rl can take e for the coupling rows.
 #pragma unroll
   for (int k = 0; k < PK_ROWS; k++) {
     const int i = tid + k * PK_BLK;
-    if (i < npk) x[r0 + i] += e[k];
+    if (i < npk) {
+      x[r0 + i] += e[k];
+      r[r0 + i] = rr[k];
+      rl[i] = e[k];
+    }
+  }
+  __syncthreads();
+  for (int c = tid; c < ncpl; c += PK_BLK) {
+    T a = T(0);
+#pragma unroll
+    for (int m = 0; m < CPL_MAXE; m++) a += cpl_v[(int64_t)m * ncpl + c] * rl[cpl_j[(int64_t)m * ncpl + c]];
+    r[cpl_row[c]] -= a;
   }
 }
 
@@ -546,9 +587,10 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
     const T vi = V[row];
     if constexpr (GUESS) {
       T bk, ax;
+      const T xi = x[row];
       sell_row2<T>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c] - vi; },
                    [&](int c) { return x[c]; }, bk, ax);
-      const T rr = bk - ax;
+      const T rr = bk - (xi + ax);  // A_hat's unit diagonal
       r[row] = rr;
       v[0] += d * (double)bk * (double)bk;
       v[1] += d * (double)rr * (double)rr;
@@ -621,9 +663,9 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
-    const T acc = sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
     const int64_t row = sl * 64 + lane;
     const T ri = r[row];
+    const T acc = ri + sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
     q[row] = {ri, acc, T(0), T(0)};
     const double rr = (double)ri;
     v[0] += rr * rr;
@@ -731,6 +773,7 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
       acc = cg_iter_row<T, false>(S, val, q0, sl, lane, a, bt);
     }
     const T rn = ri - a * si;
+    acc += rn;  // A_hat's unit diagonal
     q1[row] = {rn, acc, si, pi};
     x[row] = xi;
     const double rd = (double)rn;
