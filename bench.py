@@ -245,7 +245,8 @@ def mahajan_bench(args, vg):
     sigma = 1.0 / 1400.0
     s = vg.CardiacSolver(mesh, cell_model=vg.MODEL_MJ, precision=vg.PREC_MIXED,
                          integrator=vg.INTEG_REFERENCE, D=(sigma, sigma, sigma), dt=dt,
-                         dt_ionic=dt, cg_tol=1e-7, cg_x0=vg.X0_LAST, graph=1)
+                         dt_ionic=dt, cg_tol=1e-7, cg_x0=vg.X0_LAST, graph=args.mj_graph,
+                         row_order=args.row_order)
     setup_s = time.perf_counter() - t0
     ring = meshgen.electrodes_around((3.0, 3.0, 0.5), 5.0, 8)
     el = np.vstack([ring, [[3.0, 3.0, 3.0], [3.0, 3.0, -2.0]]])
@@ -297,7 +298,9 @@ def mahajan_bench(args, vg):
                                f"dt 0.01 ms, {args.mj_ms:g} ms",
                    "nodes": n, "tets": mesh.n_elems, "nnz": st["nnz"]},
         "cg_iters_per_step": st["cg_iterations"] / max(1, st["steps"]),
-        "cg_not_converged": st["cg_not_converged"],
+        "cg_not_converged": st["cg_not_converged"], "graph": bool(args.mj_graph),
+        "row_order": {1: "mesh", 2: "geometric"}.get(st["row_order"], st["row_order"]),
+        "sell_wide_slices": st["sell_wide_slices"],
         "activated_frac": float((at >= 0).mean()),
         "phase_singularities": {"series_ms_count_charge": ps[::4],
                                 "max": max((p[1] for p in ps), default=0),
@@ -515,6 +518,11 @@ def main():
                     help="the 4.6M-node Mahajan S1-S2 slab instead (phase singularities, ECG)")
     ap.add_argument("--mj-ms", type=float, default=1000.0)
     ap.add_argument("--mj-seed", type=int, default=7)
+    ap.add_argument("--row-order", type=int, default=0, choices=[0, 1, 2],
+                    help="myocardium row order: 0 auto (mesh order unless a column is more than "
+                         "32767 rows away), 1 mesh, 2 geometric")
+    ap.add_argument("--mj-graph", type=int, default=0, choices=[0, 1],
+                    help="Mahajan slab: 1 = captured step (hipGraph), 0 = host-driven CG")
     ap.add_argument("--surrogate", action="store_true",
                     help="evaluate the learned ionic step on the LV instead (collect, train, compare)")
     ap.add_argument("--surr-ms", type=float, default=400.0)
@@ -568,7 +576,7 @@ def main():
                               cg_x0={"zero": vg.X0_ZERO, "previous": vg.X0_PREVIOUS,
                                      "last": vg.X0_LAST,
                                      "extrapolate": vg.X0_EXTRAPOLATE}[args.cg_x0],
-                              graph=args.graph,
+                              graph=args.graph, row_order=args.row_order,
                               **MODEL_KW)
     solver.enableActivation(-40.0)
     solver.setupECG(electrodes10())
@@ -697,6 +705,7 @@ def main():
             "cg_x0": args.cg_x0,
             "cg_blocks_used": solver.cgBlocks(),
             "graph": tuned["graph"] if tuned else bool(args.graph),
+            "row_order": {1: "mesh", 2: "geometric"}.get(solver.stats()["row_order"]),
             "parallelism": "single GPU",
         },
         "roofline": {

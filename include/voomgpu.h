@@ -169,7 +169,21 @@ typedef struct {
    * (CardiacBody.cc:819-902). */
   int scheme;
   double dt_diffusion; /* explicit stride (ms); 0 = 0.1, Solve's default argument */
+  /* Row order of the myocardium rows (VG_ORDER_*).  No reference
+   * counterpart: Epetra keeps the partition's node order.  Only the storage
+   * order changes; every array in and out of the library stays in mesh
+   * order. */
+  int row_order;
 } vg_model;
+
+#define VG_ORDER_AUTO 0      /* mesh order, unless some operator row has a column
+                                more than 32767 rows away (a slice on 32-bit
+                                columns) and the geometric order has fewer such
+                                rows */
+#define VG_ORDER_MESH 1      /* always the mesh order */
+#define VG_ORDER_GEOMETRIC 2 /* always the geometric order: nodes sorted on a
+                                grid of half the median shortest edge, the
+                                longest extent slowest, the shortest fastest */
 
 #define VG_SCHEME_CN 0
 #define VG_SCHEME_BE 1
@@ -198,6 +212,8 @@ typedef struct {
   int64_t sell_padded_nnz;    /* entries the device stores: off-diagonal, slice-padded */
   double last_cg_relres;      /* ||r||_D / ||K_rhs V||_D of the last solve */
   double sim_time;            /* ms */
+  int64_t sell_wide_slices;   /* slices stored with 32-bit columns */
+  int32_t row_order;          /* VG_ORDER_MESH or VG_ORDER_GEOMETRIC: the order used */
 } vg_stats;
 
 const char *vg_version(void);

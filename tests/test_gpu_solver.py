@@ -184,10 +184,54 @@ def test_far_columns_int32_slices(gpu_device):
     states, V0 = gpu0.getStateVariables(), gpu0.getVoltage()
     gpu0.close()
     gpu, ref = _run_pair(mesh, vg.MODEL_LR, 60, 0.01, stim, 50.0 * oracle.lr_xi(), 60,
-                         dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10),
+                         dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10,
+                              row_order=vg.ORDER_MESH),
                          dict(cg_tol=1e-10), V0=V0, states=states)
     err = np.abs(gpu.getVoltage() - ref.V).max()
     moved = np.abs(ref.V - V0).max()
-    print(f"renumbered slab: max|dV| {err:.3e} mV (V moved up to {moved:.1f} mV)")
+    st = gpu.stats()
+    print(f"renumbered slab: max|dV| {err:.3e} mV (V moved up to {moved:.1f} mV), "
+          f"{st['sell_wide_slices']} int32 slices")
+    assert st["row_order"] == vg.ORDER_MESH and st["sell_wide_slices"] > 0
     assert moved > 1.0
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("order", [vg.ORDER_AUTO, vg.ORDER_GEOMETRIC])
+def test_geometric_row_order_of_a_renumbered_slab(gpu_device, order):
+    """The same randomly renumbered slab with the library's row order: the
+    rows are stored in grid order (no int32 slice left) while every array in
+    and out stays in mesh order.  FP64 step against the oracle on the mesh's
+    numbering, and the mixed-precision run against the FP64 one."""
+    nx, ny, nz = 41, 41, 21
+    m = meshgen.slab(nx, ny, nz, 0.025)
+    n = m.n_nodes
+    perm = np.random.default_rng(3).permutation(n)
+    inv = np.argsort(perm)
+    conn = m.conn.copy()
+    conn[conn >= 0] = perm[conn[conn >= 0]]
+    mesh = meshgen.Mesh(xyz=m.xyz[inv], elem_nn=m.elem_nn, conn=conn, fibers=m.fibers)
+    stim = perm[meshgen.slab_corner_nodes(nx, ny, nz, 3)].astype(np.int32)
+    gpu0 = vg.CardiacSolver(mesh, cell_model=vg.MODEL_LR, dt=0.01, D=D_SLAB)
+    states, V0 = gpu0.getStateVariables(), gpu0.getVoltage()
+    gpu0.close()
+    gpu, ref = _run_pair(mesh, vg.MODEL_LR, 60, 0.01, stim, 50.0 * oracle.lr_xi(), 60,
+                         dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_tol=1e-10,
+                              row_order=order),
+                         dict(cg_tol=1e-10), V0=V0, states=states)
+    st = gpu.stats()
+    err = np.abs(gpu.getVoltage() - ref.V).max()
+    # Slot 7 of the LR state vector holds the voltage of the ionic update, not
+    # the step's final V; the other seven are the cell's own states.
+    keep = [k for k in range(ref.states.shape[1]) if k != 7]
+    serr = np.abs(gpu.getStateVariables()[:, keep] - ref.states[:, keep]).max()
+    print(f"geometric order ({order}): max|dV| {err:.3e} mV, max|d state| {serr:.3e}, "
+          f"{st['sell_wide_slices']} int32 slices, "
+          f"padded nnz {st['sell_padded_nnz']}")
+    assert st["row_order"] == vg.ORDER_GEOMETRIC
+    assert st["sell_wide_slices"] == 0
+    assert err < 1e-8 and serr < 1e-8
+    mixed, _ = _run_pair(mesh, vg.MODEL_LR, 60, 0.01, stim, 50.0 * oracle.lr_xi(), 60,
+                         dict(precision=vg.PREC_MIXED, cg_tol=1e-10, row_order=order),
+                         dict(cg_tol=1e-10), V0=V0, states=states)
+    assert np.abs(mixed.getVoltage() - gpu.getVoltage()).max() < 0.05

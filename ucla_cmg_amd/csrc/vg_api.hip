@@ -226,6 +226,8 @@ struct vg_handle {
   int ecg_every = 0, ecg_max = 0, ecg_count = 0;
   // stats
   int64_t steps = 0, cg_iters = 0, cg_solves = 0, cg_notconv = 0;
+  int64_t n_wide = 0;             // slices on 32-bit columns
+  int row_order = VG_ORDER_MESH;  // the myocardium rows' order (VG_ORDER_*)
   double last_relres = 0.0, t = 0.0;
   bool profiling = false;
   ProfileClass prof[VG_KC_COUNT];
@@ -1127,6 +1129,11 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
     *out = h.release();
     return (*out)->fail(VG_ERR_ARG, "unknown cg_x0");
   }
+  if (model->row_order != VG_ORDER_AUTO && model->row_order != VG_ORDER_MESH &&
+      model->row_order != VG_ORDER_GEOMETRIC) {
+    *out = h.release();
+    return (*out)->fail(VG_ERR_ARG, "unknown row_order");
+  }
   if (!(model->dt > 0) || !(model->dt_ionic > 0) || !(model->Cm > 0) || !(model->cg_tol > 0) ||
       model->cg_maxit < 1) {
     *out = h.release();
@@ -1231,6 +1238,20 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   } else {
     for (int64_t i = 0; i < csr.n; i++)
       if (!h->purkinje[i]) base.push_back((int32_t)i);
+    // A far column puts its whole slice on 32-bit columns and its gathers
+    // outside the L2 the neighbouring rows are in; the geometric order keeps
+    // the band to about one layer of the mesh.
+    if (model->row_order != VG_ORDER_MESH) {
+      const int64_t far_mesh = model->row_order == VG_ORDER_AUTO ? far_rows(csr, base) : 1;
+      if (far_mesh > 0) {
+        std::vector<int32_t> geo = base;
+        geometric_order(csr, mesh->xyz, geo);
+        if (model->row_order == VG_ORDER_GEOMETRIC || far_rows(csr, geo) < far_mesh) {
+          base.swap(geo);
+          h->row_order = VG_ORDER_GEOMETRIC;
+        }
+      }
+    }
     h->n_myo = (int64_t)base.size();
     for (int64_t i = 0; i < csr.n; i++)
       if (h->purkinje[i]) base.push_back((int32_t)i);
@@ -1256,6 +1277,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->nnz = (int64_t)csr.col.size();
   h->n_slices = sell.n_slices;
   h->padded = sell.padded;
+  h->n_wide = std::count_if(sell.wide.begin(), sell.wide.end(), [](int64_t w) { return w >= 0; });
   h->n_alloc = ((h->rows + BLK - 1) / BLK) * BLK;
   h->new2old = sell.new2old;
   h->old2new = sell.old2new;
@@ -1820,6 +1842,8 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   o->sell_padded_nnz = h->padded;
   o->last_cg_relres = h->last_relres;
   o->sim_time = h->t;
+  o->sell_wide_slices = h->n_wide;
+  o->row_order = h->row_order;
   return VG_OK;
 }
 

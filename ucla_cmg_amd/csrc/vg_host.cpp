@@ -434,4 +434,77 @@ std::string build_ecg_matrix(const vg_mesh &m, unsigned quirks, const double *el
   return "";
 }
 
+int64_t far_rows(const HostCsr &c, const std::vector<int32_t> &order) {
+  std::vector<int64_t> pos(c.n, -1);
+  for (size_t k = 0; k < order.size(); k++) pos[order[k]] = (int64_t)k;
+  int64_t far = 0;
+  for (size_t k = 0; k < order.size(); k++) {
+    const int32_t i = order[k];
+    for (int64_t e = c.row_ptr[i]; e < c.row_ptr[i + 1]; e++) {
+      const int64_t q = pos[c.col[e]];
+      if (q >= 0 && (q - (int64_t)k > 32767 || (int64_t)k - q > 32767)) {
+        far++;
+        break;
+      }
+    }
+  }
+  return far;
+}
+
+void geometric_order(const HostCsr &c, const double *xyz, std::vector<int32_t> &order) {
+  if (order.size() < 2) return;
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; a++) lo[a] = hi[a] = xyz[3 * (int64_t)order[0] + a];
+  for (int32_t i : order)
+    for (int a = 0; a < 3; a++) {
+      lo[a] = std::min(lo[a], xyz[3 * (int64_t)i + a]);
+      hi[a] = std::max(hi[a], xyz[3 * (int64_t)i + a]);
+    }
+  // Grid step: half the median over nodes of the shortest edge at the node
+  // (on a structured grid of spacing h every node then has its own cell).
+  std::vector<double> shortest;
+  shortest.reserve(order.size());
+  for (int32_t i : order) {
+    double m = HUGE_VAL;
+    for (int64_t e = c.row_ptr[i]; e < c.row_ptr[i + 1]; e++) {
+      const int32_t j = c.col[e];
+      if (j == i) continue;
+      double d2 = 0.0;
+      for (int a = 0; a < 3; a++) {
+        const double d = xyz[3 * (int64_t)i + a] - xyz[3 * (int64_t)j + a];
+        d2 += d * d;
+      }
+      if (d2 > 0.0) m = std::min(m, d2);
+    }
+    if (m < HUGE_VAL) shortest.push_back(std::sqrt(m));
+  }
+  if (shortest.empty()) return;
+  std::nth_element(shortest.begin(), shortest.begin() + shortest.size() / 2, shortest.end());
+  const double step = 0.5 * shortest[shortest.size() / 2];
+  if (!(step > 0.0) || !std::isfinite(step)) return;
+  // Axes by extent: the largest slowest, the smallest fastest (ties keep
+  // x before y before z), so the band is about one layer across the two
+  // smaller extents.
+  int ax[3] = {0, 1, 2};
+  std::stable_sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+  struct Key {
+    int64_t q[3];
+    int32_t id;
+  };
+  std::vector<Key> keys(order.size());
+  for (size_t k = 0; k < order.size(); k++) {
+    const int32_t i = order[k];
+    for (int a = 0; a < 3; a++)
+      keys[k].q[a] = std::llround((xyz[3 * (int64_t)i + ax[a]] - lo[ax[a]]) / step);
+    keys[k].id = i;
+  }
+  std::sort(keys.begin(), keys.end(), [](const Key &x, const Key &y) {
+    if (x.q[0] != y.q[0]) return x.q[0] < y.q[0];
+    if (x.q[1] != y.q[1]) return x.q[1] < y.q[1];
+    if (x.q[2] != y.q[2]) return x.q[2] < y.q[2];
+    return x.id < y.id;
+  });
+  for (size_t k = 0; k < order.size(); k++) order[k] = keys[k].id;
+}
+
 }  // namespace vg

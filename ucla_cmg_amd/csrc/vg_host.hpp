@@ -66,6 +66,19 @@ struct HostSell {
 std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool lumped, int sigma,
                        const std::vector<int32_t> &base, int64_t n_first, HostSell &out);
 
+// Rows of `order` (the mesh nodes of one row segment, in row order) with a
+// column of the same segment more than 32767 rows away: each puts its slice
+// on 32-bit columns.
+int64_t far_rows(const HostCsr &c, const std::vector<int32_t> &order);
+
+// Reorders one row segment for a narrow operator band: nodes sorted
+// lexicographically on a grid of half the median shortest edge, the axis of
+// largest extent slowest and of smallest extent fastest.  A structured mesh
+// comes out in grid order with its thinnest direction fastest; an unordered
+// (e.g. TetGen) mesh gets neighbours within about one layer of the two
+// smaller extents.  Ties (nodes of one cell) go by node id.
+void geometric_order(const HostCsr &c, const double *xyz, std::vector<int32_t> &order);
+
 // Lead-field matrix of the pseudo-ECG: ecg_e = sum_n H[e][n] V_n with
 // H[e][n] = sum over quadrature points of w (dN_n/dx)^T D grad(1/r_e).
 std::string build_ecg_matrix(const vg_mesh &mesh, unsigned quirks, const double *electrodes,

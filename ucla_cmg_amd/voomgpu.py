@@ -41,6 +41,7 @@ QUIRK_MJ_INT_ABS = 16
 TT_NSCALE = 5
 X0_ZERO, X0_PREVIOUS, X0_LAST, X0_EXTRAPOLATE, X0_EXTRAPOLATE_LINEAR = 0, 1, 2, 3, 4
 SCHEME_CN, SCHEME_BE, SCHEME_EXPLICIT = 0, 1, 2
+ORDER_AUTO, ORDER_MESH, ORDER_GEOMETRIC = 0, 1, 2
 KC_RHS, KC_CG_ITER, KC_CG_START, KC_IONIC, KC_APPLY, KC_PK_PRESOLVE = range(6)
 KC_NAMES = ["rhs", "cg_iter", "cg_start", "ionic", "apply", "pk_presolve"]
 
@@ -91,7 +92,7 @@ class VgModel(C.Structure):
         ("param_index", C.c_void_p), ("sell_sigma", C.c_int), ("cg_blocks", C.c_int),
         ("cg_x0", C.c_int), ("n_instances", C.c_int), ("instance", C.c_void_p),
         ("graph", C.c_int), ("smoothing", C.c_double), ("scheme", C.c_int),
-        ("dt_diffusion", C.c_double),
+        ("dt_diffusion", C.c_double), ("row_order", C.c_int),
     ]
 
 
@@ -101,6 +102,7 @@ class VgStats(C.Structure):
         ("cg_not_converged", C.c_int64), ("newton_iterations", C.c_int64),
         ("n_nodes", C.c_int64), ("nnz", C.c_int64), ("sell_padded_nnz", C.c_int64),
         ("last_cg_relres", C.c_double), ("sim_time", C.c_double),
+        ("sell_wide_slices", C.c_int64), ("row_order", C.c_int32),
     ]
 
 
@@ -241,7 +243,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
                param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO, instance=None,
-               graph=0, smoothing=0.0, scheme=0, dt_diffusion=0.0):
+               graph=0, smoothing=0.0, scheme=0, dt_diffusion=0.0, row_order=0):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -273,6 +275,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
     m.smoothing = smoothing
     m.scheme = scheme
     m.dt_diffusion = dt_diffusion
+    m.row_order = row_order
     if instance is not None:
         ins = np.ascontiguousarray(instance, np.int32)
         keep.append(ins)
