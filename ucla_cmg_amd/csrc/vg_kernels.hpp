@@ -178,28 +178,41 @@ __device__ inline void sum_partials(const double *__restrict__ part, int nblk, i
   }
 }
 
-// Instance of this block, its block range [b0, b1) and its slices [lo, hi):
-// the instance's slices cut into b1 - b0 contiguous chunks.  With a single
-// instance over a multiple of XCDS blocks the chunks of the blocks one XCD
-// runs are adjacent, so the rows an XCD gathers from mostly sit in its own L2.
-__device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int &b0, int &b1) {
+// Instance of this block, its block range [b0, b1) and its slices: wave w
+// takes lo + w, lo + w + step, ... below hi.  A single instance over a
+// multiple of XCDS blocks is cut into one contiguous range per XCD
+// (workgroups go round-robin over the XCDs) whose slices the XCD's blocks
+// take in turn, WAVES at a time: the waves an XCD runs at any moment work on
+// one window of consecutive slices, so the rows they gather from (the
+// operator's band around that window) were loaded into that XCD's L2 moments
+// before.  Otherwise (instances, other grids) each block takes a contiguous
+// chunk of its instance's slices.
+__device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int64_t &step,
+                                 int &b0, int &b1) {
   if (cb.n_inst == 1) {  // from kernel arguments alone: no load before the first stream
     const int b = blockIdx.x, g = gridDim.x;
-    const int l = g % XCDS == 0 ? (b % XCDS) * (g / XCDS) + b / XCDS : b;
     b0 = 0;
     b1 = g;
-    lo = cb.n_slices * l / g;
-    hi = cb.n_slices * (l + 1) / g;
+    if (g % XCDS == 0) {
+      const int x = b % XCDS;
+      lo = cb.n_slices * x / XCDS + (int64_t)(b / XCDS) * WAVES;
+      hi = cb.n_slices * (x + 1) / XCDS;
+      step = (int64_t)(g / XCDS) * WAVES;
+    } else {
+      lo = cb.n_slices * b / g;
+      hi = cb.n_slices * (b + 1) / g;
+      step = WAVES;
+    }
     return 0;
   }
   const int b = blockIdx.x, k = cb.blk_inst[b];
   b0 = cb.inst_blk[k];
   b1 = cb.inst_blk[k + 1];
   const int64_t g = b1 - b0, s0 = cb.inst_slice[k], ns = cb.inst_slice[k + 1] - s0;
-  int64_t l = b - b0;
-  if (cb.n_inst == 1 && g % XCDS == 0) l = (l % XCDS) * (g / XCDS) + l / XCDS;
+  const int64_t l = b - b0;
   lo = s0 + ns * l / g;
   hi = s0 + ns * (l + 1) / g;
+  step = WAVES;
   return k;
 }
 
@@ -567,9 +580,9 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
                                              const T *__restrict__ V, const T *__restrict__ x,
                                              T *__restrict__ r, const T *__restrict__ diag,
                                              CgBufs cb) {
-  int64_t lo, hi;
+  int64_t lo, hi, step;
   int b0, b1;
-  const int inst = block_work(cb, lo, hi, b0, b1);
+  const int inst = block_work(cb, lo, hi, step, b0, b1);
   if (blockIdx.x == b0 && threadIdx.x == 0) {
     CgResult *res = cb.res + inst;
     res->iter = 0;
@@ -581,7 +594,7 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
   }
   const int lane = threadIdx.x & 63;
   double v[2] = {0.0, 0.0};
-  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step) {
     const int64_t row = sl * 64 + lane;
     const double d = (double)diag[row];
     const T vi = V[row];
@@ -639,12 +652,12 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
                                                   typename Vec4Of<T>::type *__restrict__ q,
                                                   const T *__restrict__ diag, CgBufs cb,
                                                   XHist<T> hist) {
-  int64_t lo, hi;
+  int64_t lo, hi, step;
   int b0, b1;
-  const int inst = block_work(cb, lo, hi, b0, b1);
+  const int inst = block_work(cb, lo, hi, step, b0, b1);
   CgResult *res = cb.res + inst;
   if (hist.order > 0)
-    for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES)
+    for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step)
       rotate_history(hist, sl * 64 + (threadIdx.x & 63));
   double t0[2];
   sum_partials<2>(cb.part1, gridDim.x, b0, b1, t0);
@@ -662,7 +675,7 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
   }
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step) {
     const int64_t row = sl * 64 + lane;
     const T ri = r[row];
     const T acc = ri + sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
@@ -709,9 +722,9 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
                                                  CgBufs cb, int j, double tol2, int maxit,
                                                  int guess) {
   using V4 = typename Vec4Of<T>::type;
-  int64_t lo, hi;
+  int64_t lo, hi, step;
   int b0, b1;
-  const int inst = block_work(cb, lo, hi, b0, b1);
+  const int inst = block_work(cb, lo, hi, step, b0, b1);
   CgResult *res = cb.res + inst;
   // The stop flag, the carried scalars and the partials are loaded together;
   // a launch queued past the end of its solve reads partials nobody wrote
@@ -756,7 +769,7 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
   const T a = (T)alpha, bt = (T)beta;
   const int lane = threadIdx.x & 63;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += WAVES) {
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step) {
     const int64_t row = sl * 64 + lane;
     const V4 own = q0[row];
     const T ri = own.x, wi = own.y;
