@@ -92,9 +92,10 @@ MODEL_KW = dict(D=(1.2 / 1400.0, 0.2 / 1400.0, 0.2 / 1400.0), dt=0.02, cg_tol=1e
 
 # Operator and CG-vector layout of this build (SELL-64 with entries in groups
 # of four per lane, off-diagonal entries only; CG rows as {r, w, s, p}
-# records): a traffic measurement taken on another layout does not describe
-# this kernel.
-LAYOUT = "sell64-groups4-offdiag/cg-records4"
+# records; each XCD's blocks taking its slice range in turn): a traffic
+# measurement taken on another layout or work split does not describe this
+# kernel.
+LAYOUT = "sell64-groups4-offdiag/cg-records4/xcd-window"
 
 
 def measured_traffic(nnz, stored):
@@ -246,7 +247,7 @@ def mahajan_bench(args, vg):
     s = vg.CardiacSolver(mesh, cell_model=vg.MODEL_MJ, precision=vg.PREC_MIXED,
                          integrator=vg.INTEG_REFERENCE, D=(sigma, sigma, sigma), dt=dt,
                          dt_ionic=dt, cg_tol=1e-7, cg_x0=vg.X0_LAST, graph=args.mj_graph,
-                         row_order=args.row_order)
+                         row_order=args.row_order, cg_blocks=args.cg_blocks)
     setup_s = time.perf_counter() - t0
     ring = meshgen.electrodes_around((3.0, 3.0, 0.5), 5.0, 8)
     el = np.vstack([ring, [[3.0, 3.0, 3.0], [3.0, 3.0, -2.0]]])
@@ -300,7 +301,7 @@ def mahajan_bench(args, vg):
         "cg_iters_per_step": st["cg_iterations"] / max(1, st["steps"]),
         "cg_not_converged": st["cg_not_converged"], "graph": bool(args.mj_graph),
         "row_order": {1: "mesh", 2: "geometric"}.get(st["row_order"], st["row_order"]),
-        "sell_wide_slices": st["sell_wide_slices"],
+        "sell_wide_slices": st["sell_wide_slices"], "cg_blocks_used": s.cgBlocks(),
         "activated_frac": float((at >= 0).mean()),
         "phase_singularities": {"series_ms_count_charge": ps[::4],
                                 "max": max((p[1] for p in ps), default=0),
