@@ -450,7 +450,7 @@ constexpr int pk_rows() {
 }
 constexpr int PK_MAXE = 4;  // block entries per row (a tree node: itself + 3 neighbours)
 constexpr int CPL_MAXE = 4;
-constexpr int PK_ITERS = 8;
+constexpr int PK_ITERS = 6;  // LV beat: 4 0.915, 6 0.918, 8 0.930, 12 0.950 ms/step
 
 // Block sums of two values; sm holds two slots per wave and alternates
 // between calls (buf), so one barrier suffices: a slot is written again only
