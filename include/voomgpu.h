@@ -152,9 +152,11 @@ typedef struct {
   /* 1: after a few host-driven steps, capture the whole step (both solves
    * with a fixed number of CG launches each, learned from those steps with a
    * margin; launches past convergence return at once) in a hipGraph and
-   * replay it with no host synchronisation.  A solve that would need more
-   * launches is counted in vg_stats.cg_not_converged and the step is
-   * re-captured with more. */
+   * replay it with no host synchronisation per step.  A solve still running
+   * when its captured launches run out passes its last iterate on; it is
+   * counted in vg_stats.cg_truncated (and cg_not_converged), and the host,
+   * which reads that count after every replay without synchronising,
+   * re-captures with twice the launches before the next step. */
   int graph;
   /* Diffusive smoothing factor alpha (EPSimulator's SMOOTHING keyword,
    * EP_CrankNicholson.cc:31-36): A and B get +-alpha (4/dt) K on top of
@@ -174,6 +176,14 @@ typedef struct {
    * order changes; every array in and out of the library stays in mesh
    * order. */
   int row_order;
+  /* Every this many steps (0 = never), recompute after each of the step's
+   * two solves the true residual ||K_rhs V - A_hat x||_D / ||K_rhs V||_D in
+   * FP64 arithmetic from the stored operator, and keep the largest in
+   * vg_stats.max_true_relres.  The stop test uses the CG recurrence
+   * residual; in mixed precision the two drift apart.  Counterpart of
+   * AztecOO's TrueResidual() the reference prints after each solve
+   * (EP_CrankNicholson.cc:104-109).  Host-driven steps only. */
+  int true_residual_every;
 } vg_model;
 
 #define VG_ORDER_AUTO 0      /* mesh order, unless some operator row has a column
@@ -214,6 +224,14 @@ typedef struct {
   double sim_time;            /* ms */
   int64_t sell_wide_slices;   /* slices stored with 32-bit columns */
   int32_t row_order;          /* VG_ORDER_MESH or VG_ORDER_GEOMETRIC: the order used */
+  int64_t cg_truncated;       /* captured solves still running when the captured
+                                 launches ran out: their last iterate was used and
+                                 the step re-captured with more (also counted in
+                                 cg_not_converged) */
+  double max_true_relres;     /* largest FP64 true residual of the checked solves
+                                 (vg_model.true_residual_every) */
+  double last_true_relres;    /* the last check's */
+  int64_t true_residual_checks; /* steps checked */
 } vg_stats;
 
 const char *vg_version(void);
@@ -235,6 +253,8 @@ int vg_clear_stimulus(vg_handle *h);
 
 int vg_step(vg_handle *h, int nsteps);
 int vg_synchronize(vg_handle *h);
+/* Changes vg_model.true_residual_every between steps. */
+int vg_set_true_residual_check(vg_handle *h, int every);
 
 /* Launch shape, changeable between steps (an autotuner times candidates):
  * the total CG grid (rounded up to a multiple of 8; 0 = default) and the

@@ -15,8 +15,8 @@
  *   or_assemble        closed-form P1 K_e = V G D G^T, M_e = V/20 (1+d_ab),
  *                      C_L = V/4, K*1 = 0, symmetry, sum(M) = Cm*Vol, bar
  *                      closed forms (tests/test_oracle_fe.py)
- *   or_pcg             dense numpy/scipy solve (tests/test_oracle_cg.py)
- *   or_cn_step         CN eigen-mode decay factor (tests/test_oracle_cn.py)
+ *   or_pcg             dense numpy/scipy solve (tests/test_oracle_solver.py)
+ *   or_cn_step         CN eigen-mode decay factor (tests/test_oracle_solver.py)
  *   or_tt_rates        charge conservation across dV/dt and the six
  *                      concentration equations; near-stationarity at the
  *                      reference's initial state; constants and initial

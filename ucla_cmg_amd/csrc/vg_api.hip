@@ -183,12 +183,20 @@ struct vg_handle {
   // scalars and the result record (pinned copy for the host).
   unsigned cg_blocks = 0, node_blocks = 0;
   DevBuf part, carry, res, newton;
+  // FP64 true residual after every tres_every-th step's solves (0 = off):
+  // tres = {largest so far, checks, last}.
+  int tres_every = 0;
+  DevBuf tres;
   DevBuf blk_inst, inst_blk, inst_slice, slice_inst;  // instance tables (CgBufs)
   std::vector<int64_t> inst_slice_host;               // [n_inst + 1]
   DevBuf H, ecg_part, ecg_out, ecg_frames;
   CgResult *h_res = nullptr;  // pinned
   CgResult *h_pub = nullptr;  // coherent host memory the finishing CG launch writes
   CgResult *d_pub = nullptr;  // its device address
+  // Captured solves cut short so far, published by k_fold to coherent host
+  // memory (h_trunc, device address d_trunc); replays do not publish to h_pub.
+  unsigned long long *h_trunc = nullptr, *d_trunc = nullptr;
+  bool capturing = false;
   int nsets = 1;
   int cg_guess[2] = {8, 8};  // launches expected per solve, by half-step
   int seen_max[2] = {0, 0};  // most iterations seen per half-step
@@ -201,7 +209,7 @@ struct vg_handle {
   hipGraphExec_t gexec = nullptr;
   int glaunch[2] = {0, 0};
   bool g_stim = false, g_at = false;
-  unsigned long long g_notconv_seen = 0;
+  unsigned long long g_notconv_seen = 0, g_trunc_seen = 0;
   bool mj_fail = false;  // Mahajan with MAHAJAN_FAIL's constants
   DevBuf lr_table;       // LUORUDYWITHGATE's table in T, or empty
   // Step scheme; the explicit scheme's diffusion strides of a half-step.
@@ -250,6 +258,7 @@ struct vg_handle {
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_res) (void)hipHostFree(h_res);
     if (h_pub) (void)hipHostFree(h_pub);
+    if (h_trunc) (void)hipHostFree(h_trunc);
     if (h_tot) (void)hipHostFree(h_tot);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -337,7 +346,7 @@ CgBufs cgbufs_of(vg_handle *h) {
                 h->inst_slice.as<int64_t>(),
                 h->n_inst,
                 h->n_slices,
-                h->d_pub};
+                h->capturing ? nullptr : h->d_pub};
 }
 
 // Slice -> instance table for the node-wise kernels (none with one instance).
@@ -404,6 +413,18 @@ void launch_cg_iter(vg_handle *h, int j, int which) {
                      h->q0.as<typename Vec4Of<T>::type>(), h->q1.as<typename Vec4Of<T>::type>(),
                      xbuf<T>(h, which), h->diag.as<T>(),
                      cgbufs_of(h), j, h->tol * h->tol, h->maxit, h->guess);
+}
+
+// FP64 true residual of the solve of half-step `which` that just ended
+// (right-hand side from V, increment in its buffer), folded into tres.
+template <class T>
+void launch_true_res(vg_handle *h, const T *V, int which) {
+  double *part = h->part.as<double>() + 6 * (size_t)h->cg_blocks;
+  hipLaunchKernelGGL(k_true_res<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream, sell_of(h),
+                     h->Krhs.as<T>(), h->Ahat.as<T>(), V, xbuf<T>(h, which), h->diag.as<T>(),
+                     cgbufs_of(h), part);
+  hipLaunchKernelGGL(k_true_finish, dim3(1), dim3(BLK), 0, h->stream, part, (int)h->cg_blocks,
+                     cgbufs_of(h), h->tres.as<double>());
 }
 
 // Folds the result records of a finished solve (one per instance) into the
@@ -560,7 +581,7 @@ int launch_ionic(vg_handle *h, const T *Vin, const T *x, T *Vout) {
 void launch_fold(vg_handle *h, int which, bool fold, bool tick) {
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, h->stream, h->res.as<CgResult>(), h->n_inst,
                      h->totals.as<CgTotals>(), which, fold ? 1 : 0, h->step_ctr.as<int64_t>(),
-                     tick ? 1 : 0);
+                     tick ? 1 : 0, h->d_trunc);
 }
 
 // Sums the ECG partials of the last k_apply / k_ecg launch into dst[ne].
@@ -629,13 +650,16 @@ template <class T>
 int step_t(vg_handle *h) {
   int rc = upload_stim(h);
   if (rc) return rc;
+  const bool check = h->tres_every > 0 && h->steps % h->tres_every == 0;
   // Step 1: half-step diffusion, C_m M + dT/4 K against C_m M - dT/4 K.
   if ((rc = cg_solve<T>(h, h->Va.as<T>(), 0))) return rc;
+  if (check) launch_true_res<T>(h, h->Va.as<T>(), 0);
   // Step 2: ionic update over dT in the reference's sub-steps, fused with
   // V = V_a + D^-1/2 x.
   if ((rc = launch_ionic<T>(h, h->Va.as<T>(), xbuf<T>(h, 0), h->Vb.as<T>()))) return rc;
   // Step 3: second half-step.
   if ((rc = cg_solve<T>(h, h->Vb.as<T>(), 1))) return rc;
+  if (check) launch_true_res<T>(h, h->Vb.as<T>(), 1);
   h->steps += 1;
   h->t = h->steps * h->dt;
   const bool frame = h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 &&
@@ -690,6 +714,13 @@ int capture_step(vg_handle *h) {
   }
   for (int w = 0; w < 2; w++)
     h->glaunch[w] = std::min(h->maxit + 1, h->seen_max[w] + std::max(4, h->seen_max[w] / 4) + 1);
+  // Replays run while the host is elsewhere: their solves must not publish
+  // into the record a host-driven solve waits on.
+  struct Capturing {
+    vg_handle *h;
+    explicit Capturing(vg_handle *hh) : h(hh) { h->capturing = true; }
+    ~Capturing() { h->capturing = false; }
+  } capturing_scope(h);
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   const T *V[2] = {h->Va.as<T>(), h->Vb.as<T>()};
   for (int w = 0; w < 2; w++) {
@@ -722,8 +753,12 @@ int capture_step(vg_handle *h) {
 
 // Graph mode: the first steps run host-driven (learning the launch counts),
 // then the captured step is replayed.  ECG frames are taken after a replay
-// from the new V.  Every 32 steps the device totals are read: a solve that
-// ran out of captured launches raises the counts and re-captures.
+// from the new V.  After every replay the host reads the count of solves cut
+// short (published by k_fold into coherent host memory, no synchronisation):
+// when it has grown, the host waits for the stream, doubles the launch counts
+// and re-captures before the next step.  A cut-short solve passes its last
+// iterate on; it is counted in vg_stats.cg_truncated and cg_not_converged.
+// Every 32 steps the device totals are read for non-finite residuals.
 template <class T>
 int step_graph(vg_handle *h) {
   int rc = upload_stim(h);
@@ -737,6 +772,15 @@ int step_graph(vg_handle *h) {
   HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
   h->steps += 1;
   h->t = h->steps * h->dt;
+  if (__atomic_load_n(h->h_trunc, __ATOMIC_ACQUIRE) > h->g_trunc_seen) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->h_tot, h->totals.p, sizeof(CgTotals), hipMemcpyDeviceToHost));
+    h->g_trunc_seen = __atomic_load_n(h->h_trunc, __ATOMIC_ACQUIRE);
+    h->g_notconv_seen = h->h_tot->notconv;
+    for (int w = 0; w < 2; w++)
+      h->seen_max[w] = std::max(h->seen_max[w], std::max(h->glaunch[w], h->h_tot->max_iter[w])) * 2;
+    if ((rc = capture_step<T>(h))) return rc;
+  }
   if (h->ecg_every > 0 && h->ne > 0 && (h->steps % h->ecg_every) == 0 && h->ecg_count < h->ecg_max) {
     hipLaunchKernelGGL(k_ecg<T>, dim3(h->node_blocks), dim3(BLK), 0, h->stream, h->Va.as<T>(),
                        h->rows, h->H.as<T>(), h->ne, h->n_alloc, h->ecg_part.as<double>());
@@ -748,9 +792,9 @@ int step_graph(vg_handle *h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (h->h_tot->nonfinite) return h->fail(VG_ERR_NONFINITE, "non-finite CG residual");
     if (h->h_tot->notconv > h->g_notconv_seen) {
+      // Solves that hit cg_maxit or broke down inside the captured launches
+      // (those cut short were handled after their replay).
       h->g_notconv_seen = h->h_tot->notconv;
-      for (int w = 0; w < 2; w++) h->seen_max[w] = std::max(h->seen_max[w], h->glaunch[w]) * 3 / 2;
-      if ((rc = capture_step<T>(h))) return rc;
     }
   }
   return VG_OK;
@@ -1006,8 +1050,9 @@ int set_cg_blocks(vg_handle *h, int cg_blocks) {
   HIPCHK(h, hipMemcpy(h->blk_inst.p, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(h, h->inst_blk.alloc(inst_blk.size() * 4));
   HIPCHK(h, hipMemcpy(h->inst_blk.p, inst_blk.data(), inst_blk.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(h, h->part.alloc((size_t)h->cg_blocks * 6 * 8));
-  HIPCHK(h, hipMemset(h->part.p, 0, (size_t)h->cg_blocks * 6 * 8));
+  // Two sets of 3 CG partials, then the true-residual check's 2.
+  HIPCHK(h, h->part.alloc((size_t)h->cg_blocks * 8 * 8));
+  HIPCHK(h, hipMemset(h->part.p, 0, (size_t)h->cg_blocks * 8 * 8));
   if (h->gexec) {
     HIPCHK(h, hipGraphExecDestroy(h->gexec));
     h->gexec = nullptr;
@@ -1381,6 +1426,9 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, hipMemset(H_->carry.p, 0, 2 * sizeof(CgCarry) * H_->n_inst));
   HIPCHK(H_, H_->res.alloc(sizeof(CgResult) * H_->n_inst));
   HIPCHK(H_, hipMemset(H_->res.p, 0, sizeof(CgResult) * H_->n_inst));
+  HIPCHK(H_, H_->tres.alloc(4 * sizeof(double)));
+  HIPCHK(H_, hipMemset(H_->tres.p, 0, 4 * sizeof(double)));
+  H_->tres_every = std::max(0, model->true_residual_every);
   HIPCHK(H_, H_->newton.alloc(sizeof(unsigned long long)));
   HIPCHK(H_, hipMemset(H_->newton.p, 0, sizeof(unsigned long long)));
   HIPCHK(H_, H_->step_ctr.alloc(sizeof(int64_t)));
@@ -1394,6 +1442,10 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
                            hipHostMallocCoherent | hipHostMallocMapped));
   std::memset(H_->h_pub, 0, sizeof(CgResult) * H_->n_inst);
   HIPCHK(H_, hipHostGetDevicePointer((void **)&H_->d_pub, H_->h_pub, 0));
+  HIPCHK(H_, hipHostMalloc((void **)&H_->h_trunc, sizeof(unsigned long long),
+                           hipHostMallocCoherent | hipHostMallocMapped));
+  *H_->h_trunc = 0;
+  HIPCHK(H_, hipHostGetDevicePointer((void **)&H_->d_trunc, H_->h_trunc, 0));
   if (lr_gate) {
     const std::vector<double> tab = lr_gate_table();
     if (T == 8) {
@@ -1584,7 +1636,15 @@ int vg_get_cg_blocks(vg_handle *h, int *cg_blocks) {
 
 int vg_set_graph(vg_handle *h, int enable) {
   if (!h) return VG_ERR_ARG;
+  // Queued replays end before a host-driven solve resets the published record.
+  if ((enable ? 1 : 0) != h->graph) HIPCHK(h, hipStreamSynchronize(h->stream));
   h->graph = enable ? 1 : 0;
+  return VG_OK;
+}
+
+int vg_set_true_residual_check(vg_handle *h, int every) {
+  if (!h || every < 0) return VG_ERR_ARG;
+  h->tres_every = every;
   return VG_OK;
 }
 
@@ -1844,6 +1904,12 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   o->sim_time = h->t;
   o->sell_wide_slices = h->n_wide;
   o->row_order = h->row_order;
+  o->cg_truncated = (int64_t)tot.truncated;
+  double tr[4];
+  HIPCHK(h, hipMemcpy(tr, h->tres.p, sizeof(tr), hipMemcpyDeviceToHost));
+  o->max_true_relres = tr[0];
+  o->true_residual_checks = (int64_t)tr[1];
+  o->last_true_relres = tr[2];
   return VG_OK;
 }
 

@@ -798,6 +798,9 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
   if (blockIdx.x == b0 && threadIdx.x == 0) {
     cb.carry[(par ^ 1) * cb.n_inst + inst] = CgCarry{c.rho_ref, gamma, alpha};
     res->iter = j + 1;
+    // x holds an iterate from here on: a solve cut short by the end of a
+    // captured launch sequence still passes its last iterate on.
+    res->x_valid = 1;
   }
 }
 
@@ -1215,30 +1218,113 @@ __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T
 }
 
 // Running totals of the solves of a captured step (read by the host lazily).
+// truncated: solves still running when the captured launches ran out (their
+// last iterate was used; counted in notconv too).
 struct CgTotals {
-  unsigned long long iters, solves, notconv, nonfinite;
+  unsigned long long iters, solves, notconv, nonfinite, truncated;
   int max_iter[2];
   double max_relres;
 };
 
 // One thread: folds the result records of the solve of half-step `which`
 // into the totals (fold != 0) and, with tick != 0, advances the step counter.
+// A solve that has not ended (done == 0) ran out of captured launches: the
+// running count of those goes to trunc_pub, host-coherent memory the host
+// reads after every replay without synchronising.
 __global__ void k_fold(const CgResult *__restrict__ res, int n_inst, CgTotals *tot, int which,
-                       int fold, int64_t *step_ctr, int tick) {
+                       int fold, int64_t *step_ctr, int tick, unsigned long long *trunc_pub) {
   if (threadIdx.x != 0) return;
   if (fold) {
+    bool cut = false;
     for (int k = 0; k < n_inst; k++) {
       const CgResult c = res[k];
       tot->iters += (unsigned long long)c.iter;
       tot->solves += 1;
       tot->notconv += (unsigned long long)(c.done == 0 || c.notconv != 0);
+      if (c.done == 0) {
+        tot->truncated += 1;
+        cut = true;
+      }
       if (!isfinite(c.rho) || !isfinite(c.rho_ref)) tot->nonfinite += 1;
       if (c.iter > tot->max_iter[which]) tot->max_iter[which] = c.iter;
       const double rel = c.rho_ref > 0.0 ? sqrt(c.rho / c.rho_ref) : 0.0;
       if (rel > tot->max_relres) tot->max_relres = rel;
     }
+    if (cut && trunc_pub)
+      __hip_atomic_store(trunc_pub, tot->truncated, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   if (tick) step_ctr[0] += 1;
+}
+
+// Position of entry j of a lane of SELL slice `slice` (the layout sell_dot
+// reads) and, in col, the row index it gathers from.
+__device__ inline int64_t sell_pos(const Sell &S, int64_t slice, int lane, int j, int64_t &col) {
+  const int64_t o = S.off[slice];
+  const int w4 = S.width[slice] & ~3;
+  const int64_t rel = j < w4 ? (int64_t)(j >> 2) * 256 + 4 * lane + (j & 3)
+                             : (int64_t)w4 * 64 + (int64_t)(j - w4) * 64 + lane;
+  const int64_t wo = S.wide[slice];
+  col = wo < 0 ? slice * 64 + lane + S.col16[o + rel] : (int64_t)S.col32[wo + rel];
+  return o + rel;
+}
+
+// True residual of the solve just ended, in FP64 arithmetic on the stored
+// operator: b = K_rhs V and r = b - A_hat x with every product and sum in
+// FP64 (the CG's own stop test uses the recurrence residual, which in FP32
+// drifts from b - A_hat x).  Partials of ||r||_D^2 and ||b||_D^2 per block;
+// x counts as zero when the solve left no increment (x_valid = 0).
+// The reference reports the same quantity after its solve
+// (EP_CrankNicholson.cc:104-109, TrueResidual()).
+template <class T>
+__global__ __launch_bounds__(BLK) void k_true_res(Sell S, const T *__restrict__ Krhs,
+                                                  const T *__restrict__ Ahat,
+                                                  const T *__restrict__ V,
+                                                  const T *__restrict__ x,
+                                                  const T *__restrict__ diag, CgBufs cb,
+                                                  double *__restrict__ part) {
+  int64_t lo, hi, step;
+  int b0, b1;
+  const int inst = block_work(cb, lo, hi, step, b0, b1);
+  const bool use_x = x && cb.res[inst].x_valid;
+  const int lane = threadIdx.x & 63;
+  double v[2] = {0.0, 0.0};
+  for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step) {
+    const int64_t row = sl * 64 + lane;
+    const int w = S.width[sl];
+    const double vi = (double)V[row];
+    double b = 0.0, ax = use_x ? (double)x[row] : 0.0;  // A_hat's unit diagonal
+    for (int j = 0; j < w; j++) {
+      int64_t c;
+      const int64_t e = sell_pos(S, sl, lane, j, c);
+      b += (double)Krhs[e] * ((double)V[c] - vi);
+      if (use_x) ax += (double)Ahat[e] * (double)x[c];
+    }
+    const double r = b - ax, d = (double)diag[row];
+    v[0] += d * r * r;
+    v[1] += d * b * b;
+  }
+  put_partials<2>(v, part);
+}
+
+// One block: per instance sqrt(||r||_D^2 / ||b||_D^2) from k_true_res's
+// partials (instances with b = 0 skipped); out[0] = the largest seen so far,
+// out[1] = checks so far, out[2] = this check's largest.
+__global__ __launch_bounds__(BLK) void k_true_finish(const double *__restrict__ part, int nblk,
+                                                     CgBufs cb, double *__restrict__ out) {
+  double worst = 0.0;
+  for (int k = 0; k < cb.n_inst; k++) {
+    const int b0 = cb.n_inst == 1 ? 0 : cb.inst_blk[k];
+    const int b1 = cb.n_inst == 1 ? nblk : cb.inst_blk[k + 1];
+    double t[2];
+    sum_partials<2>(part, nblk, b0, b1, t);
+    if (t[1] > 0.0) worst = fmax(worst, sqrt(t[0] / t[1]));
+    __syncthreads();  // the next sum reuses sum_partials' shared slots
+  }
+  if (threadIdx.x == 0) {
+    out[0] = fmax(out[0], worst);
+    out[1] += 1.0;
+    out[2] = worst;
+  }
 }
 
 // ECG of V alone (partials as k_apply's).

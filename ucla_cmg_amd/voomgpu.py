@@ -50,7 +50,7 @@ EXPORTED = [
     "vg_version", "vg_create", "vg_destroy", "vg_last_error", "vg_get_stream",
     "vg_set_stimulus_nodes", "vg_set_stimulus_sphere", "vg_set_stimulus_axis",
     "vg_set_stimulus_plane", "vg_clear_stimulus", "vg_step", "vg_synchronize",
-    "vg_set_cg_blocks", "vg_get_cg_blocks", "vg_set_graph",
+    "vg_set_cg_blocks", "vg_get_cg_blocks", "vg_set_graph", "vg_set_true_residual_check",
     "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
     "vg_get_states_ragged", "vg_set_states_ragged",
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
@@ -93,6 +93,7 @@ class VgModel(C.Structure):
         ("cg_x0", C.c_int), ("n_instances", C.c_int), ("instance", C.c_void_p),
         ("graph", C.c_int), ("smoothing", C.c_double), ("scheme", C.c_int),
         ("dt_diffusion", C.c_double), ("row_order", C.c_int),
+        ("true_residual_every", C.c_int),
     ]
 
 
@@ -103,6 +104,8 @@ class VgStats(C.Structure):
         ("n_nodes", C.c_int64), ("nnz", C.c_int64), ("sell_padded_nnz", C.c_int64),
         ("last_cg_relres", C.c_double), ("sim_time", C.c_double),
         ("sell_wide_slices", C.c_int64), ("row_order", C.c_int32),
+        ("cg_truncated", C.c_int64), ("max_true_relres", C.c_double),
+        ("last_true_relres", C.c_double), ("true_residual_checks", C.c_int64),
     ]
 
 
@@ -137,6 +140,7 @@ def load_library(path: str = None):
         "vg_set_cg_blocks": ([H, C.c_int], C.c_int),
         "vg_get_cg_blocks": ([H, C.POINTER(C.c_int)], C.c_int),
         "vg_set_graph": ([H, C.c_int], C.c_int),
+        "vg_set_true_residual_check": ([H, C.c_int], C.c_int),
         "vg_get_voltage": ([H, C.c_void_p], C.c_int),
         "vg_set_voltage": ([H, C.c_void_p], C.c_int),
         "vg_get_states": ([H, C.POINTER(C.c_int), C.c_void_p], C.c_int),
@@ -243,7 +247,8 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
                D=(1e-3, 1e-3, 1e-3), Dp=(1.6e-3,) * 3, DpLeft=None, Cm=1.0, dt=0.01,
                dt_ionic=None, cg_tol=1e-7, cg_maxit=1000, quirks=0, param_scales=None,
                param_index=None, sell_sigma=0, cg_blocks=0, cg_x0=X0_ZERO, instance=None,
-               graph=0, smoothing=0.0, scheme=0, dt_diffusion=0.0, row_order=0):
+               graph=0, smoothing=0.0, scheme=0, dt_diffusion=0.0, row_order=0,
+               true_residual_every=0):
     """Builds a VgModel; returns (struct, keep-alive list)."""
     m = VgModel()
     m.cell_model = cell_model
@@ -276,6 +281,7 @@ def make_model(cell_model=MODEL_TT, precision=PREC_FP64, integrator=INTEG_REFERE
     m.scheme = scheme
     m.dt_diffusion = dt_diffusion
     m.row_order = row_order
+    m.true_residual_every = true_residual_every
     if instance is not None:
         ins = np.ascontiguousarray(instance, np.int32)
         keep.append(ins)
@@ -445,6 +451,11 @@ class CardiacSolver:
             self._chk(self.L.vg_set_cg_blocks(self.h, int(cg_blocks)), "set_cg_blocks")
         if graph is not None:
             self._chk(self.L.vg_set_graph(self.h, int(bool(graph))), "set_graph")
+
+    def setTrueResidualCheck(self, every: int):
+        """FP64 true residual after both solves of every ``every``-th step
+        (0 = off); the largest goes to stats()['max_true_relres']."""
+        self._chk(self.L.vg_set_true_residual_check(self.h, int(every)), "set_true_residual_check")
 
     def cgBlocks(self) -> int:
         g = C.c_int()
