@@ -285,6 +285,17 @@ int vg_get_activation(vg_handle *h, double *t_act);
 int vg_ecg_setup(vg_handle *h, const vg_mesh *mesh, const double *electrodes,
                  int n_electrodes, const double D_ecg[3], double radius);
 int vg_ecg(vg_handle *h, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the pseudo-ECG of vg_ecg, of the current V, written to
+ * the caller's device array d_out[ne] on the handle's stream (vg_get_stream),
+ * stream-ordered, without synchronising, allocating or freeing anything.  Its
+ * partial sums live in one file-scope device array per GPU (16 leads x 1024
+ * blocks of FP64, 128 KiB, held until the process ends) that every call on
+ * that GPU shares: calls that overlap on one GPU (handles on different
+ * streams) race.  VG_ERR_ARG when the leads (at most 16) or the handle's
+ * node grid (at most 1024 blocks) exceed that array. */
+int vg_ecg_device(vg_handle *h, double *d_out);
 int vg_ecg_record(vg_handle *h, int steps_per_frame, int max_frames);
 int vg_ecg_frames(vg_handle *h, double *out, int *n_frames);
 

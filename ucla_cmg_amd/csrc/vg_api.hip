@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,33 @@ constexpr double LR_REST[7] = {0.0016534680667748946, 0.9834762339347045, 0.9896
 constexpr double LR_REST_V = -84.59413360428096;
 
 const char *hip_str(hipError_t e) { return hipGetErrorString(e); }
+
+// Partial sums of vg_ecg_device, the entry point that allocates nothing: one
+// FP64 slot per lead and block of its launch, MAX_ECG leads x ECG_SCRATCH_BLOCKS
+// blocks (128 KiB per GPU, held until the process ends).  Every call on one
+// GPU shares it.  Host code never passes `scratch` itself to a kernel (on the
+// host that name is not a device address): vg_create resolves its address on
+// each device once (scratch_of) and kernels take that pointer.
+constexpr int ECG_SCRATCH_BLOCKS = 1024;
+static __device__ __align__(256) double scratch[MAX_ECG * ECG_SCRATCH_BLOCKS];
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // per device index; null until resolved
+
+// Device address of `scratch` on `device`, which must be the current device.
+hipError_t scratch_of(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 struct DevBuf {
   void *p = nullptr;
@@ -190,6 +218,7 @@ struct vg_handle {
   DevBuf blk_inst, inst_blk, inst_slice, slice_inst;  // instance tables (CgBufs)
   std::vector<int64_t> inst_slice_host;               // [n_inst + 1]
   DevBuf H, ecg_part, ecg_out, ecg_frames;
+  double *ecg_scratch = nullptr;  // this device's `scratch` (vg_ecg_device)
   CgResult *h_res = nullptr;  // pinned
   CgResult *h_pub = nullptr;  // coherent host memory the finishing CG launch writes
   CgResult *d_pub = nullptr;  // its device address
@@ -1342,6 +1371,11 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
       *out = h.release();
       return (*out)->hipfail(he, "hipStreamCreate");
     }
+    he = scratch_of(h->device, &h->ecg_scratch);
+    if (he != hipSuccess) {
+      *out = h.release();
+      return (*out)->hipfail(he, "hipGetSymbolAddress(scratch)");
+    }
   }
   *out = h.release();
   vg_handle *H_ = *out;
@@ -1770,6 +1804,26 @@ int vg_ecg(vg_handle *h, double *out) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, h->ecg_out.p, h->ne * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return VG_OK;
+}
+
+int vg_ecg_device(vg_handle *h, double *d_out) {
+  if (!h || !d_out) return VG_ERR_ARG;
+  if (h->ne == 0) return h->fail(VG_ERR_ARG, "ECG not set up");
+  if (h->ne > MAX_ECG || h->node_blocks > (unsigned)ECG_SCRATCH_BLOCKS || !h->ecg_scratch)
+    return h->fail(VG_ERR_ARG, "vg_ecg_device: leads or grid beyond its scratch (16 x 1024)");
+  // Every block writes all MAX_ECG of its slots, and the finishing block
+  // reads only the node_blocks slots of this launch: nothing is cleared.
+  double *part = h->ecg_scratch;
+  if (h->tsz == 8)
+    hipLaunchKernelGGL(k_ecg<double>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
+                       h->Va.as<double>(), h->rows, h->H.as<double>(), h->ne, h->n_alloc, part);
+  else
+    hipLaunchKernelGGL(k_ecg<float>, dim3(h->node_blocks), dim3(BLK), 0, h->stream,
+                       h->Va.as<float>(), h->rows, h->H.as<float>(), h->ne, h->n_alloc, part);
+  hipLaunchKernelGGL(k_ecg_finish, dim3(1), dim3(BLK), 0, h->stream, part, (int)h->node_blocks,
+                     h->ne, d_out);
+  HIPCHK(h, hipGetLastError());
   return VG_OK;
 }
 

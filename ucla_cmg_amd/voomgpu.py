@@ -23,6 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvoomgpu.so")
 
 VG_OK = 0
+VG_ERR_ARG, VG_ERR_HIP, VG_ERR_NOT_CONVERGED, VG_ERR_NONFINITE = -1, -2, -3, -4
 MODEL_LR = 0
 MODEL_LR_GATE = 1  # Luo-Rudy through LuoRudyGateTable (LUORUDYWITHGATE)
 MODEL_TT = 2
@@ -54,7 +55,7 @@ EXPORTED = [
     "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
     "vg_get_states_ragged", "vg_set_states_ragged",
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
-    "vg_ecg_setup", "vg_ecg", "vg_ecg_record", "vg_ecg_frames", "vg_profile_enable",
+    "vg_ecg_setup", "vg_ecg", "vg_ecg_device", "vg_ecg_record", "vg_ecg_frames", "vg_profile_enable",
     "vg_profile_get", "vg_stats_get", "vg_assemble_csr", "vg_cells_create",
     "vg_cells_destroy", "vg_cells_set_scales", "vg_cells_set_quirks", "vg_cells_compute_ion",
     "vg_cells_get_states", "vg_cells_set_states",
@@ -152,6 +153,7 @@ def load_library(path: str = None):
         "vg_get_activation": ([H, C.c_void_p], C.c_int),
         "vg_ecg_setup": ([H, C.POINTER(VgMesh), C.c_void_p, C.c_int, dp, C.c_double], C.c_int),
         "vg_ecg": ([H, C.c_void_p], C.c_int),
+        "vg_ecg_device": ([H, C.c_void_p], C.c_int),
         "vg_ecg_record": ([H, C.c_int, C.c_int], C.c_int),
         "vg_ecg_frames": ([H, C.c_void_p, C.POINTER(C.c_int)], C.c_int),
         "vg_profile_enable": ([H, C.c_int], C.c_int),
