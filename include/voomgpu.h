@@ -231,7 +231,7 @@ typedef struct {
   double max_true_relres;     /* largest FP64 true residual of the checked solves
                                  (vg_model.true_residual_every) */
   double last_true_relres;    /* the last check's */
-  int64_t true_residual_checks; /* steps checked */
+  int64_t true_residual_checks; /* solves checked (two per checked step) */
 } vg_stats;
 
 const char *vg_version(void);

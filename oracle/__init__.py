@@ -34,6 +34,11 @@ def build() -> None:
 
 
 def _load():
+    # ORACLE_LIB names another build of these sources (the planted-mistake
+    # test runs the pins against deliberately broken copies).
+    alt = os.environ.get("ORACLE_LIB")
+    if alt:
+        return C.CDLL(alt)
     if not os.path.exists(_LIB_PATH):
         build()
     return C.CDLL(_LIB_PATH)
@@ -258,6 +263,23 @@ def tt_constants():
     return c
 
 
+# Columns of the ten Tusscher constant table that the five conductance
+# scalings multiply, in the scaling order gNa, gKr, gKs, gCaL, gto
+# (SetUpIonicConstants.h:93-145: C[16] G_Na, C[14] G_Kr, C[15] G_Ks,
+# C[18] G_CaL, C[20] G_to).
+TT_SCALED_COLUMNS = (16, 14, 15, 18, 20)
+
+
+def tt_constants_scaled(scales):
+    """One ten Tusscher constant table per row of ``scales`` [k, 5]
+    (flattened, the layout ``Simulation(tt_const=...)`` takes)."""
+    sc = np.asarray(scales, np.float64).reshape(-1, 5)
+    t = np.tile(tt_constants(), (sc.shape[0], 1))
+    for k, col in enumerate(TT_SCALED_COLUMNS):
+        t[:, col] *= sc[:, k]
+    return t.ravel()
+
+
 def tt_init():
     s = np.zeros(TT_NSTATE)
     lib().or_tt_init(s)
@@ -343,6 +365,9 @@ class Simulation:
                 one = lr_init() if model == MODEL_LR else tt_init()
                 states = np.tile(one, (n, 1))
         self.states = np.ascontiguousarray(states, np.float64).reshape(n, nst).copy()
+        # Without a V0 every node starts at its own model's initial V: the
+        # body's table here, the Purkinje table below (Purkinje.h:44-50).
+        own_v0 = V0 is None or np.ndim(V0) == 0
         if V0 is None:
             V0 = self.states[:, 7] if model == MODEL_LR else self.states[:, 0]
         self.V = np.ascontiguousarray(np.broadcast_to(V0, (n,)), np.float64).copy()
@@ -387,7 +412,7 @@ class Simulation:
             self.pk_states = np.tile(pk_init(), (n, 1))
             s.pk_flag = self.pk_flag.ctypes.data
             s.pk_states = self.pk_states.ctypes.data
-            if V0 is None or np.ndim(V0) == 0:
+            if own_v0:
                 self.V[self.pk_flag == 1] = self.pk_states[0, 0]
         self.s = s
         if lib().or_sim_init(C.byref(s)) != 0:

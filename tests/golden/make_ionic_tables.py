@@ -3,8 +3,10 @@
 tests/golden/ionic_tables.json.
 
 Reads, as text, voomEP/src/IonicModel/{SetUpIonicConstants.h, Tusscher.h,
-LR.cc, Mahajan.cc} under /root/reference and records only numbers: the ten
-Tusscher and Purkinje constant arrays, the ten Tusscher initial state (the
+LR.cc, Mahajan.cc, Purkinje.h, Purkinje.cc} under /root/reference and records
+only numbers: the ten Tusscher and Purkinje constant arrays, the Purkinje
+initial state, Xi, cell area and the centre and slope of each logistic
+steady state of its gates, the ten Tusscher initial state (the
 active block, not the commented-out one), the Luo-Rudy initial table and
 constants, and the Mahajan constants, per-tissue-type G_tof / G_Ks and
 per-type initial states.
@@ -147,6 +149,34 @@ def run_constants(body, pos):
     return [C[j] for j in range(35)]
 
 
+def purkinje_tables():
+    """Purkinje.h's initial state, Xi and cell surface area, and the
+    (centre, slope) pair of every logistic steady state of Purkinje.cc's
+    ComputeRates, x_inf = 1/(1 + exp((V + a)/b)), keyed by the state whose
+    rate uses it (numbers only)."""
+    h = strip_comments(open(os.path.join(REF, "Purkinje.h")).read())
+    init = indexed_literals(function_body(h, "Purkinje(Real *constants)"), "_State")
+    xi = float(re.search(r"_Xi\s*=\s*" + NUM, h).group(1))
+    area = float(re.search(r"_SurfaceArea\s*=\s*" + NUM, h).group(1))
+    cc = strip_comments(open(os.path.join(REF, "Purkinje.cc")).read())
+    body = function_body(cc, "void Purkinje::ComputeRates")
+    logistic = {}
+    for m in re.finditer(r"_Algebraic\[(\d+)\]\s*=\s*1\.0+/\(1\.0+\+\(exp\(\(\(_State\[0\]\s*([+-])\s*"
+                         + NUM + r"\)/" + NUM + r"\)\)\)\);", body):
+        sign = -1.0 if m.group(2) == "-" else 1.0
+        logistic[int(m.group(1))] = (sign * float(m.group(3)), float(m.group(4)))
+    for m in re.finditer(r"_Algebraic\[(\d+)\]\s*=\s*_Algebraic\[(\d+)\]\s*;", body):
+        if int(m.group(2)) in logistic:
+            logistic[int(m.group(1))] = logistic[int(m.group(2))]
+    gates = {}
+    for m in re.finditer(r"_Rates\[(\d+)\]\s*=\s*\(_Algebraic\[(\d+)\]\s*-\s*_State\[(\d+)\]\)", body):
+        g, k = int(m.group(1)), int(m.group(2))
+        if g == int(m.group(3)) and k in logistic:
+            gates[str(g)] = list(logistic[k])
+    return {"purkinje_initial_state": init, "purkinje_xi": xi, "purkinje_area_cm2": area,
+            "purkinje_logistic_gates": gates}
+
+
 def main():
     const_h = strip_comments(open(os.path.join(REF, "SetUpIonicConstants.h")).read())
     tt = indexed_literals(function_body(const_h, "void SetUpTusscherParameters"), "Constants")
@@ -172,6 +202,7 @@ def main():
         "luorudy_constants": lr_consts,
     }
     out.update(mahajan_tables(const_h, mj_cc))
+    out.update(purkinje_tables())
     fail = function_body(const_h, "void SetUpMahajan_failParameters")
     out["mahajan_fail_constants_by_type"] = [run_constants(fail, t) for t in range(9)]
     out["mahajan_fail_constants_default"] = run_constants(fail, 99)

@@ -123,3 +123,116 @@ def test_smoothing_equals_the_scaled_conductivity(gpu_device):
     Va, Vb, Vc = a.getVoltage(), b.getVoltage(), c.getVoltage()
     assert np.abs(Va - Vb).max() < 1e-8
     assert np.abs(Va - Vc).max() > 1.0  # and smoothing does change the run
+
+
+def _read_restart(path, n):
+    """InterVariables_time_* read here from its byte layout
+    (EPSolver.cc:155-176): per node an int32 count, then that many FP64."""
+    raw = open(path, "rb").read()
+    out, pos = [], 0
+    for _ in range(n):
+        nv = int(np.frombuffer(raw, np.int32, 1, pos)[0])
+        pos += 4
+        out.append(np.frombuffer(raw, np.float64, nv, pos).copy())
+        pos += 8 * nv
+    assert pos == len(raw), "trailing bytes"
+    return out
+
+
+def test_restart_file_layout_against_oracle_states(gpu_device, tmp_path):
+    """The restart and voltage files of an FP64 run over a slab with a
+    Purkinje cable, parsed here byte by byte, against the oracle's states:
+    19 values per ten Tusscher node, 22 per Purkinje node (its own V
+    first), and the raw FP64 voltage frame (EPSolver.cc:116-176)."""
+    import oracle
+    from test_gpu_purkinje import cable_slab
+    mesh, n, target = cable_slab()
+    kw = dict(D=(1e-3,) * 3, Dp=(1.6e-3,) * 3, dt=0.02)
+    gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, precision=vg.PREC_FP64,
+                           integrator=vg.INTEG_REFERENCE, cg_tol=1e-10, **kw)
+    ref = oracle.Simulation(mesh, oracle.MODEL_TT, cg_tol=1e-10, V0=gpu.getVoltage(), **kw)
+    root = np.arange(n, n + 3, dtype=np.int32)
+    gpu.setStimulusCurrent(root, 2.5e5)
+    ref.istim[root] = 2.5e5
+    for k in range(600):
+        if k == 100:
+            gpu.clearStimulus()
+            ref.istim[:] = 0.0
+        gpu.Solve(1)
+        ref.step()
+    gpu.writeRestart(12.0, str(tmp_path))
+    gpu.WriteOutput(12.0, str(tmp_path))
+    name = tmp_path / ("InterVariables_time_%08.2f.%02.0f" % (12.0, 0.0))
+    assert name.name == "InterVariables_time_00012.00.00" and name.exists()
+    st = _read_restart(name, mesh.n_nodes)
+    V = np.fromfile(tmp_path / ("Voltage_time_%08.2f.%02.0f" % (12.0, 0.0)), dtype=np.float64)
+    assert V.shape == (mesh.n_nodes,)
+    np.testing.assert_allclose(V, ref.V, rtol=0, atol=1e-6)
+    assert ref.V.max() > 0.0
+    for i in range(mesh.n_nodes):
+        if i >= n:
+            assert st[i].shape == (22,)
+            np.testing.assert_allclose(st[i], ref.pk_states[i], rtol=1e-6, atol=1e-9)
+        else:
+            assert st[i].shape == (19,)
+            # Slot 0 holds V as the cell last saw it (after the ionic
+            # update); slots 1-18 are the cell's own states.
+            np.testing.assert_allclose(st[i][1:], ref.states[i][1:], rtol=1e-6, atol=1e-9)
+
+
+def test_initialstates_deck_sets_mahajan_and_purkinje_states(gpu_device, tmp_path):
+    """INITIALSTATES (EPSimulator.cc:379-404): a CELL block (26 Mahajan
+    states) and a PURKINJE block (22 states); CardiacBody::New gives the
+    Purkinje nodes the PURKINJE block and the Mahajan myocardium the CELL
+    block.  Five steps through voomgpu_ep, then the restart file parsed here
+    against the oracle started from the same states."""
+    import oracle
+    from test_gpu_purkinje import cable_slab
+    mesh, n, target = cable_slab()
+    vg.write_mesh(f"{tmp_path}/cable", mesh)
+    cell = oracle.mj_init(5).copy()
+    cell[0] = -78.0  # off the type's rest, so the states move
+    pkb = oracle.pk_init().copy()
+    pkb[0] = -70.0
+    lines = ["Cell states"] + [repr(float(v)) for v in cell] + [""] + ["Purkinje states"] + \
+        [repr(float(v)) for v in pkb] + [""]
+    (tmp_path / "init.txt").write_text("\n".join(lines))
+    os.makedirs(tmp_path / "out")
+    deck = tmp_path / "mj.input"
+    deck.write_text(f"""MODEL {tmp_path}/cable
+OUTPUTDIR {tmp_path}/out
+CELLMODEL Mahajan
+DT 0.02
+IONICDT 0.02
+SIMULATIONTIME 0.1
+FREQUENCY 0.1
+WRITERESTART 0.1
+TWODRADIUS 0.04
+TISSUEDIFFUSION 0.001 0.001 0.001
+PURKINJEDIFFUSION 0.0016 0.0016 0.0016
+LVPURKINJEDIFFUSION 0.0016 0.0016 0.0016
+CGTOLERANCE 1e-10
+INITIALSTATES {tmp_path}/init.txt
+""")
+    _run(deck)
+    m2 = vg.read_mesh(f"{tmp_path}/cable", radius=0.04)
+    V0 = np.where((m2.tissue == 11), pkb[0], cell[0])
+    ref = oracle.Simulation(m2, oracle.MODEL_MJ, dt=0.02, D=(1e-3,) * 3, Dp=(1.6e-3,) * 3,
+                            cg_tol=1e-10, states=np.tile(cell, (m2.n_nodes, 1)), V0=V0,
+                            param_idx=np.zeros(m2.n_nodes, np.uint16))
+    ref.pk_states[:] = pkb
+    for _ in range(5):
+        ref.step()
+    st = _read_restart(vg.frame_name(str(tmp_path / "out"), "InterVariables", 0.1, 0),
+                       m2.n_nodes)
+    V = _frame(tmp_path / "out", 0.1)
+    np.testing.assert_allclose(V, ref.V, rtol=0, atol=1e-6)
+    moved = np.abs(ref.V - V0).max()
+    assert moved > 0.01
+    for i in range(m2.n_nodes):
+        if m2.tissue[i] == 11:
+            assert st[i].shape == (22,)
+            np.testing.assert_allclose(st[i], ref.pk_states[i], rtol=1e-6, atol=1e-9)
+        else:
+            assert st[i].shape == (26,)
+            np.testing.assert_allclose(st[i][1:], ref.states[i][1:], rtol=1e-6, atol=1e-9)

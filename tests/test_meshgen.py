@@ -38,3 +38,22 @@ def test_lv_shell_default_size():
     nt, nc, nl = 26, 240, 165
     assert nt * nc * nl == 1029600
     assert (nt - 1) * nc * (nl - 1) * 6 == 5904000
+
+
+def test_reduced_lv_case_covers_the_tree_presolve_paths():
+    """The reduced LV of the GPU parity tests (tests/test_gpu_lv.py) has what
+    the tree pre-solve's paths need: 1,024 < tree rows <= 4,096 (the FP64
+    workgroup's limit), branching nodes with three tree neighbours, and PMJ
+    targets coupled to several terminals."""
+    import numpy as np
+    mesh, _, pidx = meshgen.lv_workload(1 / 40, "high", seg_len=0.065)
+    nm = mesh.meta["n_myocardium"]
+    n_pk = mesh.n_nodes - nm
+    bars = mesh.conn[mesh.elem_nn == 2][:, :2]
+    deg = np.bincount(bars.ravel(), minlength=mesh.n_nodes)
+    pmj = bars[(bars[:, 0] >= nm) & (bars[:, 1] < nm)]
+    _, per_target = np.unique(pmj[:, 1], return_counts=True)
+    assert 1024 < n_pk <= 4096
+    assert (deg[nm:] == 3).sum() > 200 and deg[nm:].max() == 3
+    assert pmj.shape[0] == 256 and per_target.max() >= 2 and (per_target > 1).sum() > 20
+    assert pidx.shape == (mesh.n_nodes,) and pidx.max() < 24

@@ -208,3 +208,76 @@ def test_slab_conducts_with_per_node_tissue_types():
     far = nx - 1
     print("activated fraction", (act >= 0).mean(), "far-end AT", act[far])
     assert (act >= 0).mean() > 0.5
+
+
+def _mj_inf_tau(V, k, t=2):
+    """Steady state and time constant of gate k at V, from the exact
+    relaxation R = (x_inf - (x_inf - x) exp(-h/tau) - x)/h at x = 0 and 1."""
+    c = oracle.mj_constants(t)
+    s = oracle.mj_init(t)
+    s[0] = V
+    h = 0.01
+    s[k] = 0.0
+    r0 = oracle.mj_rates(c, s, h)[k]
+    s[k] = 1.0
+    r1 = oracle.mj_rates(c, s, h)[k]
+    return r0 / (r0 - r1), -h / np.log(1.0 - h * (r0 - r1))
+
+
+XTOF, YTOF, XTOS, YTOS = 20, 21, 22, 23
+
+
+def test_transient_outward_gates():
+    """I_to slow and fast (Mahajan.cc:352-385, after Shannon et al. 2005):
+    both share x_inf = 1/(1 + exp(-(V + 3)/15)) and y_inf = 1/(1 + exp((V +
+    33.5)/10)); tau_xs = 9/(1 + exp((V + 3)/15)) + 0.5, tau_ys = 3000/(1 +
+    exp((V + 60)/10)) + 30, tau_xf = 3.5 exp(-(V/30)^2) + 1.5, tau_yf = 20/(1 +
+    exp((V + 33.5)/10)) + 20 -- checked where each exponential is 1 and at
+    the limits."""
+    for k in (XTOF, XTOS):
+        assert _mj_inf_tau(-3.0, k)[0] == pytest.approx(0.5, abs=1e-9)
+    for k in (YTOF, YTOS):
+        assert _mj_inf_tau(-33.5, k)[0] == pytest.approx(0.5, abs=1e-9)
+    assert _mj_inf_tau(-3.0, XTOS)[1] == pytest.approx(5.0, rel=1e-8)
+    assert _mj_inf_tau(-300.0, XTOS)[1] == pytest.approx(9.5, rel=1e-8)
+    assert _mj_inf_tau(-60.0, YTOS)[1] == pytest.approx(1530.0, rel=1e-8)
+    assert _mj_inf_tau(200.0, YTOS)[1] == pytest.approx(30.0, rel=1e-8)
+    assert _mj_inf_tau(0.0, XTOF)[1] == pytest.approx(5.0, rel=1e-8)
+    assert _mj_inf_tau(300.0, XTOF)[1] == pytest.approx(1.5, rel=1e-8)
+    assert _mj_inf_tau(-33.5, YTOF)[1] == pytest.approx(30.0, rel=1e-8)
+    assert _mj_inf_tau(200.0, YTOF)[1] == pytest.approx(20.0, rel=1e-8)
+
+
+def test_sr_leak_has_no_driving_force_at_the_volume_ratio():
+    """The SR leak (Mahajan.cc:529-535) is driven by 16.667 c_j - c_i
+    (16.667 = cell/SR volume ratio) and scales with c_j^2: it vanishes at
+    c_j = c_i/16.667 and at c_j -> 0.  The SR load rate is -I_r + J_up - J_leak
+    with J_up a function of c_i alone (:197-198), so it takes the same value
+    at both c_j."""
+    c = oracle.mj_constants(2)
+    s = oracle.mj_init(2)
+    # A high c_i puts the leak's first zero where the c_j^2 factor is not
+    # small (c_j = 60 uM against its half point 50 uM).
+    s[9] = 1000.0
+    r = []
+    for cj in (s[9] / 16.667, 1e-9):
+        e = s.copy()
+        e[10] = cj
+        r.append(oracle.mj_rates(c, e, 0.01)[10])
+    assert r[0] == pytest.approx(r[1], rel=1e-9, abs=1e-15)
+    e = s.copy()
+    e[10] = 2.0 * s[9] / 16.667
+    assert oracle.mj_rates(c, e, 0.01)[10] < r[0]  # above the ratio the SR leaks out
+
+
+def test_sodium_gates_follow_luo_rudy():
+    """Mahajan's I_Na gates are the LR1991 ones (Mahajan.cc:291-313):
+    beta_m = 0.08 exp(-V/11) (0.08 at 0 mV), alpha_h = 0.135 exp(-(80 + V)/
+    6.8) below -40 mV (0.135 at -80 mV), alpha = x_inf/tau."""
+    def ab(V, k):
+        x, tau = _mj_inf_tau(V, k)
+        return x / tau, (1.0 - x) / tau
+    assert ab(0.0, 1)[1] == pytest.approx(0.08, rel=1e-8)
+    assert ab(-80.0, 2)[0] == pytest.approx(0.135, rel=1e-8)
+    a1, a2 = ab(-75.0, 2)[0], ab(-70.0, 2)[0]
+    assert np.log(a2 / a1) / 5.0 == pytest.approx(-1.0 / 6.8, rel=1e-6)

@@ -1623,11 +1623,13 @@ int vg_set_stimulus_plane(vg_handle *h, const double o[3], const double nrm[3], 
   if (!h || !o || !nrm) return VG_ERR_ARG;
   const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
   if (!(len > 0)) return h->fail(VG_ERR_ARG, "zero plane normal");
-  // CardiacBody.cc:265-281: below the plane -> istim, above -> 0, Purkinje skipped.
+  // CardiacBody.cc:265-281: below the plane -> istim, above -> 0, Purkinje
+  // skipped; the normal is normalised first, then dotted with x - origin.
+  const double nh[3] = {nrm[0] / len, nrm[1] / len, nrm[2] / len};
   for (int64_t i = 0; i < h->n; i++) {
     if (h->purkinje[i]) continue;
     double d = 0;
-    for (int k = 0; k < 3; k++) d += (h->xyz[3 * i + k] - o[k]) * nrm[k] / len;
+    for (int k = 0; k < 3; k++) d += (h->xyz[3 * i + k] - o[k]) * nh[k];
     h->istim_host[i] = (d < 0.) ? istim : 0.0;
   }
   h->stim_dirty = true;

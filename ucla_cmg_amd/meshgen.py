@@ -338,6 +338,60 @@ def purkinje_tree(mesh: Mesh, generations=8, seg_len=0.06, seed=1, offset=0.02,
                 radius=mesh.radius, meta=meta)
 
 
+# Junction resistances of the tree's high and low PMJ variants (Ohm m).
+PMJ_RESISTANCE = {"high": 500.0, "low": 125.0}
+
+
+def lv_workload(scale: float = 1.0, purkinje: str = "high", seg_len: float = 0.06):
+    """The ten Tusscher LV workload: ``lv_shell`` (its grid shrunk by
+    ``scale`` in node count, the geometry unchanged) with the seeded
+    ``purkinje_tree`` (``purkinje`` = "high", "low" or "none" for the PMJ
+    resistance; ``seg_len`` its bar length in cm), endo/M/epi cells by transmural depth (30/40/30 %) and an
+    apicobasal G_Ks gradient in 8 bands (1.5x at the apex to 1.0x at the
+    base).  Returns (mesh, scales [24, 5] in the ten Tusscher scaling order
+    gNa gKr gKs gCaL gto, per-node parameter set uint16); Purkinje nodes
+    carry a set that their own model ignores."""
+    if scale == 1.0:
+        mesh = lv_shell()
+    else:
+        mesh = lv_shell(n_trans=max(4, int(26 * scale ** (1 / 3))),
+                        n_circ=2 * max(4, int(120 * scale ** (1 / 3))),
+                        n_long=max(4, int(165 * scale ** (1 / 3))))
+    if purkinje != "none":
+        mesh = purkinje_tree(mesh, seg_len=seg_len,
+                             pmj_length=pmj_length(PMJ_RESISTANCE[purkinje]))
+    depth = mesh.meta["depth"]
+    height = np.clip(mesh.meta["height"], 0.0, 1.0)
+    # Cell type by transmural depth: endo [0, .3), M [.3, .7), epi [.7, 1].
+    ctype = np.where(depth < 0.3, 0, np.where(depth < 0.7, 1, 2))
+    band = np.minimum((height * 8).astype(int), 7)
+    ks_grad = 1.5 - 0.5 * (np.arange(8) + 0.5) / 8
+    # Published tT2006 layer differences: G_to endo 0.073 (vs 0.294),
+    # G_Ks M 0.098 (vs 0.392).
+    gto = {0: 0.073 / 0.294, 1: 1.0, 2: 1.0}
+    gks = {0: 1.0, 1: 0.098 / 0.392, 2: 1.0}
+    scales = []
+    for t in range(3):
+        for b in range(8):
+            scales.append([1.0, 1.0, gks[t] * ks_grad[b], 1.0, gto[t]])
+    pidx = (ctype * 8 + band).astype(np.uint16)
+    return mesh, np.array(scales), pidx
+
+
+def his_nodes(mesh: Mesh, n: int = 3) -> np.ndarray:
+    """The Purkinje root and the next nodes of its first branch (the His
+    stimulus of the LV workload)."""
+    r = int(mesh.meta["purkinje_root"])
+    return np.arange(r, r + n, dtype=np.int32)
+
+
+def lv_electrodes() -> np.ndarray:
+    """The LV workload's 10 electrodes: 8 on a 5 cm ring around the long
+    axis at z = -3.5 cm, one below the apex and one above the base."""
+    ring = electrodes_around((0.0, 0.0, -3.5), 5.0, 8)
+    return np.vstack([ring, [[0.0, 0.0, -9.0], [0.0, 0.0, 3.0]]])
+
+
 def scale_cross_fiber(mesh: Mesh, ratio: float) -> Mesh:
     """Copy of ``mesh`` whose sheet and normal fiber vectors are scaled by
     1/sqrt(ratio): with D = (D_l, D_l, D_l) the element tensor
