@@ -232,6 +232,8 @@ typedef struct {
                                  (vg_model.true_residual_every) */
   double last_true_relres;    /* the last check's */
   int64_t true_residual_checks; /* solves checked (two per checked step) */
+  int64_t sell_shared_offset_slices; /* slices whose rows share one set of column
+                                        offsets (no per-row columns stored) */
 } vg_stats;
 
 const char *vg_version(void);

@@ -236,3 +236,75 @@ INITIALSTATES {tmp_path}/init.txt
         else:
             assert st[i].shape == (26,)
             np.testing.assert_allclose(st[i][1:], ref.states[i][1:], rtol=1e-6, atol=1e-9)
+
+
+def test_reference_driver_switch(gpu_device, tmp_path):
+    """REFERENCEDRIVER 1 reproduces the reference driver (EPSimulator.cc):
+    the LV Purkinje diffusion is PURKINJEDIFFUSION's whatever
+    LVPURKINJEDIFFUSION says (:418-420), and stopping a zonal stimulus also
+    zeroes every other non-Purkinje node's stimulus (the fall-through into
+    the planar stop, :552-558).  Without it the driver honours both."""
+    from test_gpu_purkinje import cable_slab
+    mesh, n, target = cable_slab()
+    vg.write_mesh(f"{tmp_path}/cable", mesh)
+    far = [i for i in range(n) if mesh.xyz[i, 0] > 0.3][:9]
+    (tmp_path / "root.txt").write_text("\n".join(str(i + 1) for i in range(n, n + 3)) + "\n")
+    (tmp_path / "far.txt").write_text("\n".join(str(i + 1) for i in far) + "\n")
+    runs = {}
+    for ref in (0, 1):
+        d = tmp_path / f"out{ref}"
+        os.makedirs(d)
+        deck = tmp_path / f"d{ref}.input"
+        deck.write_text(f"""MODEL {tmp_path}/cable
+OUTPUTDIR {d}
+CELLMODEL Tusscher
+DT 0.02
+IONICDT 0.02
+SIMULATIONTIME 4
+FREQUENCY 4
+TWODRADIUS 0.04
+TISSUEDIFFUSION 0.001 0.001 0.001
+PURKINJEDIFFUSION 0.0016 0.0016 0.0016
+LVPURKINJEDIFFUSION 0.004 0.004 0.004
+STIMULUS NODAL {tmp_path}/root.txt 0. 2. 250000
+STIMULUS ZONAL x 0.0 0.5 2.5 60000
+STIMULUS NODAL {tmp_path}/far.txt 1. 3.5 100000
+REFERENCEDRIVER {ref}
+""")
+        _run(deck)
+        runs[ref] = _frame(d, 4.0)
+    m2 = vg.read_mesh(f"{tmp_path}/cable", radius=0.04)
+
+    def binding(DpLeft, far_off_at):
+        h = vg.CardiacSolver(m2, cell_model=vg.MODEL_TT, precision=vg.PREC_FP64,
+                             integrator=vg.INTEG_REFERENCE, D=(1e-3,) * 3, Dp=(0.0016,) * 3,
+                             DpLeft=DpLeft, dt=0.02, cg_tol=1e-7)
+        root = np.arange(n, n + 3, dtype=np.int32)
+        far_ids = np.array(far, np.int32)
+        # The driver's schedule: starts and stops each run through the list
+        # in order (EPSimulator.cc:478-570).
+        for count in range(200):
+            if count == 0:
+                h.setStimulusCurrent(root, 250000.0)
+            if count == 25:
+                h.setStimulusCurrent(0, 0.0, 60000.0)
+            if count == 50:
+                h.setStimulusCurrent(far_ids, 100000.0)
+            if count == 100:
+                h.setStimulusCurrent(root, 0.0)
+            if count == 125:
+                h.setStimulusCurrent(0, 0.0, 0.0)
+                if far_off_at == 125:
+                    h.setStimulusCurrent((0.0, 0.0, 0.0), -1.0, 0.0)
+            if count == 175:
+                h.setStimulusCurrent(far_ids, 0.0)
+            h.Solve(1)
+        return h.getVoltage()
+    ours = binding((0.004,) * 3, 175)
+    reference = binding((0.0016,) * 3, 125)
+    print(f"driver vs binding: max|dV| {np.abs(runs[0] - ours).max():.2e} (own rules), "
+          f"{np.abs(runs[1] - reference).max():.2e} (reference rules); the two rule sets "
+          f"differ by {np.abs(runs[0] - runs[1]).max():.1f} mV")
+    np.testing.assert_array_equal(runs[0], ours)
+    np.testing.assert_array_equal(runs[1], reference)
+    assert np.abs(runs[0] - runs[1]).max() > 1.0

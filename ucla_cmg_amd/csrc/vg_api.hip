@@ -174,7 +174,7 @@ struct vg_handle {
   bool stim_dirty = false, stim_active = false;
   SubSteps sub{};
   // device
-  DevBuf off, width, wide, col16, col32, Ahat, Krhs, sinv, diag;
+  DevBuf off, width, wide, col16, col32, ustart, uoff, Ahat, Krhs, sinv, diag;
   DevBuf Va, Vb, r, x, states, istim, pidx, scales, at;
   // CG rows {r, w, s, p} interleaved, one array per parity of the iteration
   // (the explicit scheme uses q0 as two plain work vectors).
@@ -264,6 +264,7 @@ struct vg_handle {
   // stats
   int64_t steps = 0, cg_iters = 0, cg_solves = 0, cg_notconv = 0;
   int64_t n_wide = 0;             // slices on 32-bit columns
+  int64_t n_uniform = 0;          // slices with shared column offsets
   int row_order = VG_ORDER_MESH;  // the myocardium rows' order (VG_ORDER_*)
   double last_relres = 0.0, t = 0.0;
   bool profiling = false;
@@ -360,8 +361,9 @@ void drain_profile(vg_handle *h) {
 }
 
 Sell sell_of(vg_handle *h) {
-  return Sell{h->off.as<int64_t>(), h->width.as<int>(), h->wide.as<int64_t>(),
-              h->col16.as<int16_t>(), h->col32.as<int>()};
+  return Sell{h->off.as<int64_t>(), h->width.as<int>(),  h->wide.as<int64_t>(),
+              h->col16.as<int16_t>(), h->col32.as<int>(), h->ustart.as<int64_t>(),
+              h->uoff.as<int>()};
 }
 
 CgBufs cgbufs_of(vg_handle *h) {
@@ -1352,6 +1354,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->n_slices = sell.n_slices;
   h->padded = sell.padded;
   h->n_wide = std::count_if(sell.wide.begin(), sell.wide.end(), [](int64_t w) { return w >= 0; });
+  h->n_uniform = std::count_if(sell.ustart.begin(), sell.ustart.end(), [](int64_t u) { return u >= 0; });
   h->n_alloc = ((h->rows + BLK - 1) / BLK) * BLK;
   h->new2old = sell.new2old;
   h->old2new = sell.old2new;
@@ -1391,6 +1394,11 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   HIPCHK(H_, hipMemcpy(H_->wide.p, sell.wide.data(), sell.wide.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(H_, H_->col16.alloc(sell.col16.size() * 2));
   HIPCHK(H_, hipMemcpy(H_->col16.p, sell.col16.data(), sell.col16.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->ustart.alloc(sell.ustart.size() * 8));
+  HIPCHK(H_, hipMemcpy(H_->ustart.p, sell.ustart.data(), sell.ustart.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(H_, H_->uoff.alloc(std::max<size_t>(1, sell.uoff.size()) * 4));
+  if (!sell.uoff.empty())
+    HIPCHK(H_, hipMemcpy(H_->uoff.p, sell.uoff.data(), sell.uoff.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(H_, H_->col32.alloc(sell.col32.size() * 4));
   if (!sell.col32.empty())
     HIPCHK(H_, hipMemcpy(H_->col32.p, sell.col32.data(), sell.col32.size() * 4, hipMemcpyHostToDevice));
@@ -1959,6 +1967,7 @@ int vg_stats_get(vg_handle *h, vg_stats *o) {
   o->last_cg_relres = h->last_relres;
   o->sim_time = h->t;
   o->sell_wide_slices = h->n_wide;
+  o->sell_shared_offset_slices = h->n_uniform;
   o->row_order = h->row_order;
   o->cg_truncated = (int64_t)tot.truncated;
   double tr[4];

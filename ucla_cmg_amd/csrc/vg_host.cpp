@@ -280,15 +280,40 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
     if (o.new2old[i] >= 0) o.old2new[o.new2old[i]] = (int32_t)i;
   o.width.assign(o.n_slices, 0);
   o.offset.assign(o.n_slices + 1, 0);
+  o.ustart.assign(o.n_slices, -1);
+  o.uoff.clear();
+  // A slice is stored with shared offsets (ustart >= 0) when the union U of
+  // its rows' relative column offsets has at most UNIFORM_MAX members and
+  // every row + u of the slice (padding lanes included) is a row the kernels
+  // write (< n_slices * 64): then entry j of every lane gathers row + U[j]
+  // and no per-lane column is stored.  Structured meshes in grid order have
+  // one such set for nearly every slice.
+  constexpr int UNIFORM_MAX = 20;
+  const int64_t kernel_rows = o.n_slices * SELL_C;
+  std::vector<std::vector<int64_t>> urel(o.n_slices);
   for (int64_t s = 0; s < o.n_slices; s++) {
     int w = 0;
+    std::vector<int64_t> uni;
     for (int l = 0; l < SELL_C; l++) {
       const int64_t r = s * SELL_C + l;
       if (r < rows && o.new2old[r] >= 0) {
         const int32_t old = o.new2old[r];
         // The diagonal is not stored (see below).
         w = std::max<int>(w, (int)(c.row_ptr[old + 1] - c.row_ptr[old]) - 1);
+        for (int64_t k = c.row_ptr[old]; k < c.row_ptr[old + 1]; k++)
+          if (c.col[k] != old) uni.push_back((int64_t)o.old2new[c.col[k]] - r);
       }
+    }
+    std::sort(uni.begin(), uni.end());
+    uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+    bool uniform = !uni.empty() && (int)uni.size() <= UNIFORM_MAX &&
+                   s * SELL_C + uni.front() >= 0 && s * SELL_C + SELL_C - 1 + uni.back() < kernel_rows;
+    if (uniform) {
+      w = (int)uni.size();
+      urel[s] = uni;
+      o.ustart[s] = (int64_t)o.uoff.size();
+      for (int64_t u : uni) o.uoff.push_back((int32_t)u);
+      while (o.uoff.size() % 4) o.uoff.push_back(0);
     }
     o.width[s] = w;
     o.offset[s + 1] = o.offset[s] + (int64_t)w * SELL_C;
@@ -321,6 +346,29 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
           if (c.col[k] != old) row.emplace_back(o.old2new[c.col[k]], k);
         std::sort(row.begin(), row.end());
       }
+      if (o.ustart[s] >= 0) {
+        // Shared offsets: entry j gathers r + U[j]; rows without that
+        // neighbour hold zeros there.  The row's own entries, sorted by
+        // column, keep their order among the slots.
+        const std::vector<int64_t> &U = urel[s];
+        size_t q = 0;
+        for (int j = 0; j < o.width[s]; j++) {
+          const int64_t slot = sell_slot(o.offset[s], o.width[s], j, l);
+          o.col[slot] = (int32_t)(r + U[j]);
+          if (q < row.size() && (int64_t)row[q].first - r == U[j]) {
+            const int64_t k = row[q].second;
+            const int32_t old_r = o.new2old[r], old_c = c.col[k];
+            const double mass = lumped ? (old_c == old_r ? c.CL[old_r] : 0.0) : Cm * c.M[k];
+            const double a = kc * c.K[k] + mass;
+            o.A_hat[slot] = a / std::sqrt(d[old_r] * d[old_c]);
+            o.K_rhs[slot] = -rc * c.K[k] / std::sqrt(d[old_r]);
+            o.K_exp[slot] = -c.K[k] / c.CL[old_r];
+            q++;
+          }
+        }
+        if (q != row.size()) return "shared-offset slice lost an entry (internal error)";
+        continue;
+      }
       for (int j = 0; j < o.width[s]; j++) {
         const int64_t slot = sell_slot(o.offset[s], o.width[s], j, l);
         if (j < (int)row.size()) {
@@ -344,6 +392,7 @@ std::string build_sell(const HostCsr &c, double kc, double rc, double Cm, bool l
   o.wide.assign(o.n_slices, -1);
   o.col32.clear();
   for (int64_t s = 0; s < o.n_slices; s++) {
+    if (o.ustart[s] >= 0) continue;  // no per-lane columns
     bool narrow = true;
     for (int j = 0; j < o.width[s] && narrow; j++)
       for (int l = 0; l < SELL_C; l++) {

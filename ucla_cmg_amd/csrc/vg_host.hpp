@@ -45,6 +45,11 @@ struct HostSell {
   std::vector<int16_t> col16;    // [padded]
   std::vector<int32_t> col32;    // wide slices only, same lane-fastest layout
   std::vector<int64_t> wide;     // [n_slices]
+  // Slices whose 64 rows all take their entries from one sorted set of
+  // relative offsets (a row lacking one holds a zero there): ustart[s] >= 0
+  // indexes that set in uoff (padded to a multiple of 4); -1 otherwise.
+  std::vector<int64_t> ustart;   // [n_slices]
+  std::vector<int32_t> uoff;
   std::vector<double> A_hat;     // D^-1/2 (mass + kc K) D^-1/2
   std::vector<double> K_rhs;     // -rc D^-1/2 K
   std::vector<double> K_exp;     // -C_L^-1 K (unscaled; the explicit scheme)

@@ -222,13 +222,19 @@ __device__ inline int block_work(const CgBufs &cb, int64_t &lo, int64_t &hi, int
 // value load and one 8-byte column load per lane fetch a group), the 0-3 left
 // over lane-fastest ([entry][lane]).  Columns are 16-bit offsets from the row
 // in col16 (same layout), or, for a slice with a column further than 32767
-// rows away (wide[s] >= 0), int32 in col32 from wide[s].
+// rows away (wide[s] >= 0), int32 in col32 from wide[s].  A slice whose rows
+// all draw from one sorted set of relative offsets (ustart[s] >= 0) stores
+// no per-lane columns: entry j of every lane gathers row + uoff[ustart[s] + j]
+// (a row lacking that neighbour holds a zero value there), the offsets being
+// the same for the whole wave (scalar loads).
 struct Sell {
   const int64_t *off;
   const int *width;
   const int64_t *wide;
   const int16_t *col16;
   const int *col32;
+  const int64_t *ustart;
+  const int *uoff;
 };
 
 template <class T>
@@ -307,12 +313,75 @@ __device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restri
   return acc;
 }
 
+// The same value in every lane of the wave, held in scalar registers: a
+// slice index is (all lanes of a wave work on one slice), and loads from a
+// scalar address go through the scalar cache instead of vector memory.
+__device__ inline int64_t wave_uniform(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// sell_dot for a shared-offset slice: entry j gathers row + uo[j], uo the
+// slice's (wave-uniform) offsets, padded to whole groups of four.
+template <class T, class G>
+__device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__restrict__ uo,
+                               int lane, int row, G gather) {
+  using V4 = typename Vec4Of<T>::type;
+  const int n4 = wd >> 2;
+  const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(vs) + lane;
+  const int4 *__restrict__ up = reinterpret_cast<const int4 *>(uo);
+  T acc = T(0);
+  int g = 0;
+  for (; g + 2 <= n4; g += 2) {
+    V4 v[2];
+    int4 c[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      v[q] = vp[(g + q) * 64];
+      c[q] = up[g + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      acc += v[q].x * gather(row + c[q].x);
+      acc += v[q].y * gather(row + c[q].y);
+      acc += v[q].z * gather(row + c[q].z);
+      acc += v[q].w * gather(row + c[q].w);
+    }
+  }
+  if (g < n4) {
+    const V4 v = vp[g * 64];
+    const int4 c = up[g];
+    acc += v.x * gather(row + c.x);
+    acc += v.y * gather(row + c.y);
+    acc += v.z * gather(row + c.z);
+    acc += v.w * gather(row + c.w);
+  }
+  const int rem = wd & 3;
+  if (rem) {
+    const T *__restrict__ vt = vs + (int64_t)n4 * 256 + lane;
+    const int4 c = up[n4];
+    const int cc[3] = {c.x, c.y, c.z};
+    T vv[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < rem) vv[q] = vt[q * 64];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < rem) acc += vv[q] * gather(row + cc[q]);
+  }
+  return acc;
+}
+
 // One lane's row of a slice: sum_j val[j] * gather(col[j]).
 template <class T, class G>
 __device__ inline T sell_row(const Sell &S, const T *__restrict__ val, int64_t slice, int lane,
                              G gather) {
+  slice = wave_uniform(slice);
   const int64_t o = S.off[slice];
   const int wd = S.width[slice];
+  const int64_t us = S.ustart[slice];
+  if (us >= 0) return sell_dot_u<T>(wd, val + o, S.uoff + us, lane, (int)(slice * 64 + lane), gather);
   const int64_t wo = S.wide[slice];
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
@@ -366,11 +435,63 @@ __device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__re
   acc_b = sb;
 }
 
+// sell_dot2 for a shared-offset slice (see sell_dot_u).
+template <class T, class GA, class GB>
+__device__ inline void sell_dot2_u(int wd, const T *__restrict__ as, const T *__restrict__ bs,
+                                   const int *__restrict__ uo, int lane, int row, GA ga, GB gb,
+                                   T &acc_a, T &acc_b) {
+  using V4 = typename Vec4Of<T>::type;
+  const int n4 = wd >> 2;
+  const V4 *__restrict__ ap = reinterpret_cast<const V4 *>(as) + lane;
+  const V4 *__restrict__ bp = reinterpret_cast<const V4 *>(bs) + lane;
+  const int4 *__restrict__ up = reinterpret_cast<const int4 *>(uo);
+  T sa = T(0), sb = T(0);
+  for (int g = 0; g < n4; g++) {
+    const V4 x = ap[g * 64], y = bp[g * 64];
+    const int4 c4 = up[g];
+    const int c[4] = {row + c4.x, row + c4.y, row + c4.z, row + c4.w};
+    const T xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sa += xs[q] * ga(c[q]);
+      sb += ys[q] * gb(c[q]);
+    }
+  }
+  const int rem = wd & 3;
+  if (rem) {
+    const int64_t t = (int64_t)n4 * 256 + lane;
+    const int4 c4 = up[n4];
+    const int c[3] = {row + c4.x, row + c4.y, row + c4.z};
+    T x[3], y[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < rem) {
+        x[q] = as[t + q * 64];
+        y[q] = bs[t + q * 64];
+      }
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q < rem) {
+        sa += x[q] * ga(c[q]);
+        sb += y[q] * gb(c[q]);
+      }
+  }
+  acc_a = sa;
+  acc_b = sb;
+}
+
 template <class T, class GA, class GB>
 __device__ inline void sell_row2(const Sell &S, const T *__restrict__ va, const T *__restrict__ vb,
                                  int64_t slice, int lane, GA ga, GB gb, T &acc_a, T &acc_b) {
+  slice = wave_uniform(slice);
   const int64_t o = S.off[slice];
   const int wd = S.width[slice];
+  const int64_t us = S.ustart[slice];
+  if (us >= 0) {
+    sell_dot2_u<T>(wd, va + o, vb + o, S.uoff + us, lane, (int)(slice * 64 + lane), ga, gb, acc_a,
+                   acc_b);
+    return;
+  }
   const int64_t wo = S.wide[slice];
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
@@ -1263,8 +1384,11 @@ __device__ inline int64_t sell_pos(const Sell &S, int64_t slice, int lane, int j
   const int w4 = S.width[slice] & ~3;
   const int64_t rel = j < w4 ? (int64_t)(j >> 2) * 256 + 4 * lane + (j & 3)
                              : (int64_t)w4 * 64 + (int64_t)(j - w4) * 64 + lane;
+  const int64_t us = S.ustart[slice];
   const int64_t wo = S.wide[slice];
-  col = wo < 0 ? slice * 64 + lane + S.col16[o + rel] : (int64_t)S.col32[wo + rel];
+  col = us >= 0   ? slice * 64 + lane + S.uoff[us + j]
+        : wo < 0  ? slice * 64 + lane + S.col16[o + rel]
+                  : (int64_t)S.col32[wo + rel];
   return o + rel;
 }
 

@@ -13,7 +13,8 @@
 // WRITERESTART t, STIMULUS NODAL|CIRC|ZONAL|PLANE ... (EPSimulator.cc:147-266).
 // Mechanics keywords are read and ignored.  Keywords of this driver only:
 // GPUPRECISION FP64|MIXED, IONICINTEGRATOR REFERENCE|RUSHLARSEN, CGX0
-// ZERO|PREVIOUS|LAST|EXTRAPOLATE, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE, and
+// ZERO|PREVIOUS|LAST|EXTRAPOLATE, GRAPH 0|1, QUIRKS <VG_QUIRK_* bits>, CGTOLERANCE,
+// REFERENCEDRIVER 0|1 (below), and
 // SOLVERTYPE CRANKNICKOLSON|BACKWARDEULER|EXPLICIT (the reference fixes
 // CRANKNICKOLSON, EPSimulator.cc:88, and has getSolverType for the others).
 //
@@ -26,6 +27,16 @@
 // solves with Solve()'s default 0.1); WRITERESTART below one step writes no
 // restart (the reference divides by 0); ZONAL axis letters are read in
 // either case (the reference's map lookup is case-sensitive).
+//
+// REFERENCEDRIVER 1 runs the reference's driver instead, where it differs
+// observably: LVPURKINJEDIFFUSION is read but the LV Purkinje diffusion is
+// PURKINJEDIFFUSION's (EPSimulator.cc:418-420); stopping a zonal stimulus
+// falls through into the planar stop, which with zero current zeroes every
+// non-Purkinje node whatever its plane (:552-558, CardiacBody.cc:265-281);
+// and WRITERESTART below one step stops the run where the reference would
+// divide by zero (:489, :585).  The planar stop's use of the start counter's
+// plane (:556-558) changes nothing: with zero current both sides of any
+// plane are set to zero.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -94,6 +105,7 @@ int main(int argc, char **argv) {
   double dt_diffusion = 0.1;
   int scheme = VG_SCHEME_CN;
   bool tol_given = false;
+  bool reference_driver = false;
   double D[3] = {0.001, 0.0005, 0.00025}, Dp[3] = {0.0032, 0.0032, 0.0032},
          DpLeft[3] = {0.0032, 0.0032, 0.0032};
   bool read_restart = false, write_restart = false;
@@ -139,6 +151,7 @@ int main(int argc, char **argv) {
     } else if (k == "GRAPH") graph = (int)num(t, 1);
     else if (k == "QUIRKS") quirks = (unsigned)num(t, 1);
     else if (k == "CGTOLERANCE") cg_tol = num(t, 1), tol_given = true;
+    else if (k == "REFERENCEDRIVER") reference_driver = num(t, 1) != 0.0;
     else if (k == "DIFFUSIONDT") dt_diffusion = num(t, 1);
     else if (k == "SOLVERTYPE") {
       const std::string v = upper(t.at(1));
@@ -201,7 +214,7 @@ int main(int argc, char **argv) {
   for (int j = 0; j < 3; j++) {
     model.D[j] = D[j];
     model.Dp[j] = Dp[j];
-    model.DpLeft[j] = DpLeft[j];
+    model.DpLeft[j] = reference_driver ? Dp[j] : DpLeft[j];
   }
   model.Cm = 1.0;  // EPSimulator.cc:416
   model.dt = dt;
@@ -290,6 +303,8 @@ int main(int argc, char **argv) {
   long pulse_start = stims.empty() ? -1 : (long)(stims[0].t_start / dt);
   long pulse_stop = stims.empty() ? -1 : (long)(stims[0].t_stop / dt);
   const long restart_write = (long)(restart_write_t / dt);
+  if (reference_driver && write_restart && restart_write == 0)
+    die("WRITERESTART is below one step: the reference divides by zero here");
   if (read_restart) {
     std::printf("@@ Reading Restart information\n");
     simtime = restart_read_t;
@@ -307,7 +322,16 @@ int main(int argc, char **argv) {
         break;
       }
       case 1: check(h, vg_set_stimulus_sphere(h, s.center, s.radius, amp), "stimulus"); break;
-      case 2: check(h, vg_set_stimulus_axis(h, s.axis, s.limit, amp), "stimulus"); break;
+      case 2:
+        check(h, vg_set_stimulus_axis(h, s.axis, s.limit, amp), "stimulus");
+        if (reference_driver && amp == 0.0) {
+          // The reference's zonal stop falls through into the planar stop with
+          // zero current: every non-Purkinje node gets zero (a sphere of
+          // negative radius contains no node, so it zeroes them all).
+          const double o[3] = {0.0, 0.0, 0.0};
+          check(h, vg_set_stimulus_sphere(h, o, -1.0, 0.0), "stimulus");
+        }
+        break;
       default: check(h, vg_set_stimulus_plane(h, s.center, s.normal, amp), "stimulus");
     }
   };

@@ -107,6 +107,7 @@ class VgStats(C.Structure):
         ("sell_wide_slices", C.c_int64), ("row_order", C.c_int32),
         ("cg_truncated", C.c_int64), ("max_true_relres", C.c_double),
         ("last_true_relres", C.c_double), ("true_residual_checks", C.c_int64),
+        ("sell_shared_offset_slices", C.c_int64),
     ]
 
 
