@@ -182,7 +182,8 @@ typedef struct {
    * vg_stats.max_true_relres.  The stop test uses the CG recurrence
    * residual; in mixed precision the two drift apart.  Counterpart of
    * AztecOO's TrueResidual() the reference prints after each solve
-   * (EP_CrankNicholson.cc:104-109).  Host-driven steps only. */
+   * (EP_CrankNicholson.cc:104-109).  With graph = 1 a checked step runs
+   * host-driven (the same step, bit for bit). */
   int true_residual_every;
 } vg_model;
 

@@ -127,6 +127,7 @@ class GpuCNSolver {
 
   // EPSolver::InitializeVoltage and the voltage view written by WriteOutput.
   void InitializeVoltage(const std::vector<double> &V) {
+    if (V.size() != n_) throw Error(VG_ERR_ARG, "InitializeVoltage: one value per node");
     check(vg_set_voltage(h_, V.data()), "InitializeVoltage");
   }
   void InitializeVoltage(double v) { InitializeVoltage(std::vector<double>(n_, v)); }
@@ -136,24 +137,33 @@ class GpuCNSolver {
     return V;
   }
 
-  // CardiacBody::getStateVariables / setStateVariables (CardiacBody.cc:438-463).
+  // CardiacBody::getStateVariables / setStateVariables (CardiacBody.cc:438-463):
+  // per node its own model's states (Purkinje nodes carry 22, tT myocardium
+  // 19, ...), V first.
   void getStateVariables(std::vector<int> &numVar, std::vector<std::vector<double>> &data) {
-    int nv = 0;
-    check(vg_get_states(h_, &nv, nullptr), "getStateVariables");
-    std::vector<double> flat(n_ * (size_t)nv);
-    check(vg_get_states(h_, &nv, flat.data()), "getStateVariables");
-    numVar.assign(n_, nv);
-    data.assign(n_, std::vector<double>(nv));
-    for (size_t i = 0; i < n_; i++)
-      for (int k = 0; k < nv; k++) data[i][k] = flat[i * nv + k];
+    std::vector<int32_t> nv(n_);
+    check(vg_get_states_ragged(h_, nv.data(), nullptr), "getStateVariables");
+    size_t tot = 0;
+    for (int32_t k : nv) tot += (size_t)k;
+    std::vector<double> flat(tot);
+    check(vg_get_states_ragged(h_, nv.data(), flat.data()), "getStateVariables");
+    numVar.assign(nv.begin(), nv.end());
+    data.assign(n_, std::vector<double>());
+    size_t o = 0;
+    for (size_t i = 0; i < n_; i++) {
+      data[i].assign(flat.begin() + o, flat.begin() + o + nv[i]);
+      o += (size_t)nv[i];
+    }
   }
   void setStateVariables(const std::vector<std::vector<double>> &data) {
-    if (data.size() != n_ || data.empty()) throw Error(VG_ERR_ARG, "state count");
-    const int nv = (int)data[0].size();
-    std::vector<double> flat(n_ * (size_t)nv);
-    for (size_t i = 0; i < n_; i++)
-      for (int k = 0; k < nv; k++) flat[i * nv + k] = data[i][k];
-    check(vg_set_states(h_, nv, flat.data()), "setStateVariables");
+    if (data.size() != n_) throw Error(VG_ERR_ARG, "setStateVariables: one state list per node");
+    std::vector<int32_t> nv(n_);
+    std::vector<double> flat;
+    for (size_t i = 0; i < n_; i++) {
+      nv[i] = (int32_t)data[i].size();
+      flat.insert(flat.end(), data[i].begin(), data[i].end());
+    }
+    check(vg_set_states_ragged(h_, nv.data(), flat.data()), "setStateVariables");
   }
 
   std::vector<double> lumpedCapacitance() {

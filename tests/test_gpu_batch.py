@@ -163,6 +163,23 @@ def test_captured_step_equals_host_driven_step(gpu_device, cg_x0):
     assert b[2]["cg_not_converged"] == 0
 
 
+def test_captured_step_with_true_residual_checks(gpu_device):
+    """With graph = 1 a step due an FP64 true-residual check runs host-driven:
+    the run stays bitwise the host-driven one, and both check the same solves
+    with the same result."""
+    parts, mesh, inst, offs, stim = _batch(DIMS, RATIOS, SCALES)
+    kw = dict(cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN,
+              dt=0.02, D=(DL, DL, DL), cg_tol=1e-7, param_scales=SCALES,
+              param_index=inst.astype(np.uint16), instance=inst, cg_x0=vg.X0_LAST,
+              true_residual_every=25)
+    a = _run(mesh, dict(kw, graph=0), stim, 1e5, 400, 100)
+    b = _run(mesh, dict(kw, graph=1), stim, 1e5, 400, 100)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2]["true_residual_checks"] == b[2]["true_residual_checks"] == 2 * (400 // 25)
+    assert a[2]["max_true_relres"] == b[2]["max_true_relres"] < 1e-6
+
+
 @pytest.mark.parametrize("cg_x0", [vg.X0_ZERO, vg.X0_LAST])
 def test_captured_step_with_purkinje_stream(gpu_device, cg_x0):
     """The Purkinje ionic kernel runs on its own stream inside the capture

@@ -139,10 +139,13 @@ def test_lv_tree_fp64_matches_oracle(gpu_device, lv_oracle):
     assert (nv[:nm] == 19).all() and (nv[nm:] == 22).all()
     zg, _, _, _ = res[vg.X0_ZERO]
     _, st_z = zg.getStateVariablesRagged()
-    # States at the end, on nodes the fronts have not reached (where the
-    # upstroke has not amplified the last-bit differences).
-    quiet_pk = nm + np.nonzero(at_o[nm:] < 0)[0]
-    quiet_myo = np.nonzero(at_o[:nm] < 0)[0]
+    # States at the end, on nodes the fronts have not reached: not activated
+    # and V within 0.5 mV of its first frame (a node in the foot of an
+    # upstroke amplifies last-bit differences as activated ones do).
+    still = (at_o < 0) & (np.abs(o["V"][-1] - o["V"][0]) < 0.5)
+    quiet_pk = nm + np.nonzero(still[nm:])[0]
+    quiet_myo = np.nonzero(still[:nm])[0]
+    assert len(quiet_pk) > 100 and len(quiet_myo) > 100
     for i in (quiet_pk[0], quiet_pk[-1]):
         np.testing.assert_allclose(st_z[i], o["pk_states"][i], rtol=1e-7, atol=1e-10)
     for i in (quiet_myo[0], quiet_myo[len(quiet_myo) // 2]):

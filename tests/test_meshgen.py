@@ -57,3 +57,18 @@ def test_reduced_lv_case_covers_the_tree_presolve_paths():
     assert (deg[nm:] == 3).sum() > 200 and deg[nm:].max() == 3
     assert pmj.shape[0] == 256 and per_target.max() >= 2 and (per_target > 1).sum() > 20
     assert pidx.shape == (mesh.n_nodes,) and pidx.max() < 24
+
+
+def test_phase_singularity_of_a_synthetic_spiral():
+    """A rotating phase field theta = atan2(y, x) - w t has one singularity of
+    charge +-1 at the origin; a plane wave has none."""
+    from ucla_cmg_amd import analysis
+    y, x = np.mgrid[-1:1:41j, -1:1:41j] + 0.013
+    th = np.arctan2(y, x)
+    V = lambda t: -40.0 + 50.0 * np.cos(th - t)  # noqa: E731
+    Vl = lambda t: -40.0 + 50.0 * np.sin(th - t)  # noqa: E731
+    n, q = analysis.phase_singularities(V(0.3), Vl(0.3))
+    assert n == 1 and abs(q) == 1
+    plane = -40.0 + 50.0 * np.cos(3.0 * x)
+    n, q = analysis.phase_singularities(plane, -40.0 + 50.0 * np.sin(3.0 * x))
+    assert n == 0 and q == 0

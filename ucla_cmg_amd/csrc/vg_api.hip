@@ -789,7 +789,9 @@ int capture_step(vg_handle *h) {
 // when it has grown, the host waits for the stream, doubles the launch counts
 // and re-captures before the next step.  A cut-short solve passes its last
 // iterate on; it is counted in vg_stats.cg_truncated and cg_not_converged.
-// Every 32 steps the device totals are read for non-finite residuals.
+// Every 32 steps the device totals are read for non-finite residuals.  A step
+// that is due an FP64 true-residual check runs host-driven (bitwise the same
+// step), so the captured graph carries no check.
 template <class T>
 int step_graph(vg_handle *h) {
   int rc = upload_stim(h);
@@ -798,6 +800,7 @@ int step_graph(vg_handle *h) {
     h->warm++;
     return step_t<T>(h);
   }
+  if (h->tres_every > 0 && h->steps % h->tres_every == 0) return step_t<T>(h);
   if (!h->gexec || h->g_stim != h->stim_active || h->g_at != h->at_enabled)
     if ((rc = capture_step<T>(h))) return rc;
   HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));

@@ -1,7 +1,9 @@
 // voomgpu_ep -- the EPSimulator driver (Applications/CardiacEP/EPSimulator.cc)
 // over libvoomgpu: reads the same input deck and mesh files, runs the same
 // stimulus schedule with the Crank-Nicolson step on the GPU, and writes the
-// same Voltage_time_* frames and InterVariables_time_* restart files.
+// same Voltage_time_* frames and InterVariables_time_* restart files.  Written
+// against the C++ classes of voomgpu.hpp (readMesh for Mesh::New, GpuCNSolver
+// for CardiacBody + EP_CrankNicholson) as EPSimulator is against voomEP's.
 //
 //   voomgpu_ep deck.input
 //
@@ -48,7 +50,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/voomgpu.h"
+#include "../../include/voomgpu.hpp"
 
 namespace {
 
@@ -84,14 +86,22 @@ std::vector<int32_t> read_node_set(const std::string &path) {
   return ids;
 }
 
-void check(vg_handle *h, int rc, const char *what) {
-  if (rc != VG_OK) die(std::string(what) + ": " + (h ? vg_last_error(h) : "") + " (" +
-                       std::to_string(rc) + ")");
-}
+int run(int argc, char **argv);
 
 }  // namespace
 
+// Errors of the library arrive as voomgpu::Error (code and vg_last_error).
 int main(int argc, char **argv) {
+  try {
+    return run(argc, argv);
+  } catch (const voomgpu::Error &e) {
+    die(std::string(e.what()) + " (" + std::to_string(e.code) + ")");
+  }
+}
+
+namespace {
+
+int run(int argc, char **argv) {
   if (argc != 2) {
     std::fprintf(stderr, "Usage : %s inputFile\n", argv[0]);
     return 1;
@@ -203,12 +213,8 @@ int main(int argc, char **argv) {
   else if (cell_model == "MAHAJAN_FAIL") cm = VG_MODEL_MJ_FAIL;
   else die("CELLMODEL " + cell_model + " is not supported");
 
-  char err[1024] = {0};
-  vg_meshfile *mf = nullptr;
-  if (vg_mesh_read(model_name.c_str(), radius, &mf, err, sizeof err) != VG_OK) die(err);
-  const vg_mesh *mesh = vg_meshfile_mesh(mf);
-  vg_model model{};
-  model.cell_model = cm;
+  const voomgpu::Mesh mesh = voomgpu::readMesh(model_name, radius);
+  vg_model model = voomgpu::default_model(cm);
   model.precision = precision;
   model.integrator = integ;
   for (int j = 0; j < 3; j++) {
@@ -229,16 +235,11 @@ int main(int argc, char **argv) {
   model.dt_diffusion = dt_diffusion;
   // EP_BackwardEuler iterates to 1e-10 (EP_BackwardEuler.cc:65), CN to 1e-7.
   if (scheme == VG_SCHEME_BE && !tol_given) model.cg_tol = 1e-10;
-  vg_handle *h = nullptr;
-  const int rc = vg_create(mesh, &model, &h);
-  if (rc != VG_OK) {
-    const std::string m = h ? vg_last_error(h) : "vg_create failed";
-    if (h) vg_destroy(h);
-    die("vg_create: " + m);
-  }
-  const int64_t n = mesh->n_nodes;
+  voomgpu::GpuCNSolver body(mesh, model);
+  const int64_t n = (int64_t)body.numNodes();
   std::printf("Modelname: %s\nNodes: %lld  Elements: %lld  CellModel: %s  dt: %g ms\n",
-              model_name.c_str(), (long long)n, (long long)mesh->n_elems, cell_model.c_str(), dt);
+              model_name.c_str(), (long long)n, (long long)mesh.elem_nn.size(),
+              cell_model.c_str(), dt);
 
   // INITIALSTATES (EPSimulator.cc:379-404): a "CELL" and a "PURKINJE" block
   // of one number per line.  Purkinje nodes take the PURKINJE block;
@@ -266,16 +267,12 @@ int main(int argc, char **argv) {
       if (flag == "PURKINJE") pk.push_back(std::atof(line.c_str()));
       else if (flag == "CELL") cell.push_back(std::atof(line.c_str()));
     }
-    std::vector<int32_t> nv((size_t)n);
-    check(h, vg_get_states_ragged(h, nv.data(), nullptr), "get_states_ragged");
-    std::vector<int64_t> off((size_t)n + 1, 0);
-    for (int64_t i = 0; i < n; i++) off[i + 1] = off[i] + nv[i];
-    std::vector<double> data((size_t)off[n]);
-    check(h, vg_get_states_ragged(h, nv.data(), data.data()), "get_states_ragged");
-    std::vector<double> V((size_t)n);
-    check(h, vg_get_voltage(h, V.data()), "get_voltage");
+    std::vector<int> nv;
+    std::vector<std::vector<double>> data;
+    body.getStateVariables(nv, data);
+    std::vector<double> V = body.getVoltage();
     for (int64_t i = 0; i < n; i++) {
-      const int t = mesh->tissue ? mesh->tissue[i] : 0;
+      const int t = mesh.tissue.empty() ? 0 : mesh.tissue[i];
       const double *src = nullptr;
       if ((t == 11 || t == 12) && !pk.empty()) {
         if ((int)pk.size() != nv[i]) die("PURKINJE block has the wrong number of states");
@@ -287,12 +284,12 @@ int main(int argc, char **argv) {
         else die("CELL block has the wrong number of states");
       }
       if (src) {
-        std::copy(src, src + nv[i], data.begin() + off[i]);
+        data[i].assign(src, src + nv[i]);
         V[i] = src[0];  // V is the first state of both models
       }
     }
-    check(h, vg_set_states_ragged(h, nv.data(), data.data()), "set_states_ragged");
-    check(h, vg_set_voltage(h, V.data()), "set_voltage");
+    body.setStateVariables(data);
+    body.InitializeVoltage(V);
   }
 
   // Time loop (EPSimulator.cc:478-572).
@@ -308,31 +305,31 @@ int main(int argc, char **argv) {
   if (read_restart) {
     std::printf("@@ Reading Restart information\n");
     simtime = restart_read_t;
-    check(h, vg_read_restart(h, out_dir.c_str(), restart_read_t, 0), "read_restart");
+    body.readRestart(restart_read_t, out_dir);
     count = (long)(simtime / dt);
   } else {
-    check(h, vg_write_voltage(h, out_dir.c_str(), 0.0, 0), "write_voltage");
+    body.WriteOutput(0.0, out_dir);
   }
   auto apply = [&](const Stim &s, double amp) {
     switch (s.type) {
       case 0: {
         const std::vector<int32_t> ids = read_node_set(s.node_file);
         if (amp != 0.0) std::printf("** Will Stimulate %zu Node(s)\n", ids.size());
-        check(h, vg_set_stimulus_nodes(h, ids.data(), (int64_t)ids.size(), amp), "stimulus");
+        body.setStimulusCurrent(ids, amp);
         break;
       }
-      case 1: check(h, vg_set_stimulus_sphere(h, s.center, s.radius, amp), "stimulus"); break;
+      case 1: body.setStimulusCurrent(s.center, s.radius, amp); break;
       case 2:
-        check(h, vg_set_stimulus_axis(h, s.axis, s.limit, amp), "stimulus");
+        body.setStimulusCurrent(s.axis, s.limit, amp);
         if (reference_driver && amp == 0.0) {
           // The reference's zonal stop falls through into the planar stop with
           // zero current: every non-Purkinje node gets zero (a sphere of
           // negative radius contains no node, so it zeroes them all).
           const double o[3] = {0.0, 0.0, 0.0};
-          check(h, vg_set_stimulus_sphere(h, o, -1.0, 0.0), "stimulus");
+          body.setStimulusCurrent(o, -1.0, 0.0);
         }
         break;
-      default: check(h, vg_set_stimulus_plane(h, s.center, s.normal, amp), "stimulus");
+      default: body.setStimulusCurrent(s.center, s.normal, amp);
     }
   };
   while (simtime < sim_time) {
@@ -348,25 +345,24 @@ int main(int argc, char **argv) {
       if (++stop_ctr == stims.size()) stop_ctr--;
       pulse_stop = (long)(stims[stop_ctr].t_stop / dt);
     }
-    check(h, vg_step(h, 1), "step");
+    body.Solve(1);
     simtime += dt;
     count++;
     if (count % steps_per_frame == 0) {
-      check(h, vg_write_voltage(h, out_dir.c_str(), simtime, 0), "write_voltage");
+      body.WriteOutput(simtime, out_dir);
       std::printf("** Writing Output at Time: %g (ms)\n", simtime);
       std::fflush(stdout);
     }
     if (write_restart && restart_write > 0 && count % restart_write == 0) {
-      check(h, vg_write_restart(h, out_dir.c_str(), simtime, 0), "write_restart");
+      body.writeRestart(simtime, out_dir);
       std::printf("@@ Writing Restart at Time: %g (ms)\n", simtime);
     }
   }
-  vg_stats st;
-  check(h, vg_stats_get(h, &st), "stats");
+  const vg_stats st = body.stats();
   std::printf("done: %lld steps, %.2f CG iterations per solve, %lld unconverged solves\n",
               (long long)st.steps, st.cg_solves ? (double)st.cg_iterations / st.cg_solves : 0.0,
               (long long)st.cg_not_converged);
-  vg_destroy(h);
-  vg_meshfile_free(mf);
   return 0;
 }
+
+}  // namespace
