@@ -239,6 +239,125 @@ def sweep_bench(vg, torch, steps, tune):
     return out
 
 
+def sweep_full(args, vg, torch):
+    """The batched sweep over its configured horizon (96 ten Tusscher slabs,
+    6.62M nodes, dt 0.02 ms, --sweep-ms, 500 ms by default): the product form
+    (mixed precision, Rush-Larsen, CG to 1e-7, autotuned launch shape) timed
+    over the whole run, then the same 96 instances with the reference's own
+    integrators in FP64 (backward-Euler/Newton ten Tusscher, CG to 1e-10) --
+    the oracle's method, which the batched FP64 path matches per instance to
+    1e-6 mV in the GPU tests -- on every --sweep-check-every-th instance (that
+    run costs ~5.5x the mixed one per node), and per checked instance: the
+    largest activation-time difference, activation mismatches and the largest
+    V RMS difference over frames every 50 ms.  One JSON line."""
+    from ucla_cmg_amd import autotune, meshgen
+    t_setup = time.perf_counter()
+    meshes, scales, ratios = meshgen.slab_sweep()
+    mesh, inst, offs = meshgen.concat(meshes)
+    dl = MODEL_KW["D"][0]
+    dt = 0.02
+    T = int(round(args.sweep_ms / dt))
+    stim_off = int(round(2.0 / dt))
+    frame_every = int(round(50.0 / dt))
+    n = mesh.n_nodes
+    setup_s = time.perf_counter() - t_setup
+
+    def log(msg):
+        print(f"[sweep] {msg}", file=sys.stderr, flush=True)
+
+    def run(name, prec, integ, tol, tune, parts):
+        sub, sinst, soffs = (mesh, inst, offs) if len(parts) == len(meshes) else \
+            meshgen.concat([meshes[k] for k in parts])
+        solver = vg.CardiacSolver(sub, cell_model=vg.MODEL_TT, precision=prec, integrator=integ,
+                                  dt=dt, D=(dl, dl, dl), cg_tol=tol, param_scales=scales[parts],
+                                  param_index=sinst.astype(np.uint16), instance=sinst,
+                                  cg_x0=vg.X0_LAST)
+        solver.enableActivation(-40.0)
+        solver.setStimulusCurrent(np.concatenate(
+            [meshgen.slab_corner_nodes(*meshes[k].meta["shape"], 3) + soffs[q]
+             for q, k in enumerate(parts)]).astype(np.int32), 50.0 * 2000.0)
+        state = {"k": 0, "solve_s": 0.0, "frames": []}
+
+        def step(m):
+            # Advances m steps in chunks that end at the stimulus switch and at
+            # frame steps; only the solves are timed.
+            end = state["k"] + m
+            while state["k"] < end:
+                k = state["k"]
+                if k == stim_off:
+                    solver.clearStimulus()
+                nxt = min(end, (k // frame_every + 1) * frame_every,
+                          stim_off if k < stim_off else end)
+                t0 = time.perf_counter()
+                solver.Solve(nxt - k)
+                solver.synchronize()
+                state["solve_s"] += time.perf_counter() - t0
+                state["k"] = nxt
+                if nxt % frame_every == 0:
+                    state["frames"].append(solver.getVoltage())
+                    log(f"{name}: {nxt * dt:.0f} ms, {state['solve_s']:.1f} s")
+
+        tuned = None
+        if tune:
+            step(10)
+            tuned = autotune.tune(solver, steps_each=6, blocks=(0, 2048, 4096), step_fn=step)
+        step(T - state["k"])
+        st = solver.stats()
+        res = {"at": solver.activationTimes(), "frames": np.array(state["frames"]),
+               "solve_s": state["solve_s"], "stats": st, "tuned": tuned, "offs": soffs,
+               "nodes": sub.n_nodes}
+        solver.close()
+        return res
+
+    every = list(range(len(meshes)))
+    checked = every[::args.sweep_check_every]
+    mixed = run("mixed", vg.PREC_MIXED, vg.INTEG_RUSH_LARSEN, 1e-7, True, every)
+    ref = run("fp64", vg.PREC_FP64, vg.INTEG_REFERENCE, 1e-10, False, checked)
+    dat, mism, rms, act = [], [], [], []
+    for q, k in enumerate(checked):
+        a, b = offs[k], offs[k + 1]
+        ra, rb = ref["offs"][q], ref["offs"][q + 1]
+        ag, ar = mixed["at"][a:b], ref["at"][ra:rb]
+        both = (ag >= 0) & (ar >= 0)
+        dat.append(float(np.abs(ag[both] - ar[both]).max()) if both.any() else 0.0)
+        mism.append(int(((ag >= 0) != (ar >= 0)).sum()))
+        rms.append(float(np.sqrt(((mixed["frames"][:, a:b] - ref["frames"][:, ra:rb]) ** 2)
+                                 .mean(axis=1)).max()))
+        act.append(float(both.mean()))
+    st = mixed["stats"]
+    out = {
+        "metric": "aggregate node-steps/s, 96 batched tT slabs (per-instance CG), whole run",
+        "value": n * T / mixed["solve_s"], "unit": "node-steps/s", "n_gpus": 1, "steps": T,
+        "ms_per_step": mixed["solve_s"] * 1e3 / T, "higher_is_better": True,
+        "dtype": "fp32 (ionic, SpMV, vectors) + fp64 (CG dots)",
+        "data": "synthetic slabs (seeded sizes, scalings, anisotropy), model initial states",
+        "config": {"workload": f"96 ten Tusscher slabs, {n} nodes "
+                               f"({min(m.n_nodes for m in meshes)}-{max(m.n_nodes for m in meshes)} "
+                               f"each), G_Na/G_Kr/G_CaL ~ U[0.5, 1.5], anisotropy ~ U[2, 10], "
+                               f"corner stimulus 2 ms, dt 0.02 ms, {args.sweep_ms:g} ms"},
+        "cg_iters_per_instance_solve": st["cg_iterations"] / max(1, st["cg_solves"]),
+        "cg_not_converged": st["cg_not_converged"], "cg_truncated": st["cg_truncated"],
+        "autotune": mixed["tuned"]["table"] if mixed["tuned"] else None,
+        "launch": {"cg_blocks": mixed["tuned"]["cg_blocks"], "graph": mixed["tuned"]["graph"]}
+                  if mixed["tuned"] else None,
+        "fp64_reference": {"ms_per_step": ref["solve_s"] * 1e3 / T, "nodes": ref["nodes"],
+                           "cg_not_converged": ref["stats"]["cg_not_converged"]},
+        "per_instance_vs_fp64": {
+            "instances": checked,
+            "nodes": [int(meshes[k].n_nodes) for k in checked],
+            "at_max_err_ms": [round(v, 4) for v in dat],
+            "activation_mismatch": mism,
+            "v_rms_max_mv": [round(v, 4) for v in rms],
+            "activated_frac": [round(v, 4) for v in act],
+            "worst_at_err_ms": max(dat), "median_at_err_ms": float(np.median(dat)),
+            "worst_v_rms_mv": max(rms), "median_v_rms_mv": float(np.median(rms)),
+            "frames_ms": [50.0 * (j + 1) for j in range(mixed["frames"].shape[0])],
+        },
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out))
+
+
 def phase_singularities(Vt, Vlag, vstar=-40.0):
     from ucla_cmg_amd import analysis
     return analysis.phase_singularities(Vt, Vlag, vstar)
@@ -374,9 +493,13 @@ def surrogate_kernel_timing(args, vg):
 
 def surrogate_bench(args, vg, torch):
     """The learned ten Tusscher step on the LV: record the plain model's
-    ionic updates from a plain mixed-precision run, train the network on the
-    GPU, then run the plain FP64, plain mixed and surrogate forms over the
-    same beat and compare activation times, ECG and speed."""
+    ionic updates from plain mixed-precision runs -- by default of 24 slabs
+    of the batched-sweep generator (its sizes, anisotropies and G_Na/G_Kr/
+    G_CaL draws), each carrying one of the LV's 24 G_Ks/G_to parameter sets,
+    paced at a corner; with --surr-train lv, of the LV beat itself -- train
+    the network on the GPU, then run the plain FP64 (the judge), plain mixed
+    and surrogate forms over the LV beat, all host-driven, and compare
+    activation times, ECG and speed (ionic update alone and whole step)."""
     import math
     mesh, scales, pidx = lv_workload(args.scale, args.purkinje)
     if args.purkinje == "none":
@@ -432,20 +555,42 @@ def surrogate_bench(args, vg, torch):
     out = {"metric": "node-steps/s, 1M-node tenTusscher heart, 1 GPU, learned ionic step",
            "unit": "node-steps/s", "n_gpus": 1, "higher_is_better": True, "steps": T,
            "data": "synthetic LV shell mesh; network trained on this run's plain-model updates"}
-    # 1. Samples from the plain mixed run.
+    # 1. Samples from plain mixed runs.
     t0 = time.perf_counter()
     sur = vg.Surrogate(args.surr_seed)
-    h = make(vg.PREC_MIXED, 0)
-    n_myo = int(mesh.meta.get("n_myocardium", n))
-    per = -(-n_myo // args.surr_stride)
+    if args.surr_train == "slabs":
+        from ucla_cmg_amd import meshgen
+        meshes, sweep_scales, _ = meshgen.slab_sweep()
+        pick = list(range(0, len(meshes), 4))[:24]
+        rows = np.array([[sweep_scales[k][0], sweep_scales[k][1], scales[q][2],
+                          sweep_scales[k][3], scales[q][4]] for q, k in enumerate(pick)])
+        tmesh, tinst, toffs = meshgen.concat([meshes[k] for k in pick])
+        dl = MODEL_KW["D"][0]
+        h = vg.CardiacSolver(tmesh, cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED,
+                             integrator=vg.INTEG_RUSH_LARSEN, dt=dt, D=(dl, dl, dl), cg_tol=1e-7,
+                             param_scales=rows, param_index=tinst.astype(np.uint16),
+                             instance=tinst, cg_x0=vg.X0_LAST)
+        h.setStimulusCurrent(np.concatenate(
+            [meshgen.slab_corner_nodes(*meshes[k].meta["shape"], 3) + toffs[q]
+             for q, k in enumerate(pick)]).astype(np.int32), 50.0 * 2000.0)
+        n_rec = tmesh.n_nodes
+        out["data"] = (f"synthetic LV shell mesh; network trained on the plain-model updates of "
+                       f"{len(pick)} slabs of the sweep generator ({n_rec} nodes, corner pacing, "
+                       f"each with one of the LV's G_Ks/G_to sets), not on the LV")
+    else:
+        h = make(vg.PREC_MIXED, 0)
+        n_rec = int(mesh.meta.get("n_myocardium", n))
     records = -(-(T - stim_off) // args.surr_every)
+    # Node stride: --surr-stride, or more to keep to about 2e8 samples.
+    stride = max(args.surr_stride, -(-n_rec * records // 200_000_000))
+    per = -(-n_rec // stride)
     sur.reserve(per * records)
     h.Solve(stim_off)
     h.clearStimulus()
     k = stim_off
     while k < T:
         m = min(2500, T - k)
-        sur.collect(h, m, every=args.surr_every, node_stride=args.surr_stride)
+        sur.collect(h, m, every=args.surr_every, node_stride=stride)
         k += m
         log(f"collect: step {k}/{T}, {sur.num_samples()} samples")
     h.close()
@@ -467,15 +612,20 @@ def surrogate_bench(args, vg, torch):
     out["train"] = {"iterations": its, "batch": args.surr_batch, "seconds": train_s, "lr": args.surr_lr,
                     "loss_curve": curve,
                     "tflops": 3 * 2 * 57344 * its * args.surr_batch / train_s / 1e12}
-    # 3. The three forms over the same beat.
+    # 3. The three forms over the same beat, host-driven (every solve to its
+    # tolerance, checked below).
     runs = {}
     for name, prec, use in (("fp64", vg.PREC_FP64, False), ("plain", vg.PREC_MIXED, False),
                             ("surrogate", vg.PREC_MIXED, True)):
-        h = make(prec, 1)
+        h = make(prec, 0)
         if use:
             sur.attach(h)
         wall = timed_run(h, name)
         runs[name] = result(h, wall)
+        st = h.stats()
+        runs[name]["cg"] = {"not_converged": st["cg_not_converged"],
+                            "truncated": st["cg_truncated"],
+                            "its_per_step": st["cg_iterations"] / max(1, st["steps"])}
         if use:
             runs[name]["usage"] = vg.Surrogate.usage(h)
         if name != "fp64":
@@ -495,7 +645,9 @@ def surrogate_bench(args, vg, torch):
             "activated_mismatch": int(((r["at"] >= 0) != (ref["at"] >= 0)).sum()),
             "ecg_rel_l2": float(np.linalg.norm(r["ecg"] - ref["ecg"]) / np.linalg.norm(ref["ecg"])),
             "v_rms_mv_end": float(np.sqrt(np.mean((r["V"] - ref["V"]) ** 2))),
+            "cg": r["cg"],
         }
+    out["fp64_cg"] = ref["cg"]
     u = runs["surrogate"]["usage"]
     out["surrogate"]["network_frac"] = u[0] / max(1, u[0] + u[1])
     out["value"] = out["surrogate"]["node_steps_per_s"]
@@ -507,7 +659,7 @@ def surrogate_bench(args, vg, torch):
                     "speedup_ge_2": out["speedup_vs_plain"] >= 2.0}
     out["config"] = {"workload": f"tT2006 LV shell + Purkinje tree ({args.purkinje}), {n} nodes, "
                                  f"{args.surr_ms:g} ms beat, dt {dt} ms, network 25-128x4-19 SiLU BF16",
-                     "sample_every_steps": args.surr_every, "sample_node_stride": args.surr_stride}
+                     "sample_every_steps": args.surr_every, "sample_node_stride": stride}
     print(json.dumps(out))
 
 
@@ -606,6 +758,12 @@ def main():
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the batched 96-slab sweep measurement")
     ap.add_argument("--sweep-steps", type=int, default=100)
+    ap.add_argument("--sweep-full", action="store_true",
+                    help="the 96-slab sweep over --sweep-ms (whole run timed) with per-instance "
+                         "errors against the FP64 reference-integrator run, instead")
+    ap.add_argument("--sweep-ms", type=float, default=500.0)
+    ap.add_argument("--sweep-check-every", type=int, default=4,
+                    help="--sweep-full: instances checked against the FP64 reference run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB 20 times (PMC byte "
                          "counter calibration) and exit")
@@ -625,6 +783,8 @@ def main():
     ap.add_argument("--surrogate", action="store_true",
                     help="evaluate the learned ionic step on the LV instead (collect, train, compare)")
     ap.add_argument("--surr-ms", type=float, default=400.0)
+    ap.add_argument("--surr-train", choices=["slabs", "lv"], default="slabs",
+                    help="training data: 24 sweep-generator slabs (default) or the LV beat")
     ap.add_argument("--surr-every", type=int, default=20)
     ap.add_argument("--surr-stride", type=int, default=5)
     ap.add_argument("--surr-iters", type=int, default=160000)
@@ -656,6 +816,9 @@ def main():
         return
     if args.c1:
         c1_bench(args, vg, torch)
+        return
+    if args.sweep_full:
+        sweep_full(args, vg, torch)
         return
     if args.mahajan:
         mahajan_bench(args, vg)

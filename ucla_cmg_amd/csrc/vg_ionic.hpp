@@ -34,6 +34,47 @@ template <> struct Mth<float, false> {
   __device__ static float em1(float x) { return expm1f(x); }
 };
 
+// A float whose division is the hardware reciprocal (1 ulp) times the
+// dividend, for the fast FP32 paths: the rate expressions hold ~100 divisions
+// per node, and an IEEE-exact float division is a ten-instruction sequence
+// on gfx950 (scale, reciprocal, Newton steps, fix-up).  A constant divisor is
+// folded into an exact constant reciprocal.  Only the arithmetic the rate
+// code uses is defined.
+struct RcpF {
+  float v;
+  RcpF() = default;
+  __device__ constexpr RcpF(float x) : v(x) {}
+  __device__ constexpr RcpF(double x) : v((float)x) {}
+  __device__ constexpr RcpF(int x) : v((float)x) {}
+  __device__ constexpr explicit operator float() const { return v; }
+  // A hidden friend, found only for RcpF arguments (a namespace-scope
+  // overload would hide ::fabs from the double and float code here).
+  friend __device__ inline RcpF fabs(RcpF a) { return RcpF(fabsf(a.v)); }
+};
+__device__ inline RcpF operator+(RcpF a, RcpF b) { return RcpF(a.v + b.v); }
+__device__ inline RcpF operator-(RcpF a, RcpF b) { return RcpF(a.v - b.v); }
+__device__ inline RcpF operator*(RcpF a, RcpF b) { return RcpF(a.v * b.v); }
+__device__ inline RcpF operator-(RcpF a) { return RcpF(-a.v); }
+__device__ inline RcpF operator/(RcpF a, RcpF b) {
+  if (__builtin_constant_p(b.v)) return RcpF(a.v * (1.0f / b.v));
+  return RcpF(a.v * __builtin_amdgcn_rcpf(b.v));
+}
+__device__ inline RcpF &operator+=(RcpF &a, RcpF b) { return a = a + b; }
+__device__ inline bool operator<(RcpF a, RcpF b) { return a.v < b.v; }
+__device__ inline bool operator>(RcpF a, RcpF b) { return a.v > b.v; }
+__device__ inline bool operator<=(RcpF a, RcpF b) { return a.v <= b.v; }
+__device__ inline bool operator>=(RcpF a, RcpF b) { return a.v >= b.v; }
+template <bool FAST> struct Mth<RcpF, FAST> {
+  __device__ static RcpF ex(RcpF x) { return RcpF(__expf(x.v)); }
+  __device__ static RcpF lg(RcpF x) { return RcpF(__logf(x.v)); }
+  __device__ static RcpF sq(RcpF x) { return RcpF(__fsqrt_rn(x.v)); }
+  __device__ static RcpF pw(RcpF x, RcpF y) { return RcpF(__powf(x.v, y.v)); }
+  __device__ static RcpF em1(RcpF x) { return RcpF(expm1f(x.v)); }
+};
+// The arithmetic type of a model's rate code: RcpF for the fast FP32 paths.
+template <class T, bool FAST> struct RateT { using type = T; };
+template <> struct RateT<float, true> { using type = RcpF; };
+
 // ======================= Luo-Rudy 1991 ========================================
 // State: 0 m, 1 h, 2 j, 3 d, 4 f, 5 X, 6 Ca_i, 7 V (LR.cc:18-26 with the gate
 // order of make_parameter1 :166-181).
@@ -325,22 +366,32 @@ __device__ inline void tt_currents(const T *S, T istim_term, const TTG<T> &g, T 
 // gates first (at V and the old Ca_ss), then currents at the new gates and old
 // concentrations, mirroring the reference's gates-then-concentrations order.
 // Returns dV/dt; the caller advances V.
+// The fast FP32 form computes in RcpF (reciprocal divisions).
 template <class T, bool FAST>
 __device__ inline T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TTG<T> &g) {
-  using M = Mth<T, FAST>;
-  S[0] = V;
-  T inf[12], tau[12];
-  tt_gates<T, FAST>(V, S[10], inf, tau);
+  using U = typename RateT<T, FAST>::type;
+  using M = Mth<U, FAST>;
+  U Su[TT_NSTATE];
+  Su[0] = U(V);
+#pragma unroll
+  for (int k = 1; k < TT_NSTATE; k++) Su[k] = U(S[k]);
+  const TTG<U> gu{U(g.gNa), U(g.gKr), U(g.gKs), U(g.gCaL), U(g.gto)};
+  const U dtu = U(dt);
+  U inf[12], tau[12];
+  tt_gates<U, FAST>(Su[0], Su[10], inf, tau);
 #pragma unroll
   for (int i = 0; i < 12; i++) {
     const int k = TT_GATE_STATE[i];
-    S[k] = inf[i] - (inf[i] - S[k]) * M::ex(-dt / tau[i]);
+    Su[k] = inf[i] - (inf[i] - Su[k]) * M::ex(-dtu / tau[i]);
   }
-  T R7[7];
-  tt_currents<T, FAST, true>(S, istim_term, g, R7);
+  U R7[7];
+  tt_currents<U, FAST, true>(Su, U(istim_term), gu, R7);
 #pragma unroll
-  for (int i = 0; i < 6; i++) S[TT_NR_STATE[i]] += dt * R7[1 + i];
-  return R7[0];
+  for (int i = 0; i < 6; i++) Su[TT_NR_STATE[i]] += dtu * R7[1 + i];
+  S[0] = V;
+#pragma unroll
+  for (int k = 1; k < TT_NSTATE; k++) S[k] = T(Su[k]);
+  return T(R7[0]);
 }
 
 // 6x6 solve with partial pivoting of J d = B for the row-major array J
@@ -579,24 +630,31 @@ __device__ inline void pk_rates(const T S[PK_NSTATE], T stim, T R[PK_NSTATE]) {
 // Purkinje::Compute_Ion + UpdateStateVariables (Purkinje.cc:179-210):
 // forward Euler on all 22 states, V included, in 10 sub-steps of dt/10 when
 // |dV/dt| > 1.5 mV/ms; returns the last dV/dt evaluated.
+// The fast FP32 form computes the rates in RcpF (reciprocal divisions).
 template <class T, bool FAST>
 __device__ inline T pk_step(T S[PK_NSTATE], T V, T istim, T dt) {
+  using U = typename RateT<T, FAST>::type;
   S[0] = V;
-  const T stim = T(-1E6) * istim * T(PK_AREA) / T(PK_XI);
-  T R[PK_NSTATE];
-  pk_rates<T, FAST>(S, stim, R);
-  if (fabs(R[0]) > T(1.5)) {
-    const T h = dt / T(10);
-    for (int k = 0; k < 10; k++) {
-      pk_rates<T, FAST>(S, stim, R);
+  U Su[PK_NSTATE];
 #pragma unroll
-      for (int j = 0; j < PK_NSTATE; j++) S[j] += R[j] * h;
+  for (int j = 0; j < PK_NSTATE; j++) Su[j] = U(S[j]);
+  const U stim = U(T(-1E6) * istim * T(PK_AREA) / T(PK_XI));
+  U R[PK_NSTATE];
+  pk_rates<U, FAST>(Su, stim, R);
+  if (fabs(T(R[0])) > T(1.5)) {
+    const U h = U(dt / T(10));
+    for (int k = 0; k < 10; k++) {
+      pk_rates<U, FAST>(Su, stim, R);
+#pragma unroll
+      for (int j = 0; j < PK_NSTATE; j++) Su[j] += R[j] * h;
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < PK_NSTATE; j++) S[j] += R[j] * dt;
+    for (int j = 0; j < PK_NSTATE; j++) Su[j] += R[j] * U(dt);
   }
-  return R[0];
+#pragma unroll
+  for (int j = 0; j < PK_NSTATE; j++) S[j] = T(Su[j]);
+  return T(R[0]);
 }
 
 // ======================= Mahajan 2008 (UCLA rabbit ventricle) ================
@@ -876,27 +934,34 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
 // the MJ_ACC states, which the updates accumulate.  Forward Euler over dt, or
 // 10 sub-steps of dt/10 when |dV/dt| > 20 (int_abs: when |trunc(dV/dt)| > 20);
 // returns the last dV/dt evaluated.
+// The fast FP32 form computes the rates in RcpF (reciprocal divisions).
 template <class T, bool FAST, bool FAIL = false>
 __device__ inline T mj_step(T S[MJ_NSTATE], double acc[MJ_NACC], T V, T istim, T dt, T gtof,
                             T gks, bool int_abs) {
-  T R[MJ_NSTATE];
+  using U = typename RateT<T, FAST>::type;
+  U Su[MJ_NSTATE], R[MJ_NSTATE];
   S[0] = V;
-  mj_rates<T, FAST, FAIL>(S, dt, istim, gtof, gks, R);
-  const bool adapt = int_abs ? fabs(trunc(R[0])) > T(20) : fabs(R[0]) > T(20);
+#pragma unroll
+  for (int k = 0; k < MJ_NSTATE; k++) Su[k] = U(S[k]);
+  mj_rates<U, FAST, FAIL>(Su, U(dt), U(istim), U(gtof), U(gks), R);
+  const T r0 = T(R[0]);
+  const bool adapt = int_abs ? fabs(trunc(r0)) > T(20) : fabs(r0) > T(20);
   const int nsub = adapt ? 10 : 1;
   const T h = adapt ? dt / T(10) : dt;
   for (int q = 0; q < nsub; q++) {
-    if (adapt) mj_rates<T, FAST, FAIL>(S, h, istim, gtof, gks, R);
-    S[0] += R[0] * h;
+    if (adapt) mj_rates<U, FAST, FAIL>(Su, U(h), U(istim), U(gtof), U(gks), R);
+    Su[0] += R[0] * U(h);
 #pragma unroll
-    for (int k = 0; k < MJ_NGATE; k++) S[MJ_GATE[k]] += R[MJ_GATE[k]] * h;
+    for (int k = 0; k < MJ_NGATE; k++) Su[MJ_GATE[k]] += R[MJ_GATE[k]] * U(h);
 #pragma unroll
     for (int k = 0; k < MJ_NACC; k++) {
-      acc[k] += (double)R[MJ_ACC[k]] * (double)h;
-      S[MJ_ACC[k]] = (T)acc[k];
+      acc[k] += (double)T(R[MJ_ACC[k]]) * (double)h;
+      Su[MJ_ACC[k]] = U((T)acc[k]);
     }
   }
-  return R[0];
+#pragma unroll
+  for (int k = 0; k < MJ_NSTATE; k++) S[k] = T(Su[k]);
+  return T(R[0]);
 }
 
 }  // namespace vg
