@@ -52,17 +52,35 @@ int64_t n_params_unpadded() {
   return o;
 }
 
+// What a GEMM does with its result tile (row-major, ldc columns):
+//   EPI_STORE    C[z] = the product;
+//   EPI_BIAS_ACT Z = product + bias on the real columns (0 on padding), and
+//                when act, A = SiLU(Z) in BF16 (a layer's forward);
+//   EPI_ACT_BACK dZ = product * SiLU'(Zin) on the real columns, FP32 and
+//                BF16 (the backward through a hidden layer's activation).
+constexpr int EPI_STORE = 0, EPI_BIAS_ACT = 1, EPI_ACT_BACK = 2;
+struct Epi {
+  const float *bias = nullptr;
+  int nreal = 0, act = 0;
+  float *Z = nullptr;
+  __bf16 *A = nullptr;
+  const float *Zin = nullptr;
+  float *dZf = nullptr;
+  __bf16 *dZ = nullptr;
+};
+
 // ---- GEMM: C[z] = op(A) op(B) on MFMA -----------------------------------------
 // op(A) is M x K (stored [M][lda], or [K][lda] when AT), op(B) is K x N
 // (stored [K][ldb], or [N][ldb] when BT).  One wave per 32 x 32 tile of C,
 // four waves per block stacked along M; blockIdx.z takes the k range
 // [z*kchunk, (z+1)*kchunk) and writes C + z*zstride (split K).  M, N are
-// multiples of 32, K and kchunk of 16, lda/ldb of 8.
-template <bool AT, bool BT>
+// multiples of 32, K and kchunk of 16, lda/ldb of 8.  The epilogue EPI
+// (above) is applied in registers, with C's layout.
+template <bool AT, bool BT, int EPI = EPI_STORE>
 __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int lda,
                                               const __bf16 *__restrict__ B, int ldb,
                                               float *__restrict__ C, int ldc, int M, int N, int K,
-                                              int kchunk, int64_t zstride) {
+                                              int kchunk, int64_t zstride, Epi ep = Epi{}) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int row0 = (blockIdx.x * 4 + wave) * 32, col0 = blockIdx.y * 32;
   if (row0 >= M) return;
@@ -85,9 +103,104 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
-  float *Cz = C + (int64_t)blockIdx.z * zstride;
+  const int n = col0 + r;
+  if constexpr (EPI == EPI_STORE) {
+    float *Cz = C + (int64_t)blockIdx.z * zstride;
 #pragma unroll
-  for (int j = 0; j < 16; j++) Cz[(size_t)(row0 + mfma_row(l, j)) * ldc + col0 + r] = acc[j];
+    for (int j = 0; j < 16; j++) Cz[(size_t)(row0 + mfma_row(l, j)) * ldc + n] = acc[j];
+  } else if constexpr (EPI == EPI_BIAS_ACT) {
+    const bool real = n < ep.nreal;
+    const float bn = real ? ep.bias[n] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const size_t i = (size_t)(row0 + mfma_row(l, j)) * ldc + n;
+      const float z = real ? acc[j] + bn : 0.0f;
+      ep.Z[i] = z;
+      if (ep.act) ep.A[i] = (__bf16)(real ? sg_silu(z) : 0.0f);
+    }
+  } else {
+    const bool real = n < ep.nreal;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const size_t i = (size_t)(row0 + mfma_row(l, j)) * ldc + n;
+      float g = 0.0f;
+      if (real) {
+        const float z = ep.Zin[i];
+        const float sg = sg_sigmoid(z);
+        g = acc[j] * sg * (1.0f + z * (1.0f - sg));
+      }
+      ep.dZf[i] = g;
+      ep.dZ[i] = (__bf16)g;
+    }
+  }
+}
+
+// ---- weight gradient: P[z] = A^T D over a batch chunk, staged through LDS ------
+// A is [rows][M] and D [rows][N] (BF16, row-major, M and N multiples of 32 up
+// to 128); block z takes the batch rows [z*kchunk, (z+1)*kchunk) (rows past
+// the chunk's end read as zero) and writes
+// the M x N partial P + z*M*N (row-major).  Both operands need their batch
+// index along the MFMA k dimension, i.e. transposed: the block copies 64
+// batch rows at a time from HBM with 16-byte row reads and stores them
+// transposed into LDS (rows padded by 8), from where each MFMA fragment is one
+// 16-byte read.  Four waves share the (M/32) x (N/32) output tiles.
+constexpr int TN_ROWS = 64, TN_LD = TN_ROWS + 8;
+constexpr int TN_CHUNKS = 256;  // batch chunks (blocks) of a weight-gradient GEMM
+__global__ __launch_bounds__(256) void k_gemm_tn(const __bf16 *__restrict__ A, int M,
+                                                 const __bf16 *__restrict__ D, int N, int kchunk,
+                                                 float *__restrict__ P) {
+  __shared__ __attribute__((aligned(16))) __bf16 At[SG_HID * TN_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Dt[SG_HID * TN_LD];
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63, r = l & 31, kq = (l >> 5) * 8;
+  const int tm = M / 32, ntiles = tm * (N / 32);
+  f32x16 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) acc[q] = f32x16{};
+  const int64_t k0 = (int64_t)blockIdx.x * kchunk;
+  for (int kk = 0; kk < kchunk; kk += TN_ROWS) {
+    __syncthreads();  // the previous sub-chunk's fragments have been read
+    // Copy TN_ROWS rows of A (M columns) and D (N columns), 8 columns per
+    // thread and pass, transposed into At[m][row] and Dt[n][row].
+    for (int e = t; e < TN_ROWS * (M / 8); e += 256) {
+      const int row = e % TN_ROWS, g = e / TN_ROWS;
+      bf16x8 v = {};
+      if (kk + row < kchunk) v = *reinterpret_cast<const bf16x8 *>(A + (k0 + kk + row) * M + g * 8);
+#pragma unroll
+      for (int q = 0; q < 8; q++) At[(g * 8 + q) * TN_LD + row] = v[q];
+    }
+    for (int e = t; e < TN_ROWS * (N / 8); e += 256) {
+      const int row = e % TN_ROWS, g = e / TN_ROWS;
+      bf16x8 v = {};
+      if (kk + row < kchunk) v = *reinterpret_cast<const bf16x8 *>(D + (k0 + kk + row) * N + g * 8);
+#pragma unroll
+      for (int q = 0; q < 8; q++) Dt[(g * 8 + q) * TN_LD + row] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = wave + 4 * q;
+      if (tile < ntiles) {
+        const int mt = tile % tm, nt = tile / tm;
+#pragma unroll
+        for (int ks = 0; ks < TN_ROWS; ks += 16) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(At + (mt * 32 + r) * TN_LD + ks + kq);
+          const bf16x8 b = *reinterpret_cast<const bf16x8 *>(Dt + (nt * 32 + r) * TN_LD + ks + kq);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float *Pz = P + (int64_t)blockIdx.x * M * N;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int tile = wave + 4 * q;
+    if (tile < ntiles) {
+      const int mt = tile % tm, nt = tile / tm;
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        Pz[(size_t)(mt * 32 + mfma_row(l, j)) * N + nt * 32 + r] = acc[q][j];
+    }
+  }
 }
 
 // ---- elementwise kernels ---------------------------------------------------------
@@ -101,32 +214,22 @@ __device__ inline uint64_t splitmix(uint64_t x) {
 // Batch of B samples (hash-drawn from Ns, or rows first.. when idx_seed is 0
 // and first >= 0), normalised: X0[B][32] BF16 (zero padded), Yn[B][19].
 __global__ void k_sg_gather(const float *__restrict__ X, const float *__restrict__ Y, int64_t Ns,
-                            uint64_t seed, int64_t first, int B, const float *__restrict__ mu_x,
+                            uint64_t seed, const int64_t *__restrict__ step, int64_t first, int B,
+                            const float *__restrict__ mu_x,
                             const float *__restrict__ sd_x, const float *__restrict__ mu_y,
                             const float *__restrict__ sd_y, const float *__restrict__ c_y,
                             __bf16 *__restrict__ X0,
                             float *__restrict__ Yn) {
   const int i = blockIdx.x * GB + threadIdx.x;
   if (i >= B) return;
+  // Training batches: the seed of Adam step t + 1 (step holds t).
+  if (step) seed = seed * 0x100000001B3ull + (uint64_t)(*step + 1) * 0x9E3779B97F4A7C15ull;
   const int64_t s = first >= 0 ? first + i : (int64_t)(splitmix(seed + (uint64_t)i) % (uint64_t)Ns);
   for (int f = 0; f < SG_KIN; f++)
     X0[(size_t)i * SG_KIN + f] =
         (__bf16)(f < SG_NIN ? (X[s * SG_NIN + f] - mu_x[f]) / sd_x[f] : 0.0f);
   for (int f = 0; f < SG_NOUT; f++)
     Yn[(size_t)i * SG_NOUT + f] = sg_to_target(Y[s * SG_NOUT + f], mu_y[f], sd_y[f], c_y[f]);
-}
-
-// Z = C + b on the layer's real columns (0 on padding); A = SiLU(Z) in BF16
-// when act (hidden layers).
-__global__ void k_sg_bias_act(const float *__restrict__ C, const float *__restrict__ b, int np,
-                              int nreal, int64_t total, int act, float *__restrict__ Z,
-                              __bf16 *__restrict__ Aout) {
-  const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-  if (i >= total) return;
-  const int n = (int)(i % np);
-  const float z = n < nreal ? C[i] + b[n] : 0.0f;
-  Z[i] = z;
-  if (act) Aout[i] = (__bf16)(n < nreal ? sg_silu(z) : 0.0f);
 }
 
 // Mean-square loss over the 19 outputs: dZ = 2 (pred - y) / (B*19) (FP32 and
@@ -157,29 +260,41 @@ __global__ void k_sg_loss(const float *__restrict__ Z, const float *__restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// dZ = dA * SiLU'(Z) on the real columns.
-__global__ void k_sg_act_back(const float *__restrict__ dA, const float *__restrict__ Z, int np,
-                              int nreal, int64_t total, float *__restrict__ dZf,
-                              __bf16 *__restrict__ dZ) {
-  const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-  if (i >= total) return;
-  const int n = (int)(i % np);
-  float g = 0.0f;
-  if (n < nreal) {
-    const float z = Z[i];
-    const float sg = sg_sigmoid(z);
-    g = dA[i] * sg * (1.0f + z * (1.0f - sg));
+// Column sums of M[rows][np] in two fixed-order stages.  Stage 1: block b
+// takes rows [64 b, 64 b + 64); thread t reads column t % np of every
+// (GB/np)-th of them (all loads issued, then summed in order), and the block
+// adds its GB/np lanes per column into part[b][np].
+__global__ void k_sg_colsum_part(const float *__restrict__ Mx, int rows, int np,
+                                 float *__restrict__ part) {
+  __shared__ float sm[GB];
+  const int t = threadIdx.x, lanes = GB / np, c = t % np, q = t / np;
+  const int r0 = blockIdx.x * 64, per = 64 / lanes;
+  float v[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int r = r0 + q + j * lanes;
+    v[j] = (j < per && r < rows) ? Mx[(size_t)r * np + c] : 0.0f;
   }
-  dZf[i] = g;
-  dZ[i] = (__bf16)g;
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 32; j++) s += v[j];
+  sm[t] = s;
+  __syncthreads();
+  if (t < np) {
+    float a = 0.0f;
+    for (int k = 0; k < lanes; k++) a += sm[k * np + t];
+    part[(size_t)blockIdx.x * np + t] = a;
+  }
 }
 
-// out[n] = sum over rows of M[rows][np] (column sums, one block per column).
-__global__ void k_sg_colsum(const float *__restrict__ Mx, int rows, int np, float *__restrict__ out) {
+// Stage 2, one block per column n: out[n] = the sum of the nparts partials of
+// column n by a fixed tree over the block.
+__global__ void k_sg_colsum_fin(const float *__restrict__ part, int nparts, int np,
+                                float *__restrict__ out) {
   __shared__ float sm[GB];
   const int n = blockIdx.x;
   float s = 0.0f;
-  for (int r = threadIdx.x; r < rows; r += GB) s += Mx[(size_t)r * np + n];
+  for (int z = threadIdx.x; z < nparts; z += GB) s += part[(size_t)z * np + n];
   sm[threadIdx.x] = s;
   __syncthreads();
   for (int o = GB / 2; o > 0; o >>= 1) {
@@ -189,29 +304,77 @@ __global__ void k_sg_colsum(const float *__restrict__ Mx, int rows, int np, floa
   if (threadIdx.x == 0) out[n] = sm[0];
 }
 
-// out[i] = sum_z P[z][i] on the real (k < kreal, n < nreal) entries, 0 else.
+// out[i] = sum_z P[z][i] on the real (k < kreal, n < nreal) entries, 0 else;
+// the loads are issued 16 at a time, the sum taken in z order.
 __global__ void k_sg_reduce_z(const float *__restrict__ P, int nz, int np, int kreal, int nreal,
                               int64_t mn, float *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
   if (i >= mn) return;
   const int n = (int)(i % np), k = (int)(i / np);
   float s = 0.0f;
-  if (k < kreal && n < nreal)
-    for (int z = 0; z < nz; z++) s += P[(int64_t)z * mn + i];
+  if (k < kreal && n < nreal) {
+    int z = 0;
+    for (; z + 16 <= nz; z += 16) {
+      float v[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) v[q] = P[(int64_t)(z + q) * mn + i];
+#pragma unroll
+      for (int q = 0; q < 16; q++) s += v[q];
+    }
+    for (; z < nz; z++) s += P[(int64_t)z * mn + i];
+  }
   out[i] = s;
 }
 
+// Adam's step count and learning rate, in device memory.
+struct TrainState {
+  int64_t t;
+  float lr;
+};
+
+// Advances the Adam step count (after the step's kernels have read it).
+__global__ void k_sg_tick(TrainState *ts) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ts->t += 1;
+}
+
+// The BF16 copies of the parameters are pbf[0, n) in the padded layout and,
+// at pbf[n + ...], every layer's W transposed (W^T[np][kp] at W's offset), so
+// the forward GEMMs read whole 16-byte weight fragments.  The transposed
+// index of padded parameter i, or -1 for a bias.
+__device__ inline int64_t sg_transposed(int64_t i) {
+#pragma unroll
+  for (int l = 0; l < SG_LAYERS; l++) {
+    const int64_t w = LAY_W_OFF(l), nw = (int64_t)sg_kp(l) * sg_np(l);
+    if (i >= w && i < w + nw) {
+      const int64_t o = i - w;
+      return w + (o % sg_np(l)) * sg_kp(l) + o / sg_np(l);
+    }
+  }
+  return -1;
+}
+
+__device__ inline void sg_store_bf16(__bf16 *pbf, int64_t n, int64_t i, float v) {
+  const __bf16 q = (__bf16)v;
+  pbf[i] = q;
+  const int64_t t = sg_transposed(i);
+  if (t >= 0) pbf[n + t] = q;
+}
+
 // Adam on the padded parameter array; padding entries stay zero.  Then the
-// BF16 copy of the weights.
+// BF16 copies of the weights.
+// Step t + 1 of Adam, t and the learning rate read from the training state
+// (k_sg_tick advances t afterwards), so a captured iteration replays as is.
 __global__ void k_sg_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                           float *__restrict__ v, const unsigned char *__restrict__ mask, int64_t n,
-                          float lr, float b1, float b2, float eps, float c1, float c2,
+                          const TrainState *__restrict__ ts, float b1, float b2, float eps,
                           __bf16 *__restrict__ pbf) {
   const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
   if (i >= n) return;
+  const float tt = (float)(ts->t + 1), lr = ts->lr;
+  const float c1 = 1.0f - powf(b1, tt), c2 = 1.0f - powf(b2, tt);
   if (!mask[i]) {
     p[i] = 0.0f;
-    pbf[i] = (__bf16)0.0f;
+    sg_store_bf16(pbf, n, i, 0.0f);
     return;
   }
   const float gi = g[i];
@@ -221,7 +384,7 @@ __global__ void k_sg_adam(float *__restrict__ p, const float *__restrict__ g, fl
   v[i] = vi;
   const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   p[i] = pi;
-  pbf[i] = (__bf16)pi;
+  sg_store_bf16(pbf, n, i, pi);
 }
 
 // The step kernel's LDS image: wt[off(l) + j*ld(l) + k] = W_l[k][j] (BF16),
@@ -237,7 +400,7 @@ __global__ void k_sg_wt_image(const float *__restrict__ p, __bf16 *__restrict__ 
 
 __global__ void k_sg_to_bf16(const float *__restrict__ p, int64_t n, __bf16 *__restrict__ pbf) {
   const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-  if (i < n) pbf[i] = (__bf16)p[i];
+  if (i < n) sg_store_bf16(pbf, n, i, p[i]);
 }
 
 // Per-block partial sums, sums of squares, minima, maxima and sums of |x| of
@@ -295,6 +458,15 @@ struct vg_surr {
   // the weights, and the biases, scaling and envelope (FP32).
   DBuf wt, dep;
   int64_t adam_t = 0;
+  DBuf ts;  // TrainState mirror of adam_t and the learning rate
+  // A captured training iteration (gather, forward, backward, Adam, tick),
+  // valid while the batch size, the dataset (buffers and count) and the work
+  // buffers are those it was captured with.
+  hipGraphExec_t graph = nullptr;
+  int graph_batch = 0;
+  const void *graph_x = nullptr;
+  int64_t graph_count = 0;
+  int work_gen = 0, graph_gen = -1;
   // Scaling and envelope [SG_NIN | SG_NOUT] and whether fitted.
   DBuf scl;  // one vg_surr_scaling
   bool scaled = false;
@@ -303,12 +475,13 @@ struct vg_surr {
   int64_t count = 0, cap = 0;
   // Training work space for batches up to wb samples.
   int wb = 0;
-  DBuf X0, Yn, Zs, As, C, dZf, dZ, P, lpart;
+  DBuf X0, Yn, Zs, As, dZf, dZ, dZ2, P, cpart, lpart;
   int fail(int c, const std::string &m) {
     err = m;
     return c;
   }
   ~vg_surr() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -377,15 +550,17 @@ namespace {
 
 int ensure_work(vg_surr *s, int B) {
   if (B <= s->wb) return VG_OK;
+  s->work_gen++;  // a captured training iteration holds the old buffers
   const size_t b = (size_t)B;
   SCHK(s, s->X0.alloc(b * SG_KIN * 2));
   SCHK(s, s->Yn.alloc(b * SG_NOUT * 4));
   SCHK(s, s->Zs.alloc(b * SG_HID * 4 * SG_LAYERS));
   SCHK(s, s->As.alloc(b * SG_HID * 2 * SG_LAYERS));
-  SCHK(s, s->C.alloc(b * SG_HID * 4));
+  SCHK(s, s->dZ2.alloc(b * SG_HID * 2));
+  SCHK(s, s->cpart.alloc(((b + 63) / 64) * SG_HID * 4));
   SCHK(s, s->dZf.alloc(b * SG_HID * 4));
   SCHK(s, s->dZ.alloc(b * SG_HID * 2));
-  SCHK(s, s->P.alloc((size_t)64 * SG_HID * SG_HID * 4));
+  SCHK(s, s->P.alloc((size_t)TN_CHUNKS * SG_HID * SG_HID * 4));
   SCHK(s, s->lpart.alloc(((b * SG_NOUTP + GB - 1) / GB) * 8));
   s->wb = B;
   return VG_OK;
@@ -408,13 +583,16 @@ int fwd_bwd(vg_surr *s, int B, bool grads, double *sse) {
   }
   for (int l = 0; l < SG_LAYERS; l++) {
     const int kp = sg_kp(l), np = sg_np(l);
-    hipLaunchKernelGGL((k_gemm<false, false>), dim3((B + 127) / 128, np / 32, 1), dim3(256), 0, st,
-                       Al[l], kp, pbf + LAY.w[l], np, s->C.as<float>(), np, B, np, kp, kp,
-                       (int64_t)0);
     const bool last = l == SG_LAYERS - 1;
-    hipLaunchKernelGGL(k_sg_bias_act, dim3(nb((int64_t)B * np)), dim3(GB), 0, st, s->C.as<float>(),
-                       par + LAY.b[l], np, sg_n(l), (int64_t)B * np, last ? 0 : 1, Zl[l],
-                       last ? (__bf16 *)nullptr : Al[l + 1]);
+    Epi ep;
+    ep.bias = par + LAY.b[l];
+    ep.nreal = sg_n(l);
+    ep.act = last ? 0 : 1;
+    ep.Z = Zl[l];
+    ep.A = last ? (__bf16 *)nullptr : Al[l + 1];
+    hipLaunchKernelGGL((k_gemm<false, true, EPI_BIAS_ACT>), dim3((B + 127) / 128, np / 32, 1),
+                       dim3(256), 0, st, Al[l], kp, pbf + LAY.total + LAY.w[l], kp,
+                       (float *)nullptr, np, B, np, kp, kp, (int64_t)0, ep);
   }
   const unsigned lb = nb((int64_t)B * SG_NOUTP);
   hipLaunchKernelGGL(k_sg_loss, dim3(lb), dim3(GB), 0, st, Zl[SG_LAYERS - 1], s->Yn.as<float>(), B,
@@ -424,25 +602,32 @@ int fwd_bwd(vg_surr *s, int B, bool grads, double *sse) {
     for (int l = SG_LAYERS - 1; l >= 0; l--) {
       const int kp = sg_kp(l), np = sg_np(l);
       // dW = A_l^T dZ_l, split over the batch in nz chunks.
-      int nz = std::max(1, std::min(64, B / 256));
-      while (B % (nz * 16)) nz--;
+      int nz = std::max(1, std::min(TN_CHUNKS, B / TN_ROWS));
+      while (B % nz) nz--;
       const int kc = B / nz;
-      hipLaunchKernelGGL((k_gemm<true, false>), dim3((kp + 127) / 128, np / 32, nz), dim3(256), 0,
-                         st, Al[l], kp, s->dZ.as<__bf16>(), np, s->P.as<float>(), np, kp, np, B,
-                         kc, (int64_t)kp * np);
+      hipLaunchKernelGGL(k_gemm_tn, dim3(nz), dim3(256), 0, st, Al[l], kp, s->dZ.as<__bf16>(), np,
+                         kc, s->P.as<float>());
       hipLaunchKernelGGL(k_sg_reduce_z, dim3(nb((int64_t)kp * np)), dim3(GB), 0, st,
                          s->P.as<float>(), nz, np, sg_k(l), sg_n(l), (int64_t)kp * np,
                          g + LAY.w[l]);
-      hipLaunchKernelGGL(k_sg_colsum, dim3(np), dim3(GB), 0, st, s->dZf.as<float>(), B, np,
-                         g + LAY.b[l]);
+      const int parts = (B + 63) / 64;
+      hipLaunchKernelGGL(k_sg_colsum_part, dim3(parts), dim3(GB), 0, st, s->dZf.as<float>(), B,
+                         np, s->cpart.as<float>());
+      hipLaunchKernelGGL(k_sg_colsum_fin, dim3(np), dim3(GB), 0, st, s->cpart.as<float>(), parts,
+                         np, g + LAY.b[l]);
       if (l > 0) {
         // dA_{l-1} = dZ_l W_l^T, then through SiLU'(Z_{l-1}).
-        hipLaunchKernelGGL((k_gemm<false, true>), dim3((B + 127) / 128, kp / 32, 1), dim3(256), 0,
-                           st, s->dZ.as<__bf16>(), np, pbf + LAY.w[l], np, s->C.as<float>(), kp, B,
-                           kp, np, np, (int64_t)0);
-        hipLaunchKernelGGL(k_sg_act_back, dim3(nb((int64_t)B * kp)), dim3(GB), 0, st,
-                           s->C.as<float>(), Zl[l - 1], kp, sg_n(l - 1), (int64_t)B * kp,
-                           s->dZf.as<float>(), s->dZ.as<__bf16>());
+        // (dZ of layer l is read by this GEMM and overwritten by its
+        // epilogue: the next layer's dZ goes to the other buffer.)
+        Epi ep;
+        ep.nreal = sg_n(l - 1);
+        ep.Zin = Zl[l - 1];
+        ep.dZf = s->dZf.as<float>();
+        ep.dZ = s->dZ2.as<__bf16>();
+        hipLaunchKernelGGL((k_gemm<false, true, EPI_ACT_BACK>), dim3((B + 127) / 128, kp / 32, 1),
+                           dim3(256), 0, st, s->dZ.as<__bf16>(), np, pbf + LAY.w[l], np,
+                           (float *)nullptr, kp, B, kp, np, np, (int64_t)0, ep);
+        std::swap(s->dZ.p, s->dZ2.p);
       }
     }
   }
@@ -468,7 +653,8 @@ int load_host_batch(vg_surr *s, const float *x, const float *y, int n) {
   if (y) SCHK(s, hipMemcpy(ys.p, y, (size_t)n * SG_NOUT * 4, hipMemcpyHostToDevice));
   else SCHK(s, hipMemset(ys.p, 0, (size_t)n * SG_NOUT * 4));
   hipLaunchKernelGGL(k_sg_gather, dim3(nb(n)), dim3(GB), 0, s->stream, xs.as<float>(),
-                     ys.as<float>(), (int64_t)n, (uint64_t)0, (int64_t)0, n, SCL(s, mu_x),
+                     ys.as<float>(), (int64_t)n, (uint64_t)0, (const int64_t *)nullptr, (int64_t)0,
+                     n, SCL(s, mu_x),
                      SCL(s, sd_x), SCL(s, mu_y), SCL(s, sd_y),
                      SCL(s, c_y), s->X0.as<__bf16>(), s->Yn.as<float>());
   SCHK(s, hipStreamSynchronize(s->stream));
@@ -510,12 +696,15 @@ int vg_surr_create(uint64_t seed, vg_surr **out) {
     if (b->alloc(n * 4) != hipSuccess || hipMemsetAsync(b->p, 0, n * 4, S->stream) != hipSuccess)
       return VG_ERR_HIP;
   if (hipStreamSynchronize(S->stream) != hipSuccess) return VG_ERR_HIP;
-  if (S->parbf.alloc(n * 2) != hipSuccess || S->mask.alloc(n) != hipSuccess) return VG_ERR_HIP;
+  if (S->parbf.alloc(n * 4) != hipSuccess || S->mask.alloc(n) != hipSuccess) return VG_ERR_HIP;
   if (hipMemcpy(S->par.p, p.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
   if (hipMemcpy(S->mask.p, mk.data(), n, hipMemcpyHostToDevice) != hipSuccess) return VG_ERR_HIP;
   hipLaunchKernelGGL(k_sg_to_bf16, dim3(nb(n)), dim3(GB), 0, S->stream, S->par.as<float>(), n,
                      S->parbf.as<__bf16>());
   if (S->scl.alloc(sizeof(vg_surr_scaling)) != hipSuccess) return VG_ERR_HIP;
+  if (S->ts.alloc(sizeof(TrainState)) != hipSuccess ||
+      hipMemsetAsync(S->ts.p, 0, sizeof(TrainState), S->stream) != hipSuccess)
+    return VG_ERR_HIP;
   if (hipStreamSynchronize(S->stream) != hipSuccess) return VG_ERR_HIP;
   *out = s.release();
   return VG_OK;
@@ -674,6 +863,7 @@ int vg_surr_set_params(vg_surr *s, const float *in) {
                      LAY.total, s->parbf.as<__bf16>());
   SCHK(s, hipStreamSynchronize(s->stream));
   s->adam_t = 0;
+  SCHK(s, hipMemset(s->ts.p, 0, sizeof(TrainState)));
   SCHK(s, hipMemsetAsync(s->m1.p, 0, LAY.total * 4, s->stream));
   SCHK(s, hipMemsetAsync(s->m2.p, 0, LAY.total * 4, s->stream));
   SCHK(s, hipStreamSynchronize(s->stream));
@@ -730,31 +920,63 @@ int vg_surr_train(vg_surr *s, int iters, int batch, double lr, double *loss) {
   if (!s || iters < 1 || batch < 128 || batch % 128 || !s->scaled || s->count < 1) return VG_ERR_ARG;
   int rc = ensure_work(s, batch);
   if (rc) return rc;
-  double acc = 0.0;
-  int nacc = 0;
   const float b1 = 0.9f, b2 = 0.999f;
-  for (int it = 0; it < iters; it++) {
-    const uint64_t seed = s->seed * 0x100000001B3ull + (uint64_t)(s->adam_t + 1) * 0x9E3779B97F4A7C15ull;
+  // The step count and learning rate go to device memory; every iteration
+  // then launches the same kernels with the same arguments.
+  const TrainState ts0{s->adam_t, (float)lr};
+  SCHK(s, hipMemcpyAsync(s->ts.p, &ts0, sizeof ts0, hipMemcpyHostToDevice, s->stream));
+  auto iteration = [&](double *sse) -> int {
     hipLaunchKernelGGL(k_sg_gather, dim3(nb(batch)), dim3(GB), 0, s->stream, s->X.as<float>(),
-                       s->Y.as<float>(), s->count, seed, (int64_t)-1, batch, SCL(s, mu_x),
-                       SCL(s, sd_x), SCL(s, mu_y), SCL(s, sd_y),
-                       SCL(s, c_y), s->X0.as<__bf16>(), s->Yn.as<float>());
-    const bool report = it >= iters - 50;
-    double sse = 0.0;
-    if ((rc = fwd_bwd(s, batch, true, report ? &sse : nullptr))) return rc;
-    if (report) {
-      acc += sse / ((double)batch * SG_NOUT);
-      nacc++;
-    }
-    s->adam_t++;
-    const float c1 = 1.0f - std::pow(b1, (float)s->adam_t), c2 = 1.0f - std::pow(b2, (float)s->adam_t);
+                       s->Y.as<float>(), s->count, s->seed, (const int64_t *)s->ts.p,
+                       (int64_t)-1, batch, SCL(s, mu_x), SCL(s, sd_x), SCL(s, mu_y),
+                       SCL(s, sd_y), SCL(s, c_y), s->X0.as<__bf16>(), s->Yn.as<float>());
+    if (const int r = fwd_bwd(s, batch, true, sse)) return r;
     hipLaunchKernelGGL(k_sg_adam, dim3(nb(LAY.total)), dim3(GB), 0, s->stream, s->par.as<float>(),
                        s->grad.as<float>(), s->m1.as<float>(), s->m2.as<float>(),
-                       s->mask.as<unsigned char>(), LAY.total, (float)lr, b1, b2, 1e-8f, c1, c2,
-                       s->parbf.as<__bf16>());
+                       s->mask.as<unsigned char>(), LAY.total, s->ts.as<TrainState>(), b1, b2,
+                       1e-8f, s->parbf.as<__bf16>());
+    hipLaunchKernelGGL(k_sg_tick, dim3(1), dim3(64), 0, s->stream, s->ts.as<TrainState>());
+    return VG_OK;
+  };
+  // All but the last 50 iterations replay one captured iteration (the host
+  // launches one graph instead of ~35 kernels); the last 50 run eagerly and
+  // report their loss.  (fwd_bwd swaps the two dZ buffers an even number of
+  // times, so a replay finds them where the capture did.)
+  const int tail = std::min(iters, 50);
+  const int replays = iters - tail;
+  if (replays > 0) {
+    if (!s->graph || s->graph_batch != batch || s->graph_x != s->X.p ||
+        s->graph_count != s->count || s->graph_gen != s->work_gen) {
+      if (s->graph) SCHK(s, hipGraphExecDestroy(s->graph));
+      s->graph = nullptr;
+      SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      const int r = iteration(nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      SCHK(s, e);
+      const hipError_t ei = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      SCHK(s, ei);
+      s->graph_batch = batch;
+      s->graph_x = s->X.p;
+      s->graph_count = s->count;
+      s->graph_gen = s->work_gen;
+    }
+    for (int it = 0; it < replays; it++) SCHK(s, hipGraphLaunch(s->graph, s->stream));
   }
+  double acc = 0.0;
+  for (int it = 0; it < tail; it++) {
+    double sse = 0.0;
+    if ((rc = iteration(&sse))) return rc;
+    acc += sse / ((double)batch * SG_NOUT);
+  }
+  s->adam_t += iters;
   SCHK(s, hipStreamSynchronize(s->stream));
-  if (loss) *loss = nacc ? acc / nacc : 0.0;
+  if (loss) *loss = acc / tail;
   return VG_OK;
 }
 
