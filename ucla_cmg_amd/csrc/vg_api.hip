@@ -196,6 +196,10 @@ struct vg_handle {
   // tree fits (pk_ok).
   bool pk_ok = false;
   int pk_ncpl = 0;
+  // Gershgorin interval of the tree block; the Chebyshev form runs when it
+  // is well inside (0, 2) and the direction fits LDS twice.
+  bool pk_cheb = false;
+  double pk_lmin = 0.0, pk_lmax = 0.0;
   DevBuf pk_blk_v, pk_blk_j, pk_cpl_row, pk_cpl_v, pk_cpl_j;
   DevBuf xhist;
   int solves_done[2] = {0, 0};
@@ -397,10 +401,17 @@ template <class T>
 void launch_pk_presolve(vg_handle *h, int which) {
   if (!h->guess || !h->pk_ok) return;
   ProfScope ps(h, VG_KC_PK_PRESOLVE);
-  hipLaunchKernelGGL(k_pk_presolve<T>, dim3(1), dim3(PK_BLK), (unsigned)(h->n_pk * sizeof(T)),
-                     h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
-                     h->pk_blk_j.as<int>(), h->n_myo, (int)h->n_pk, h->pk_cpl_row.as<int64_t>(),
-                     h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(), h->pk_ncpl);
+  if (h->pk_cheb)
+    hipLaunchKernelGGL(k_pk_cheb<T>, dim3(1), dim3(PK_BLK), (unsigned)(2 * h->n_pk * sizeof(T)),
+                       h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
+                       h->pk_blk_j.as<int>(), h->n_myo, (int)h->n_pk,
+                       h->pk_cpl_row.as<int64_t>(), h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(),
+                       h->pk_ncpl, h->pk_lmin, h->pk_lmax);
+  else
+    hipLaunchKernelGGL(k_pk_presolve<T>, dim3(1), dim3(PK_BLK), (unsigned)(h->n_pk * sizeof(T)),
+                       h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
+                       h->pk_blk_j.as<int>(), h->n_myo, (int)h->n_pk, h->pk_cpl_row.as<int64_t>(),
+                       h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(), h->pk_ncpl);
 }
 
 template <class T>
@@ -944,6 +955,17 @@ int setup_pk_presolve(vg_handle *h, const HostSell &sell) {
   HIPCHK(h, h->pk_cpl_row.alloc(std::max<int64_t>(nc, 1) * 8));
   if (nc) HIPCHK(h, hipMemcpy(h->pk_cpl_row.p, myo.data(), nc * 8, hipMemcpyHostToDevice));
   h->pk_ncpl = (int)nc;
+  // Gershgorin: A_pp has a unit diagonal, so its eigenvalues lie within
+  // 1 -+ the largest off-diagonal row sum of |a_ij|.
+  double smax = 0.0;
+  for (int64_t i = 0; i < npk; i++) {
+    double si = 0.0;
+    for (int m = 1; m < PK_MAXE; m++) si += std::fabs(bv[m * npk + i]);
+    smax = std::max(smax, si);
+  }
+  h->pk_lmin = 1.0 - smax;
+  h->pk_lmax = 1.0 + smax;
+  h->pk_cheb = h->pk_lmin >= 0.1 && 2 * npk * (int64_t)h->tsz <= 61440;
   h->pk_ok = true;
   return VG_OK;
 }

@@ -690,6 +690,90 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_presolve(T *__restrict__ r, T *__
   }
 }
 
+// The same local solve as a Chebyshev iteration on the block's eigenvalue
+// interval [lmin, lmax] (Gershgorin bounds of A_pp, computed on the host;
+// Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1): no dot
+// products, so no block reductions -- each iteration is one product with the
+// direction d (neighbours' d from LDS, double-buffered) and one barrier.
+// With [0.51, 1.49] (the LV tree) PK_ITERS = 6 iterations bound the error
+// reduction at 2 / T_6(theta / delta) ~ 1e-3.
+template <class T>
+__global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__restrict__ x,
+                                                    const T *__restrict__ blk_v,
+                                                    const int *__restrict__ blk_j, int64_t r0,
+                                                    int npk, const int64_t *__restrict__ cpl_row,
+                                                    const T *__restrict__ cpl_v,
+                                                    const int *__restrict__ cpl_j, int ncpl,
+                                                    double lmin, double lmax) {
+  extern __shared__ double pk_lds[];
+  T *dl0 = reinterpret_cast<T *>(pk_lds), *dl1 = dl0 + npk;
+  const int tid = threadIdx.x;
+  constexpr int PK_ROWS = pk_rows<T>();
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  T e[PK_ROWS], rr[PK_ROWS], d[PK_ROWS], ev[PK_ROWS][PK_MAXE];
+  int ec[PK_ROWS][PK_MAXE];
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; k++) {
+    const int i = tid + k * PK_BLK;
+    e[k] = rr[k] = d[k] = T(0);
+    const bool on = i < npk;
+#pragma unroll
+    for (int m = 0; m < PK_MAXE; m++) {
+      ev[k][m] = on ? blk_v[(int64_t)m * npk + i] : T(0);
+      ec[k][m] = on ? blk_j[(int64_t)m * npk + i] : 0;
+    }
+    if (on) {
+      rr[k] = r[r0 + i];
+      d[k] = (T)((double)rr[k] / theta);
+      dl0[i] = d[k];
+    }
+  }
+  __syncthreads();
+  double rho = 1.0 / sigma;
+  for (int it = 0; it < PK_ITERS; it++) {
+    const T *dc = (it & 1) ? dl1 : dl0;
+    T *dn = (it & 1) ? dl0 : dl1;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      T a = T(0);
+#pragma unroll
+      for (int m = 0; m < PK_MAXE; m++) a += ev[k][m] * dc[ec[k][m]];
+      e[k] += d[k];
+      rr[k] -= a;
+    }
+    if (it + 1 == PK_ITERS) break;
+    const double rho_n = 1.0 / (2.0 * sigma - rho);
+    const T c0 = (T)(rho_n * rho), c1 = (T)(2.0 * rho_n / delta);
+    rho = rho_n;
+#pragma unroll
+    for (int k = 0; k < PK_ROWS; k++) {
+      const int i = tid + k * PK_BLK;
+      d[k] = c0 * d[k] + c1 * rr[k];
+      if (i < npk) dn[i] = d[k];
+    }
+    // The new direction before the next product; the buffer written next
+    // (dc) is read no more after this barrier.
+    __syncthreads();
+  }
+  __syncthreads();  // every read of the direction buffers is done
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; k++) {
+    const int i = tid + k * PK_BLK;
+    if (i < npk) {
+      x[r0 + i] += e[k];
+      r[r0 + i] = rr[k];
+      dl0[i] = e[k];
+    }
+  }
+  __syncthreads();
+  for (int c = tid; c < ncpl; c += PK_BLK) {
+    T a = T(0);
+#pragma unroll
+    for (int m = 0; m < CPL_MAXE; m++) a += cpl_v[(int64_t)m * ncpl + c] * dl0[cpl_j[(int64_t)m * ncpl + c]];
+    r[cpl_row[c]] -= a;
+  }
+}
+
 // ---- diffusion right-hand side ---------------------------------------------
 // b = K_rhs V and r_0 = b - A_hat x_0 (x_0 = 0 unless GUESS: then x holds the
 // guess of this solve); partials of ||b||_D^2 and ||r_0||_D^2.  Opens a solve:
