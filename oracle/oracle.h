@@ -20,14 +20,34 @@
  *   or_tt_rates        charge conservation across dV/dt and the six
  *                      concentration equations; near-stationarity at the
  *                      reference's initial state; constants and initial
- *                      state against tests/golden/ionic_tables.json
+ *                      state against tests/golden/ionic_tables.json; gate
+ *                      steady states at their published half points and
+ *                      slopes, time constants at exact points, RyR recovery
+ *                      (tests/test_oracle_gate_kinetics.py)
  *   or_tt_compute_ion  backward-Euler gate closed form; Newton fixed point
  *                      solves x - x^n - dt f(x) = 0; J and J^T linear solves
  *                      against numpy.linalg.solve
  *   or_lr_compute_ion  Rush-Larsen gate closed form at fixed V; Ca_i forward
- *                      Euler closed form; constants against the golden table.
- *                      The individual alpha/beta rate expressions of LR are
- *                      parity unpinned beyond rest/upstroke markers.
+ *                      Euler closed form with the slow inward current;
+ *                      constants against the golden table; every alpha/beta
+ *                      of make_parameter1 at its centre, its asymptotic
+ *                      slopes and its guarded singularity's limit; I_Na
+ *                      reversal and conductance; the background K currents
+ *                      vanishing at E_K1 (tests/test_oracle_gate_kinetics.py);
+ *                      the gate table against the formulas
+ *                      (tests/test_oracle_lr_gate.py)
+ *   or_pk_rates        constants, derived constants and initial state against
+ *                      the reference tables; logistic steady states at their
+ *                      half points and slopes, the other steady states and
+ *                      time constants at exact points; guarded singularities;
+ *                      charge conservation; currents vanishing at their
+ *                      reversal potentials; the exchanger; the adaptive
+ *                      sub-steps; AP markers (tests/test_oracle_purkinje.py)
+ *   all four models    a planted-mistake harness: 41 one-spot mutations of
+ *                      oracle_ionic.c (a digit, sign, centre, slope, a dropped
+ *                      current, a stoichiometric factor, a state index), each
+ *                      of which fails at least one pin
+ *                      (tests/test_oracle_mutations.py)
  *   or_ecg_direct      brute-force numpy quadrature sum, uniform V -> 0,
  *                      dipole far-field scaling
  *   or_mj_rates        Ca and Na conservation across the flux terms; the
