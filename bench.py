@@ -615,10 +615,22 @@ def surrogate_bench(args, vg, torch):
     # 3. The three forms over the same beat, host-driven (every solve to its
     # tolerance, checked below).
     runs = {}
-    for name, prec, use in (("fp64", vg.PREC_FP64, False), ("plain", vg.PREC_MIXED, False),
-                            ("surrogate", vg.PREC_MIXED, True)):
+    # Fallback settings judged beside the network alone: "up:V_min:h_min"
+    # (upstroke) and/or "band:V_lo:V_hi", joined by "+".
+    inf = float("inf")
+    forms = [("fp64", vg.PREC_FP64, None), ("plain", vg.PREC_MIXED, None),
+             ("surrogate", vg.PREC_MIXED, {"up": (inf, inf), "band": (inf, -inf)})]
+    for spec in [u for u in args.surr_fallback.split(",") if u]:
+        fb = {"up": (inf, inf), "band": (inf, -inf)}
+        for part in spec.split("+"):
+            kind, a, b = part.split(":")
+            fb[kind] = (float(a), float(b))
+        forms.append((f"surrogate_{spec}", vg.PREC_MIXED, fb))
+    for name, prec, use in forms:
         h = make(prec, 0)
         if use:
+            sur.set_upstroke_fallback(*use["up"])
+            sur.set_band_fallback(*use["band"])
             sur.attach(h)
         wall = timed_run(h, name)
         runs[name] = result(h, wall)
@@ -632,7 +644,7 @@ def surrogate_bench(args, vg, torch):
             runs[name]["ionic_us"] = ionic_us(h)
         h.close()
     ref = runs["fp64"]
-    for name in ("plain", "surrogate"):
+    for name in [f[0] for f in forms[1:]]:
         r = runs[name]
         both = (r["at"] >= 0) & (ref["at"] >= 0)
         dat = np.abs(r["at"][both] - ref["at"][both])
@@ -648,8 +660,14 @@ def surrogate_bench(args, vg, torch):
             "cg": r["cg"],
         }
     out["fp64_cg"] = ref["cg"]
-    u = runs["surrogate"]["usage"]
-    out["surrogate"]["network_frac"] = u[0] / max(1, u[0] + u[1])
+    for name, _, use in forms[2:]:
+        u = runs[name]["usage"]
+        out[name]["network_frac"] = u[0] / max(1, u[0] + u[1])
+        out[name]["fallback"] = {k: list(v) for k, v in use.items() if v[0] != inf}
+        out[name]["gates"] = {"at_le_1ms": out[name]["at_max_err_ms"] <= 1.0,
+                              "ecg_le_3pct": out[name]["ecg_rel_l2"] <= 0.03,
+                              "speedup_ge_2": out["plain"]["ms_per_step"]
+                              / out[name]["ms_per_step"] >= 2.0}
     out["value"] = out["surrogate"]["node_steps_per_s"]
     out["ms_per_step"] = out["surrogate"]["ms_per_step"]
     out["speedup_vs_plain"] = out["plain"]["ms_per_step"] / out["surrogate"]["ms_per_step"]
@@ -791,6 +809,10 @@ def main():
     ap.add_argument("--surr-batch", type=int, default=16384)
     ap.add_argument("--surr-seed", type=int, default=1)
     ap.add_argument("--surr-lr", type=float, default=1e-3, help="peak Adam step (cosine decay)")
+    ap.add_argument("--surr-fallback", default="up:-80:0.05",
+                    help="comma-separated fallback settings judged beside the network alone, "
+                         "each 'up:V_min:h_min' (upstroke) and/or 'band:V_lo:V_hi' joined "
+                         "by '+' (empty: none)")
     ap.add_argument("--surr-kernel-only", action="store_true",
                     help="with --surrogate: time the learned step's kernel alone")
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",

@@ -400,6 +400,15 @@ typedef struct vg_surr_scaling {
   float mu_y[19], sd_y[19], c_y[19], t_lo[19], t_hi[19];
 } vg_surr_scaling;
 int vg_surr_fit_scaling(vg_surr *s, double envelope_margin);
+/* Upstroke fallback: rows whose V exceeds v_min while their sodium
+ * inactivation gate h exceeds h_min (depolarised, the channel not yet
+ * inactivated: the upstroke that sets the activation time) are left to the
+ * plain model.  Off (+inf) by default; takes effect at the next attach. */
+int vg_surr_set_upstroke_fallback(vg_surr *s, double v_min, double h_min);
+/* Voltage-band fallback: rows with v_lo < V < v_hi (e.g. the repolarisation
+ * band) are left to the plain model.  Off (an empty band) by default; takes
+ * effect at the next attach. */
+int vg_surr_set_band_fallback(vg_surr *s, double v_lo, double v_hi);
 int vg_surr_get_scaling(vg_surr *s, vg_surr_scaling *out);
 int vg_surr_set_scaling(vg_surr *s, const vg_surr_scaling *in);
 int vg_surr_num_params(int64_t *n);

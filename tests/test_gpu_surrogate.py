@@ -29,8 +29,10 @@ def _scaled(seed, x, y):
 
 
 def _norm32(x, y, sc):
-    """Network inputs and targets as the device forms them (float32)."""
-    xn = ((x.astype(np.float32) - sc["mu_x"]) / sc["sd_x"]).astype(np.float64)
+    """Network inputs ((x - mu) times the correctly rounded 1 / sd) and
+    targets as the device forms them (float32)."""
+    inv = np.float32(1.0) / sc["sd_x"].astype(np.float32)
+    xn = ((x.astype(np.float32) - sc["mu_x"]) * inv).astype(np.float64)
     t = np.arcsinh(y.astype(np.float32) / sc["c_y"])
     yn = ((t - sc["mu_y"]) / sc["sd_y"]).astype(np.float64)
     return xn, yn
@@ -251,3 +253,68 @@ def test_stimulated_nodes_fall_back(gpu_device):
     used, fb = vg.Surrogate.usage(g)
     assert fb == 10 and used == mesh.n_nodes - 10
     g.close()
+
+
+def test_upstroke_and_band_rows_fall_back(gpu_device):
+    """With the upstroke fallback set, exactly the rows with V above v_min
+    and the h gate above h_min take the plain update (bit for bit), the
+    others keep the network's (unset, the network takes every row); a
+    voltage band adds its rows to the fallback."""
+    mesh, st = _varied_states()
+    v_min, h_min = -70.0, 0.05
+    up = (st[:, 0] > v_min) & (st[:, 8] > h_min)
+    assert 0 < up.sum() < mesh.n_nodes // 2
+    s = vg.Surrogate(6)
+    s.reserve(2 * mesh.n_nodes)
+    g = _isolated(mesh, vg.PREC_MIXED)
+    _load(g, st)
+    s.collect(g, 1)
+    s.fit_scaling(0.5)
+    sc = s.get_scaling()
+    sc["t_lo"][:], sc["t_hi"][:] = -1e30, 1e30
+    s.set_scaling(sc)
+    s.set_params(sr.random_params(23, scale=0.5).astype(np.float32))
+    plain = _isolated(mesh, vg.PREC_MIXED)
+    _load(plain, st)
+    plain.Solve(1)
+    ref = plain.getStateVariables()
+    off = _isolated(mesh, vg.PREC_MIXED)
+    _load(off, st)
+    s.attach(off)
+    off.Solve(1)
+    assert vg.Surrogate.usage(off) == (mesh.n_nodes, 0)
+    net_all = off.getStateVariables()
+    s.set_upstroke_fallback(v_min, h_min)
+    on = _isolated(mesh, vg.PREC_MIXED)
+    _load(on, st)
+    s.attach(on)
+    on.Solve(1)
+    used, fb = vg.Surrogate.usage(on)
+    assert fb == up.sum() and used == (~up).sum()
+    got = on.getStateVariables()
+    # States bit for bit; V up to the ~1e-14 cm^2/ms coupling of the
+    # otherwise isolated nodes (the two runs' fallback rows differ).
+    np.testing.assert_array_equal(got[up, 1:], ref[up, 1:])
+    np.testing.assert_array_equal(got[~up, 1:], net_all[~up, 1:])
+    np.testing.assert_allclose(got[up, 0], ref[up, 0], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(got[~up, 0], net_all[~up, 0], rtol=0, atol=1e-6)
+    # With a voltage band as well, the band's rows fall back too.
+    lo, hi = -10.0, 40.0  # plateau rows (h ~ 0), outside the upstroke test
+    band = (st[:, 0] > lo) & (st[:, 0] < hi)
+    assert (band & ~up).any()
+    s.set_band_fallback(lo, hi)
+    both = _isolated(mesh, vg.PREC_MIXED)
+    _load(both, st)
+    s.attach(both)
+    both.Solve(1)
+    fbk = up | band
+    assert vg.Surrogate.usage(both) == ((~fbk).sum(), fbk.sum())
+    got = both.getStateVariables()
+    np.testing.assert_array_equal(got[fbk, 1:], ref[fbk, 1:])
+    np.testing.assert_array_equal(got[~fbk, 1:], net_all[~fbk, 1:])
+    with pytest.raises(vg.VoomError):
+        s.set_upstroke_fallback(float("nan"), 0.0)
+    with pytest.raises(vg.VoomError):
+        s.set_band_fallback(0.0, float("nan"))
+    for h in (g, plain, off, on, both):
+        h.close()

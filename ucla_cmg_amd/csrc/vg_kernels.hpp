@@ -1245,41 +1245,44 @@ __device__ inline void surr_hand_over(const f32x16 (&acc)[4], const float *bias,
   }
 }
 
-// Output units of lane half H: 8 g + 4 H + r4.  Checks the real ones against
-// their envelope (oc rows: t_lo, t_hi, mu_y, sd_y, c_y) and turns them into
-// increments y (indexed by state).
-template <int H>
-__device__ inline bool surr_outputs(const f32x16 &acc, const float *bias, const float *oc,
-                                    float (&y)[SG_NOUT]) {
+// The last layer's results j = 4 g + r4 of lane half h are output units
+// 8 g + 4 h + r4; slots j < SG_OUT_SLOTS (g < 3) hold every real unit.
+constexpr int SG_OUT_SLOTS = 12;
+// States clamped to [0, 1] after the increment: the gates and R'.
+constexpr uint32_t SG_UNIT_STATES = (0x3Fu << 4) | (0x3Fu << 11) | (1u << 18);
+
+// Checks the real units of lane half h against their envelope (oc rows:
+// t_lo, t_hi, mu_y, sd_y, c_y) and turns every slot into an increment
+// y[j] (padding units: c_y = 0, so 0).  The half is a runtime index, not a
+// template argument: both halves of a wave then run one instruction stream
+// instead of each half's copy under its own mask.
+__device__ inline bool surr_outputs(const f32x16 &acc, const float *bias, const float *oc, int h,
+                                    float (&y)[SG_OUT_SLOTS]) {
   bool ok = true;
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int u = 8 * (j >> 2) + 4 * H + (j & 3);
-    if (u < SG_NOUT) {
-      const float t = acc[j] + bias[u];
-      ok = ok && t >= oc[u] && t <= oc[32 + u];
-      y[u] = sg_from_target(t, oc[64 + u], oc[96 + u], oc[128 + u]);
-    }
+  for (int j = 0; j < SG_OUT_SLOTS; j++) {
+    const int u = 8 * (j >> 2) + 4 * h + (j & 3);
+    const float t = acc[j] + bias[u];
+    ok = ok && (u >= SG_NOUT || (t >= oc[u] && t <= oc[32 + u]));
+    y[j] = sg_from_target(t, oc[64 + u], oc[96 + u], oc[128 + u]);
   }
   return ok;
 }
 
-// New V and states; the start values are read again (cache hits) rather
-// than held across the network.
-template <class T, int H>
-__device__ inline void surr_store(const float (&y)[SG_NOUT], T V, int64_t i, int64_t n_alloc,
-                                  T *__restrict__ Vout, T *__restrict__ st) {
+// New V and states of lane half h's units; the start values are read again
+// (cache hits) rather than held across the network.
+template <class T>
+__device__ inline void surr_store(const float (&y)[SG_OUT_SLOTS], T V, int64_t i, int64_t n_alloc,
+                                  int h, T *__restrict__ Vout, T *__restrict__ st) {
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int k = 8 * (j >> 2) + 4 * H + (j & 3);
-    if (k < SG_NOUT) {
-      T v = (k == 0 ? V : st[(k - 1) * n_alloc + i]) + (T)y[k];
-      if (k == 0) {
-        Vout[i] = v;
-      } else {
-        if ((k >= 4 && k <= 9) || (k >= 11 && k <= 16) || k == 18) v = fmin(fmax(v, T(0)), T(1));
-        st[(k - 1) * n_alloc + i] = v;
-      }
+  for (int j = 0; j < SG_OUT_SLOTS; j++) {
+    const int k = 8 * (j >> 2) + 4 * h + (j & 3);
+    if (k == 0) {
+      Vout[i] = V + (T)y[j];
+    } else if (k < SG_NOUT) {
+      T v = st[(k - 1) * n_alloc + i] + (T)y[j];
+      if ((SG_UNIT_STATES >> k) & 1u) v = fmin(fmax(v, T(0)), T(1));
+      st[(k - 1) * n_alloc + i] = v;
     }
   }
 }
@@ -1301,7 +1304,7 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
   __shared__ __attribute__((aligned(16))) __bf16 wl[SG_WT_TOTAL];
   __shared__ float bl[SG_BIAS_TOTAL];
   __shared__ float oc[5 * 32];
-  __shared__ float ic[4 * 32];  // lo, hi, mu_x, sd_x
+  __shared__ float ic[4 * 32];  // lo, hi, mu_x, 1 / sd_x
   for (int q = threadIdx.x; q < SG_WT_TOTAL / 8; q += SBLK)
     reinterpret_cast<bf16x8 *>(wl)[q] = reinterpret_cast<const bf16x8 *>(sd.wt)[q];
   for (int q = threadIdx.x; q < SG_BIAS_TOTAL; q += SBLK) {
@@ -1316,7 +1319,8 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
   for (int q = threadIdx.x; q < 4 * 32; q += SBLK) {
     const int row = q / 32, u = q % 32;
     const float *src[4] = {sd.lo, sd.hi, sd.mu_x, sd.sd_x};
-    ic[q] = u < SG_NIN ? src[row][u] : 1.0f;
+    const float v = u < SG_NIN ? src[row][u] : 1.0f;
+    ic[q] = row == 3 ? 1.0f / v : v;
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 31, h = l >> 5, kq = h * 8;
@@ -1340,11 +1344,13 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
       f[19] = log_dt;
 #pragma unroll
       for (int k = 0; k < VG_TT_NSCALE; k++) f[20 + k] = (float)sc[k];
-      ok = !(istim && istim[i] != T(0));
+      ok = !(istim && istim[i] != T(0)) && !(f[0] > sd.up_v && f[SG_STATE_H] > sd.up_h) &&
+           !(f[0] > sd.band_lo && f[0] < sd.band_hi);
 #pragma unroll
       for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= ic[k] && f[k] <= ic[32 + k];
 #pragma unroll
-      for (int k = 0; k < SG_KIN; k++) xn[k] = ok && k < SG_NIN ? (f[k] - ic[64 + k]) / ic[96 + k] : 0.0f;
+      for (int k = 0; k < SG_KIN; k++)
+        xn[k] = ok && k < SG_NIN ? sg_norm_in(f[k], ic[64 + k], ic[96 + k]) : 0.0f;
     } else {
 #pragma unroll
       for (int k = 0; k < SG_KIN; k++) xn[k] = 0.0f;
@@ -1368,15 +1374,11 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
       surr_hand_over(acc, bl + L * SG_HID, h, b);
     }
     surr_mm<8, 1>(wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, b, acc);
-    float y[SG_NOUT];
-    const bool ok_out = h ? surr_outputs<1>(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, y)
-                          : surr_outputs<0>(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, y);
+    float y[SG_OUT_SLOTS];
+    const bool ok_out = surr_outputs(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, h, y);
     const int both = __shfl_xor((int)ok_out, 32, 64) & (int)ok_out;
     ok = ok && both;
-    if (mine && ok) {
-      if (h) surr_store<T, 1>(y, V, i, n_alloc, Vout, st);
-      else surr_store<T, 0>(y, V, i, n_alloc, Vout, st);
-    }
+    if (mine && ok) surr_store<T>(y, V, i, n_alloc, h, Vout, st);
     if (mine && h == 0) {
       fallback[i] = ok ? 0 : 1;
       if (ok) n_net++;

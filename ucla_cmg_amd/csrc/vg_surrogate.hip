@@ -227,7 +227,7 @@ __global__ void k_sg_gather(const float *__restrict__ X, const float *__restrict
   const int64_t s = first >= 0 ? first + i : (int64_t)(splitmix(seed + (uint64_t)i) % (uint64_t)Ns);
   for (int f = 0; f < SG_KIN; f++)
     X0[(size_t)i * SG_KIN + f] =
-        (__bf16)(f < SG_NIN ? (X[s * SG_NIN + f] - mu_x[f]) / sd_x[f] : 0.0f);
+        (__bf16)(f < SG_NIN ? sg_norm_in(X[s * SG_NIN + f], mu_x[f], 1.0f / sd_x[f]) : 0.0f);
   for (int f = 0; f < SG_NOUT; f++)
     Yn[(size_t)i * SG_NOUT + f] = sg_to_target(Y[s * SG_NOUT + f], mu_y[f], sd_y[f], c_y[f]);
 }
@@ -470,6 +470,10 @@ struct vg_surr {
   // Scaling and envelope [SG_NIN | SG_NOUT] and whether fitted.
   DBuf scl;  // one vg_surr_scaling
   bool scaled = false;
+  // Upstroke fallback thresholds (V in mV, the h gate); +inf: off.
+  float up_v = INFINITY, up_h = INFINITY;
+  // Voltage-band fallback (mV); empty: off.
+  float band_lo = INFINITY, band_hi = -INFINITY;
   // Dataset (FP32 raw features / increments).
   DBuf X, Y;
   int64_t count = 0, cap = 0;
@@ -541,6 +545,10 @@ bool surr_device(vg_surr *s, SurrDevice *d) {
   d->c_y = field(offsetof(vg_surr_scaling, c_y));
   d->t_lo = field(offsetof(vg_surr_scaling, t_lo));
   d->t_hi = field(offsetof(vg_surr_scaling, t_hi));
+  d->up_v = s->up_v;
+  d->up_h = s->up_h;
+  d->band_lo = s->band_lo;
+  d->band_hi = s->band_hi;
   return true;
 }
 
@@ -808,6 +816,22 @@ int vg_surr_fit_scaling(vg_surr *s, double margin) {
     SCHK(s, hipMemcpy(s->scl.p, &sc, sizeof sc, hipMemcpyHostToDevice));
   }
   s->scaled = true;
+  return VG_OK;
+}
+
+int vg_surr_set_upstroke_fallback(vg_surr *s, double v_min, double h_min) {
+  if (!s) return VG_ERR_ARG;
+  if (std::isnan(v_min) || std::isnan(h_min)) return s->fail(VG_ERR_ARG, "thresholds must not be NaN");
+  s->up_v = (float)v_min;
+  s->up_h = (float)h_min;
+  return VG_OK;
+}
+
+int vg_surr_set_band_fallback(vg_surr *s, double v_lo, double v_hi) {
+  if (!s) return VG_ERR_ARG;
+  if (std::isnan(v_lo) || std::isnan(v_hi)) return s->fail(VG_ERR_ARG, "band edges must not be NaN");
+  s->band_lo = (float)v_lo;
+  s->band_hi = (float)v_hi;
   return VG_OK;
 }
 

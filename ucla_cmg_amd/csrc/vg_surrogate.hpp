@@ -65,14 +65,40 @@ struct SurrDevice {
   const float *mu_y, *sd_y;    // [SG_NOUT] of the transformed outputs
   const float *c_y;            // [SG_NOUT] scale of the asinh transform
   const float *t_lo, *t_hi;    // [SG_NOUT] envelope of the network's outputs
+  // Upstroke fallback: rows with V > up_v and h > up_h (depolarised, the
+  // sodium channel's inactivation gate still open) go to the plain model;
+  // +inf turns it off.
+  float up_v, up_h;
+  // Voltage-band fallback: rows with band_lo < V < band_hi go to the plain
+  // model (an empty band, lo >= hi, turns it off).
+  float band_lo, band_hi;
 };
+// Index of the sodium inactivation gate h among the ten Tusscher states.
+constexpr int SG_STATE_H = 8;
+
+// Normalised input (x - mu) / sd, formed as (x - mu) * (1 / sd) with the
+// reciprocal correctly rounded: training and the step evaluate the same
+// expression, so their BF16 inputs agree bit for bit.
+__device__ inline float sg_norm_in(float x, float mu, float inv_sd) { return (x - mu) * inv_sd; }
+
+// sinh for the output transform: Taylor terms below |u| = 1/16 (truncation
+// below 1e-9 relative), else (e^|u| - e^-|u|) / 2 from the hardware
+// exponential and reciprocal, the sign restored; ~1e-6 relative against
+// sinh, in ~12 instructions where the library's sinhf takes ~70.
+__device__ inline float sg_sinh(float u) {
+  const float a = fabsf(u), a2 = a * a;
+  const float e = __expf(a);
+  const float big = 0.5f * (e - __builtin_amdgcn_rcpf(e));
+  const float small = a * (1.0f + a2 * (1.0f / 6.0f + a2 * (1.0f / 120.0f)));
+  return copysignf(a < 0.0625f ? small : big, u);
+}
 
 // Increment -> network target and back.
 __device__ inline float sg_to_target(float y, float mu, float sd, float c) {
   return (asinhf(y / c) - mu) / sd;
 }
 __device__ inline float sg_from_target(float t, float mu, float sd, float c) {
-  return c * sinhf(t * sd + mu);
+  return c * sg_sinh(t * sd + mu);
 }
 
 // Hooks between the surrogate object and the time step (same library).

@@ -61,7 +61,9 @@ EXPORTED = [
     "vg_cells_get_states", "vg_cells_set_states",
     "vg_surr_create", "vg_surr_destroy", "vg_surr_last_error", "vg_surr_reserve",
     "vg_surr_add_samples", "vg_surr_num_samples", "vg_surr_get_samples", "vg_surr_collect",
-    "vg_surr_fit_scaling", "vg_surr_get_scaling", "vg_surr_set_scaling", "vg_surr_num_params",
+    "vg_surr_fit_scaling", "vg_surr_set_upstroke_fallback", "vg_surr_set_band_fallback",
+    "vg_surr_get_scaling",
+    "vg_surr_set_scaling", "vg_surr_num_params",
     "vg_surr_get_params", "vg_surr_set_params", "vg_surr_loss_grad", "vg_surr_predict",
     "vg_surr_train", "vg_surr_attach", "vg_surr_usage",
     "vg_mesh_read", "vg_meshfile_mesh", "vg_meshfile_free", "vg_mesh_write", "vg_frame_name",
@@ -180,6 +182,8 @@ def load_library(path: str = None):
         "vg_surr_get_samples": ([H, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
         "vg_surr_collect": ([H, H, C.c_int, C.c_int, C.c_int], C.c_int),
         "vg_surr_fit_scaling": ([H, C.c_double], C.c_int),
+        "vg_surr_set_upstroke_fallback": ([H, C.c_double, C.c_double], C.c_int),
+        "vg_surr_set_band_fallback": ([H, C.c_double, C.c_double], C.c_int),
         "vg_surr_get_scaling": ([H, C.c_void_p], C.c_int),
         "vg_surr_set_scaling": ([H, C.c_void_p], C.c_int),
         "vg_surr_num_params": ([C.POINTER(C.c_int64)], C.c_int),
@@ -692,6 +696,19 @@ class Surrogate:
 
     def fit_scaling(self, margin: float = 0.0):
         self._chk(self.L.vg_surr_fit_scaling(self.h, margin), "fit_scaling")
+
+    def set_upstroke_fallback(self, v_min: float = float("inf"), h_min: float = float("inf")):
+        """Leaves rows with V > v_min and the h gate above h_min (the
+        upstroke) to the plain model from the next attach; the defaults
+        turn it off."""
+        self._chk(self.L.vg_surr_set_upstroke_fallback(self.h, float(v_min), float(h_min)),
+                  "set_upstroke_fallback")
+
+    def set_band_fallback(self, v_lo: float = float("inf"), v_hi: float = float("-inf")):
+        """Leaves rows with v_lo < V < v_hi to the plain model from the next
+        attach; the defaults (an empty band) turn it off."""
+        self._chk(self.L.vg_surr_set_band_fallback(self.h, float(v_lo), float(v_hi)),
+                  "set_band_fallback")
 
     def get_scaling(self) -> dict:
         sc = VgSurrScaling()
