@@ -653,6 +653,8 @@ def surrogate_bench(args, vg, torch):
             "ms_per_step": r["wall"] * 1e3 / (T - stim_off),
             "ionic_kernel_us": r["ionic_us"],
             "at_max_err_ms": float(dat.max()), "at_mean_err_ms": float(dat.mean()),
+            "at_p99_err_ms": float(np.percentile(dat, 99)),
+            "at_frac_over_1ms": float((dat > 1.0).mean()),
             "activated_frac": float(both.mean()),
             "activated_mismatch": int(((r["at"] >= 0) != (ref["at"] >= 0)).sum()),
             "ecg_rel_l2": float(np.linalg.norm(r["ecg"] - ref["ecg"]) / np.linalg.norm(ref["ecg"])),
@@ -668,13 +670,15 @@ def surrogate_bench(args, vg, torch):
                               "ecg_le_3pct": out[name]["ecg_rel_l2"] <= 0.03,
                               "speedup_ge_2": out["plain"]["ms_per_step"]
                               / out[name]["ms_per_step"] >= 2.0}
-    out["value"] = out["surrogate"]["node_steps_per_s"]
-    out["ms_per_step"] = out["surrogate"]["ms_per_step"]
-    out["speedup_vs_plain"] = out["plain"]["ms_per_step"] / out["surrogate"]["ms_per_step"]
-    out["ionic_speedup_vs_plain"] = out["plain"]["ionic_kernel_us"] / out["surrogate"]["ionic_kernel_us"]
-    out["gates"] = {"at_le_1ms": out["surrogate"]["at_max_err_ms"] <= 1.0,
-                    "ecg_le_3pct": out["surrogate"]["ecg_rel_l2"] <= 0.03,
-                    "speedup_ge_2": out["speedup_vs_plain"] >= 2.0}
+    # The headline form: the first fallback setting given (the default is the
+    # upstroke fallback), else the network alone.
+    main_form = forms[3][0] if len(forms) > 3 else "surrogate"
+    out["form"] = main_form
+    out["value"] = out[main_form]["node_steps_per_s"]
+    out["ms_per_step"] = out[main_form]["ms_per_step"]
+    out["speedup_vs_plain"] = out["plain"]["ms_per_step"] / out[main_form]["ms_per_step"]
+    out["ionic_speedup_vs_plain"] = out["plain"]["ionic_kernel_us"] / out[main_form]["ionic_kernel_us"]
+    out["gates"] = out[main_form]["gates"]
     out["config"] = {"workload": f"tT2006 LV shell + Purkinje tree ({args.purkinje}), {n} nodes, "
                                  f"{args.surr_ms:g} ms beat, dt {dt} ms, network 25-128x4-19 SiLU BF16",
                      "sample_every_steps": args.surr_every, "sample_node_stride": stride}
