@@ -813,7 +813,7 @@ def main():
     ap.add_argument("--surr-batch", type=int, default=16384)
     ap.add_argument("--surr-seed", type=int, default=1)
     ap.add_argument("--surr-lr", type=float, default=1e-3, help="peak Adam step (cosine decay)")
-    ap.add_argument("--surr-fallback", default="up:-80:0.05",
+    ap.add_argument("--surr-fallback", default="up:-80:0.05+band:-200:-80,up:-80:0.05",
                     help="comma-separated fallback settings judged beside the network alone, "
                          "each 'up:V_min:h_min' (upstroke) and/or 'band:V_lo:V_hi' joined "
                          "by '+' (empty: none)")
