@@ -45,6 +45,11 @@ BEAT_MS = 400.0
 TRUE_RESIDUAL_EVERY = 200
 
 
+# Multiply-adds per node of the learned step as the MFMA tiles run it: layer 0
+# 64 x 128 (25 inputs as BF16 his and los, padded), three 128 x 128, 128 x 32.
+SURR_MACS = 64 * 128 + 3 * 128 * 128 + 128 * 32
+
+
 def lv_workload(scale: float = 1.0, purkinje: str = "high"):
     from ucla_cmg_amd import meshgen
     return meshgen.lv_workload(scale, purkinje)
@@ -487,7 +492,7 @@ def surrogate_kernel_timing(args, vg):
     print(json.dumps({"metric": "learned ionic step kernel time, 1.03M-node LV", "unit": "us",
                       "plain_ionic_us": plain, "surrogate_ionic_us": surr,
                       "network_rows": used, "fallback_rows": fb,
-                      "surrogate_tflops": 2 * 57344 * int(mesh.meta.get("n_myocardium", n))
+                      "surrogate_tflops": 2 * SURR_MACS * int(mesh.meta.get("n_myocardium", n))
                       / (surr * 1e-6) / 1e12}))
 
 
@@ -611,7 +616,7 @@ def surrogate_bench(args, vg, torch):
     train_s = time.perf_counter() - t0
     out["train"] = {"iterations": its, "batch": args.surr_batch, "seconds": train_s, "lr": args.surr_lr,
                     "loss_curve": curve,
-                    "tflops": 3 * 2 * 57344 * its * args.surr_batch / train_s / 1e12}
+                    "tflops": 3 * 2 * SURR_MACS * its * args.surr_batch / train_s / 1e12}
     # 3. The three forms over the same beat, host-driven (every solve to its
     # tolerance, checked below).
     runs = {}
@@ -813,7 +818,7 @@ def main():
     ap.add_argument("--surr-batch", type=int, default=16384)
     ap.add_argument("--surr-seed", type=int, default=1)
     ap.add_argument("--surr-lr", type=float, default=1e-3, help="peak Adam step (cosine decay)")
-    ap.add_argument("--surr-fallback", default="up:-80:0.05+band:-200:-80,up:-80:0.05",
+    ap.add_argument("--surr-fallback", default="up:-80:0.05,up:-80:0.05+band:-200:-80",
                     help="comma-separated fallback settings judged beside the network alone, "
                          "each 'up:V_min:h_min' (upstroke) and/or 'band:V_lo:V_hi' joined "
                          "by '+' (empty: none)")

@@ -371,7 +371,8 @@ int vg_ecg_leadfield(const vg_mesh *mesh, unsigned quirks, const double *electro
 /* ---- learned ten Tusscher ionic step ------------------------------------
  * No counterpart in the reference: a small MLP (25 inputs: V and the 18
  * other tT states at the start of the ionic update, log dt, the five
- * conductance scalings; 4 hidden layers of 128, SiLU; 19 state increments)
+ * conductance scalings, each entered as two BF16 halves; 4 hidden layers of
+ * 128, SiLU; 19 state increments)
  * trained on the GPU with BF16 MFMA kernels on samples recorded from the
  * plain Rush-Larsen model in tissue, and run in place of that model per node
  * while the node's inputs lie inside the training envelope (the plain model
@@ -412,7 +413,8 @@ int vg_surr_set_band_fallback(vg_surr *s, double v_lo, double v_hi);
 int vg_surr_get_scaling(vg_surr *s, vg_surr_scaling *out);
 int vg_surr_set_scaling(vg_surr *s, const vg_surr_scaling *in);
 int vg_surr_num_params(int64_t *n);
-/* per layer W[k][n] row-major then b[n]; layers 25->128->128->128->128->19 */
+/* per layer W[k][n] row-major then b[n]; layers 50->128->128->128->128->19
+ * (layer 0's rows: the 25 inputs' BF16 high halves, then their low halves) */
 int vg_surr_get_params(vg_surr *s, float *p);
 int vg_surr_set_params(vg_surr *s, const float *p);
 /* mean squared error of the transformed outputs t and its gradient (n a

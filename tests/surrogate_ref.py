@@ -1,16 +1,18 @@
 """NumPy restatement of the learned ten Tusscher step's network, used by the
 tests as the checker of the device's training and inference kernels.
 
-Layers 25 -> 128 -> 128 -> 128 -> 128 -> 19, SiLU between them, loss the mean
+Layers 50 -> 128 -> 128 -> 128 -> 128 -> 19 (layer 0 reading the 25 normalised
+inputs as two halves each), SiLU between them, loss the mean
 over batch and outputs of the squared error of the normalised increments.
 With ``bf16=True`` the values are rounded to bfloat16 (nearest even) exactly
-where the device kernels store bfloat16: the normalised inputs, the weights,
+where the device kernels store bfloat16: the normalised inputs (as two
+halves, hi = bf16(x) and lo = bf16(x - hi)), the weights,
 the hidden activations and the output gradients fed to the backward GEMMs;
 sums stay in float64 here (float32 on the device).  With ``bf16=False`` the
 whole computation is float64, which the finite-difference test pins."""
 import numpy as np
 
-LAYERS = [(25, 128), (128, 128), (128, 128), (128, 128), (128, 19)]
+LAYERS = [(50, 128), (128, 128), (128, 128), (128, 128), (128, 19)]
 
 
 def bf16(a):
@@ -45,10 +47,23 @@ def normalise(x, y, sc):
     return xn, yn
 
 
+def split(xn, bf=False):
+    """Layer 0's inputs: each normalised input as two halves, his then los.
+    With bf16 rounding, hi = bf16(x) and lo = bf16(x - hi) (x - hi in
+    float32, where it is exact); without it, hi = x and lo = 0."""
+    if not bf:
+        x = np.asarray(xn, np.float64)
+        return np.concatenate([x, np.zeros_like(x)], axis=1)
+    x = np.asarray(xn, np.float32)
+    hi = bf16(x)
+    lo = bf16((x - hi.astype(np.float32)).astype(np.float32))
+    return np.concatenate([hi, lo], axis=1)
+
+
 def forward(p, xn, bf=False):
     """Pre-activations Z[l] and layer inputs A[l] of normalised inputs xn."""
     r = bf16 if bf else (lambda a: a)
-    A, Z = [r(xn)], []
+    A, Z = [split(xn, bf)], []
     layers = unpack(p)
     for l, (W, b) in enumerate(layers):
         z = A[-1] @ r(W) + b

@@ -47,3 +47,21 @@ def test_layout_matches_library_parameter_count():
     n = sum(k * m + m for k, m in sr.LAYERS)
     assert n == vg.Surrogate.num_params()
     assert [(k, m) for k, m in sr.LAYERS] == vg.SURR_LAYERS
+
+
+def test_input_halves_carry_16_bits():
+    """Layer 0's two BF16 halves of an input: hi = bf16(x), lo = bf16(x - hi);
+    hi alone is within 2^-8 of x relative (8 significant bits), hi + lo
+    within 2^-16 (lo rounds a remainder at most 2^-8 of x); without rounding
+    the halves are (x, 0)."""
+    rng = np.random.default_rng(7)
+    x = (rng.normal(size=(2000, 25)) * 10.0 ** rng.integers(-3, 3, (2000, 25))).astype(np.float32)
+    h = sr.split(x, bf=True)
+    hi, lo = h[:, :25], h[:, 25:]
+    np.testing.assert_array_equal(hi, sr.bf16(x))
+    xd = x.astype(np.float64)
+    rel_hi = np.abs(hi - xd) / np.abs(xd)
+    rel = np.abs(hi + lo - xd) / np.abs(xd)
+    assert rel_hi.max() <= 2.0 ** -8 and rel.max() <= 2.0 ** -16
+    assert rel_hi.max() > 2.0 ** -10  # hi alone is visibly coarser
+    np.testing.assert_array_equal(sr.split(x, bf=False), np.concatenate([xd, 0 * xd], axis=1))

@@ -1331,7 +1331,7 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
     const int64_t i = tile * 32 + r;
     const bool mine = i < n;
     T V = T(0);
-    float xn[SG_KIN];
+    float xn[SG_NIN];
     bool ok = false;
     if (mine) {
       V = Vin[i];
@@ -1349,24 +1349,24 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
 #pragma unroll
       for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= ic[k] && f[k] <= ic[32 + k];
 #pragma unroll
-      for (int k = 0; k < SG_KIN; k++)
-        xn[k] = ok && k < SG_NIN ? sg_norm_in(f[k], ic[64 + k], ic[96 + k]) : 0.0f;
+      for (int k = 0; k < SG_NIN; k++) xn[k] = ok ? sg_norm_in(f[k], ic[64 + k], ic[96 + k]) : 0.0f;
     } else {
 #pragma unroll
-      for (int k = 0; k < SG_KIN; k++) xn[k] = 0.0f;
+      for (int k = 0; k < SG_NIN; k++) xn[k] = 0.0f;
     }
-    // Layer 0 operand: features 16 ks + 8 h + q.
+    // Layer 0 operand: input values 16 ks + 8 h + q (his, los, zeros).
     bf16x8 b[8];
 #pragma unroll
-    for (int ks = 0; ks < 2; ks++) {
+    for (int ks = 0; ks < SG_KIN / 16; ks++) {
       float v[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = h ? xn[16 * ks + 8 + q] : xn[16 * ks + q];
+      for (int q = 0; q < 8; q++)
+        v[q] = h ? sg_in_value(xn, 16 * ks + 8 + q) : sg_in_value(xn, 16 * ks + q);
       b[ks] = sg_frag(sg_pack2(v[0], v[1]), sg_pack2(v[2], v[3]), sg_pack2(v[4], v[5]),
                       sg_pack2(v[6], v[7]));
     }
     f32x16 acc[4];
-    surr_mm<2, 4>(wl + sg_wt_off(0), sg_wt_ld(0), r, kq, b, acc);
+    surr_mm<SG_KIN / 16, 4>(wl + sg_wt_off(0), sg_wt_ld(0), r, kq, b, acc);
     surr_hand_over(acc, bl, h, b);
 #pragma unroll 1
     for (int L = 1; L < SG_LAYERS - 1; L++) {

@@ -212,7 +212,8 @@ __device__ inline uint64_t splitmix(uint64_t x) {
 }
 
 // Batch of B samples (hash-drawn from Ns, or rows first.. when idx_seed is 0
-// and first >= 0), normalised: X0[B][32] BF16 (zero padded), Yn[B][19].
+// and first >= 0), normalised: X0[B][64] BF16 (layer 0's values: his, los, zeros),
+// Yn[B][19].
 __global__ void k_sg_gather(const float *__restrict__ X, const float *__restrict__ Y, int64_t Ns,
                             uint64_t seed, const int64_t *__restrict__ step, int64_t first, int B,
                             const float *__restrict__ mu_x,
@@ -225,9 +226,9 @@ __global__ void k_sg_gather(const float *__restrict__ X, const float *__restrict
   // Training batches: the seed of Adam step t + 1 (step holds t).
   if (step) seed = seed * 0x100000001B3ull + (uint64_t)(*step + 1) * 0x9E3779B97F4A7C15ull;
   const int64_t s = first >= 0 ? first + i : (int64_t)(splitmix(seed + (uint64_t)i) % (uint64_t)Ns);
-  for (int f = 0; f < SG_KIN; f++)
-    X0[(size_t)i * SG_KIN + f] =
-        (__bf16)(f < SG_NIN ? sg_norm_in(X[s * SG_NIN + f], mu_x[f], 1.0f / sd_x[f]) : 0.0f);
+  float xn[SG_NIN];
+  for (int f = 0; f < SG_NIN; f++) xn[f] = sg_norm_in(X[s * SG_NIN + f], mu_x[f], 1.0f / sd_x[f]);
+  for (int f = 0; f < SG_KIN; f++) X0[(size_t)i * SG_KIN + f] = (__bf16)sg_in_value(xn, f);
   for (int f = 0; f < SG_NOUT; f++)
     Yn[(size_t)i * SG_NOUT + f] = sg_to_target(Y[s * SG_NOUT + f], mu_y[f], sd_y[f], c_y[f]);
 }

@@ -6,7 +6,10 @@
 //   inputs  (25): V and the 18 other states at the start of the ionic update,
 //                 log(dt), and the node's five conductance scalings
 //                 (gNa, gKr, gKs, gCaL, gto), normalised by the training data's
-//                 per-feature mean and spread;
+//                 per-feature mean and spread, and entered as two BF16 halves
+//                 (x = hi + lo, hi = bf16(x), lo = bf16(x - hi): ~16 bits of
+//                 each input instead of 8, so the network sees the plateau's
+//                 small changes of V), layer 0 reading 50 values;
 //   hidden      : 4 layers of 128 with SiLU;
 //   outputs (19): the state increments y through the transform
 //                 t = (asinh(y / c) - mu) / sd per output, c a small scale
@@ -14,8 +17,9 @@
 //                 error, so the small increments of the plateau and of rest
 //                 are fitted as well as the upstroke's large ones.
 // Every layer is a BF16 GEMM on MFMA (v_mfma_f32_32x32x16_bf16) with FP32
-// accumulation; widths are padded to the MFMA tile (25 -> 32 inputs, 19 -> 32
-// outputs, the padding weights kept at zero).
+// accumulation; widths are padded to the MFMA tile (layer 0's 50 input values
+// -> 64: his 0-24, los 25-49, zeros; 19 -> 32 outputs; the padding weights
+// kept at zero).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,12 +28,13 @@
 namespace vg {
 
 constexpr int SG_NIN = 25, SG_NOUT = 19, SG_HID = 128, SG_LAYERS = 5;
-constexpr int SG_KIN = 32, SG_NOUTP = 32;
+constexpr int SG_NIN2 = 2 * SG_NIN;  // layer 0's input values: his, then los
+constexpr int SG_KIN = 64, SG_NOUTP = 32;
 // Padded (K, N) of each layer.
 __host__ __device__ constexpr int sg_kp(int l) { return l == 0 ? SG_KIN : SG_HID; }
 __host__ __device__ constexpr int sg_np(int l) { return l == SG_LAYERS - 1 ? SG_NOUTP : SG_HID; }
 // Unpadded (K, N).
-__host__ __device__ constexpr int sg_k(int l) { return l == 0 ? SG_NIN : SG_HID; }
+__host__ __device__ constexpr int sg_k(int l) { return l == 0 ? SG_NIN2 : SG_HID; }
 __host__ __device__ constexpr int sg_n(int l) { return l == SG_LAYERS - 1 ? SG_NOUT : SG_HID; }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -80,6 +85,27 @@ constexpr int SG_STATE_H = 8;
 // reciprocal correctly rounded: training and the step evaluate the same
 // expression, so their BF16 inputs agree bit for bit.
 __device__ inline float sg_norm_in(float x, float mu, float inv_sd) { return (x - mu) * inv_sd; }
+
+// A normalised input as layer 0 takes it: hi = bf16(x) and lo = bf16(x - hi)
+// (x - hi is exact in FP32).  Training and the step share this.
+__device__ inline void sg_split(float x, float &hi, float &lo) {
+  hi = (float)(__bf16)x;
+  lo = (float)(__bf16)(x - hi);
+}
+// Layer 0 input value f (0 <= f < SG_KIN) from the normalised inputs xn.
+template <class XN>
+__device__ inline float sg_in_value(const XN &xn, int f) {
+  float hi, lo;
+  if (f < SG_NIN) {
+    sg_split(xn[f], hi, lo);
+    return hi;
+  }
+  if (f < SG_NIN2) {
+    sg_split(xn[f - SG_NIN], hi, lo);
+    return lo;
+  }
+  return 0.0f;
+}
 
 // sinh for the output transform: Taylor terms below |u| = 1/16 (truncation
 // below 1e-9 relative), else (e^|u| - e^-|u|) / 2 from the hardware

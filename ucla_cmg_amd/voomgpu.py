@@ -73,7 +73,8 @@ EXPORTED = [
 # Learned ten Tusscher step: inputs V, 18 states, log(dt), 5 scalings;
 # outputs the 19 increments; 4 hidden layers of 128.
 SURR_NIN, SURR_NOUT, SURR_HID = 25, 19, 128
-SURR_LAYERS = [(25, 128), (128, 128), (128, 128), (128, 128), (128, 19)]
+# Layer 0 reads each of the 25 normalised inputs as two BF16 halves (his, then los).
+SURR_LAYERS = [(50, 128), (128, 128), (128, 128), (128, 128), (128, 19)]
 
 
 class VgMesh(C.Structure):
