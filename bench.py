@@ -603,6 +603,16 @@ def surrogate_bench(args, vg, torch):
     out["collect_s"] = time.perf_counter() - t0
     # 2. Training (cosine learning-rate decay).
     sur.fit_scaling(0.02)
+    if args.surr_v_weight != 1.0:
+        # V's transformed increment spread by k: its squared error weighs k^2 in the
+        # loss (the step's inverse transform uses the same spread; its output
+        # envelope, in the same units, scales with it).
+        sc = sur.get_scaling()
+        k = float(args.surr_v_weight)
+        sc["sd_y"][0] /= k
+        sc["t_lo"][0] *= k
+        sc["t_hi"][0] *= k
+        sur.set_scaling(sc)
     t0 = time.perf_counter()
     its, done, curve = args.surr_iters, 0, []
     while done < its:
@@ -684,6 +694,7 @@ def surrogate_bench(args, vg, torch):
     out["speedup_vs_plain"] = out["plain"]["ms_per_step"] / out[main_form]["ms_per_step"]
     out["ionic_speedup_vs_plain"] = out["plain"]["ionic_kernel_us"] / out[main_form]["ionic_kernel_us"]
     out["gates"] = out[main_form]["gates"]
+    out["train"]["v_weight"] = args.surr_v_weight
     out["config"] = {"workload": f"tT2006 LV shell + Purkinje tree ({args.purkinje}), {n} nodes, "
                                  f"{args.surr_ms:g} ms beat, dt {dt} ms, network 25-128x4-19 SiLU BF16",
                      "sample_every_steps": args.surr_every, "sample_node_stride": stride}
@@ -822,6 +833,8 @@ def main():
                     help="comma-separated fallback settings judged beside the network alone, "
                          "each 'up:V_min:h_min' (upstroke) and/or 'band:V_lo:V_hi' joined "
                          "by '+' (empty: none)")
+    ap.add_argument("--surr-v-weight", type=float, default=1.0,
+                    help="weight k of V's output in the training loss (as k^2)")
     ap.add_argument("--surr-kernel-only", action="store_true",
                     help="with --surrogate: time the learned step's kernel alone")
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",
