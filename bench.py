@@ -613,23 +613,6 @@ def surrogate_bench(args, vg, torch):
         sc["t_lo"][0] *= k
         sc["t_hi"][0] *= k
         sur.set_scaling(sc)
-    t0 = time.perf_counter()
-    its, done, curve = args.surr_iters, 0, []
-    while done < its:
-        m = min(2000, its - done)
-        lr = 2e-5 + 0.5 * (args.surr_lr - 2e-5) * (1 + math.cos(math.pi * done / its))
-        loss = sur.train(m, batch=args.surr_batch, lr=lr)
-        done += m
-        if done % 20000 == 0 or done == its:
-            curve.append((done, loss))
-            log(f"train: {done}/{its} loss {loss:.3e}")
-    train_s = time.perf_counter() - t0
-    out["train"] = {"iterations": its, "batch": args.surr_batch, "seconds": train_s, "lr": args.surr_lr,
-                    "loss_curve": curve,
-                    "tflops": 3 * 2 * SURR_MACS * its * args.surr_batch / train_s / 1e12}
-    # 3. The three forms over the same beat, host-driven (every solve to its
-    # tolerance, checked below).
-    runs = {}
     # Fallback settings judged beside the network alone: "up:V_min:h_min"
     # (upstroke) and/or "band:V_lo:V_hi", joined by "+".
     inf = float("inf")
@@ -641,6 +624,82 @@ def surrogate_bench(args, vg, torch):
             kind, a, b = part.split(":")
             fb[kind] = (float(a), float(b))
         forms.append((f"surrogate_{spec}", vg.PREC_MIXED, fb))
+    head_fb = forms[3][2] if len(forms) > 3 else forms[2][2]
+
+    # Checkpoint selection (--surr-select): a sweep-generator slab that neither
+    # training nor the judge sees, run coupled with the headline form against
+    # the plain mixed model; the training checkpoint with the lowest ECG error
+    # there is the one judged.
+    val = None
+    if args.surr_select:
+        from ucla_cmg_amd import meshgen
+        vmeshes, vscales, _ = meshgen.slab_sweep()
+        vk = 30  # 38,025 nodes, not among the training slabs (every 4th)
+        vmesh = vmeshes[vk]
+        vrow = np.array([[vscales[vk][0], vscales[vk][1], scales[12][2], vscales[vk][3],
+                          scales[12][4]]])
+        lo_, hi_ = vmesh.xyz.min(axis=0), vmesh.xyz.max(axis=0)
+        ctr = 0.5 * (lo_ + hi_)
+        vel = np.vstack([meshgen.electrodes_around(ctr, 2.0 * float((hi_ - lo_)[:2].max()), 8),
+                         [ctr + [0, 0, 2.0], ctr - [0, 0, 2.0]]])
+        vT = int(round(args.surr_val_ms / dt))
+        dl = MODEL_KW["D"][0]
+
+        def val_run(net):
+            g = vg.CardiacSolver(vmesh, cell_model=vg.MODEL_TT, precision=vg.PREC_MIXED,
+                                 integrator=vg.INTEG_RUSH_LARSEN, dt=dt, D=(dl, dl, dl),
+                                 cg_tol=1e-7, param_scales=vrow,
+                                 param_index=np.zeros(vmesh.n_nodes, np.uint16), cg_x0=vg.X0_LAST)
+            g.enableActivation(-40.0)
+            g.setupECG(vel)
+            g.recordECG(50, vT // 50 + 2)
+            if net:
+                sur.set_upstroke_fallback(*head_fb["up"])
+                sur.set_band_fallback(*head_fb["band"])
+                sur.attach(g)
+            g.setStimulusCurrent(meshgen.slab_corner_nodes(*vmesh.meta["shape"], 3), 50.0 * 2000.0)
+            g.Solve(stim_off)
+            g.clearStimulus()
+            g.Solve(vT - stim_off)
+            r = dict(at=g.activationTimes(), ecg=g.ecgFrames())
+            g.close()
+            return r
+
+        vref = val_run(False)
+        val = {"slab": vk, "nodes": vmesh.n_nodes, "ms": args.surr_val_ms, "checks": []}
+        best = None
+
+    t0 = time.perf_counter()
+    its, done, curve = args.surr_iters, 0, []
+    while done < its:
+        m = min(2000, its - done)
+        lr = 2e-5 + 0.5 * (args.surr_lr - 2e-5) * (1 + math.cos(math.pi * done / its))
+        loss = sur.train(m, batch=args.surr_batch, lr=lr)
+        done += m
+        if done % 20000 == 0 or done == its:
+            curve.append((done, loss))
+            log(f"train: {done}/{its} loss {loss:.3e}")
+            if val is not None and done >= its // 4:
+                r = val_run(True)
+                both = (r["at"] >= 0) & (vref["at"] >= 0)
+                e = float(np.linalg.norm(r["ecg"] - vref["ecg"]) / np.linalg.norm(vref["ecg"]))
+                a = float(np.abs(r["at"][both] - vref["at"][both]).max()) if both.any() else 1e9
+                val["checks"].append({"iterations": done, "loss": loss, "ecg_rel_l2": e,
+                                      "at_max_err_ms": a})
+                log(f"validation at {done}: ECG {e:.4f}, max dAT {a:.3f} ms")
+                if best is None or e < best[0]:
+                    best = (e, done, sur.get_params())
+    train_s = time.perf_counter() - t0
+    out["train"] = {"iterations": its, "batch": args.surr_batch, "seconds": train_s, "lr": args.surr_lr,
+                    "loss_curve": curve,
+                    "tflops": 3 * 2 * SURR_MACS * its * args.surr_batch / train_s / 1e12}
+    if val is not None:
+        sur.set_params(best[2])
+        val["chosen_iterations"] = best[1]
+        out["train"]["selection"] = val
+    # 3. The three forms over the same beat, host-driven (every solve to its
+    # tolerance, checked below).
+    runs = {}
     for name, prec, use in forms:
         h = make(prec, 0)
         if use:
@@ -833,6 +892,10 @@ def main():
                     help="comma-separated fallback settings judged beside the network alone, "
                          "each 'up:V_min:h_min' (upstroke) and/or 'band:V_lo:V_hi' joined "
                          "by '+' (empty: none)")
+    ap.add_argument("--surr-select", action="store_true",
+                    help="judge the training checkpoint with the lowest ECG error on a held-out "
+                         "sweep slab (checked every 20k iterations from a quarter of training)")
+    ap.add_argument("--surr-val-ms", type=float, default=300.0)
     ap.add_argument("--surr-v-weight", type=float, default=1.0,
                     help="weight k of V's output in the training loss (as k^2)")
     ap.add_argument("--surr-kernel-only", action="store_true",
