@@ -15,8 +15,11 @@ from ucla_cmg_amd import voomgpu as vg
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
-ECG_TOOL = os.path.join(os.path.dirname(HERE), "ucla_cmg_amd", "voomgpu_ecg")
-EP_TOOL = os.path.join(os.path.dirname(HERE), "ucla_cmg_amd", "voomgpu_ep")
+# $VOOMGPU_TOOLS names another build of the two drivers (the AddressSanitizer
+# one of the Makefile's asan target, say).
+TOOLS = os.environ.get("VOOMGPU_TOOLS") or os.path.join(os.path.dirname(HERE), "ucla_cmg_amd")
+ECG_TOOL = os.path.join(TOOLS, "voomgpu_ecg")
+EP_TOOL = os.path.join(TOOLS, "voomgpu_ep")
 
 
 def test_reads_the_hand_written_reference_files():

@@ -14,8 +14,10 @@ from ucla_cmg_amd import voomgpu as vg
 
 pytestmark = pytest.mark.gpu
 
-EP_TOOL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                       "ucla_cmg_amd", "voomgpu_ep")
+# $VOOMGPU_TOOLS names another build of the driver (the AddressSanitizer one
+# of the Makefile's asan target, say).
+EP_TOOL = os.path.join(os.environ.get("VOOMGPU_TOOLS") or os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ucla_cmg_amd"), "voomgpu_ep")
 DL, DT = 1.2 / 1400.0, 0.2 / 1400.0
 NX, NY, NZ = 12, 10, 3
 
