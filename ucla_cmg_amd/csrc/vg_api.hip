@@ -200,6 +200,7 @@ struct vg_handle {
   // is well inside (0, 2) and the direction fits LDS twice.
   bool pk_cheb = false;
   double pk_lmin = 0.0, pk_lmax = 0.0;
+  PkCheb pk_coef{};  // Chebyshev coefficients of [pk_lmin, pk_lmax]
   DevBuf pk_blk_v, pk_blk_j, pk_cpl_row, pk_cpl_v, pk_cpl_j;
   DevBuf xhist;
   int solves_done[2] = {0, 0};
@@ -406,7 +407,7 @@ void launch_pk_presolve(vg_handle *h, int which) {
                        h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
                        h->pk_blk_j.as<int>(), h->n_myo, (int)h->n_pk,
                        h->pk_cpl_row.as<int64_t>(), h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(),
-                       h->pk_ncpl, h->pk_lmin, h->pk_lmax);
+                       h->pk_ncpl, h->pk_coef);
   else
     hipLaunchKernelGGL(k_pk_presolve<T>, dim3(1), dim3(PK_BLK), (unsigned)(h->n_pk * sizeof(T)),
                        h->stream, h->r.as<T>(), xbuf<T>(h, which), h->pk_blk_v.as<T>(),
@@ -966,6 +967,7 @@ int setup_pk_presolve(vg_handle *h, const HostSell &sell) {
   h->pk_lmin = 1.0 - smax;
   h->pk_lmax = 1.0 + smax;
   h->pk_cheb = h->pk_lmin >= 0.1 && 2 * npk * (int64_t)h->tsz <= 61440;
+  if (h->pk_cheb) h->pk_coef = pk_cheb_coef(h->pk_lmin, h->pk_lmax);
   h->pk_ok = true;
   return VG_OK;
 }

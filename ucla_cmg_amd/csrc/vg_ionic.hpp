@@ -10,6 +10,12 @@
 
 namespace vg {
 
+// The per-node step functions are forced inline: as calls (the inliner
+// declines them once a kernel has two call sites of a large body) their state
+// arrays would live in scratch memory and every sub-step would save and
+// restore the callee's registers there.
+#define VG_INLINE inline __attribute__((always_inline))
+
 // ---- math shims --------------------------------------------------------------
 template <class T, bool FAST> struct Mth;
 template <bool FAST> struct Mth<double, FAST> {
@@ -147,7 +153,7 @@ constexpr int LR_TABLE_ROWS = 290001;
 // (LUORUDYWITHGATE) the 14 values inside (-90, 200) mV are interpolated
 // linearly from it (LuoRudyGateTable::get_parameters, LRGateTable.h:93-110).
 template <class T, bool FAST>
-__device__ inline T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T gna_scale,
+__device__ VG_INLINE T lr_compute_ion(T s[LR_NSTATE], T Cm, T dt, T v, T istim, T gna_scale,
                                    T gsi_scale, const T *__restrict__ table = nullptr) {
   using M = Mth<T, FAST>;
   const T c_K0 = T(5.4);
@@ -368,7 +374,7 @@ __device__ inline void tt_currents(const T *S, T istim_term, const TTG<T> &g, T 
 // Returns dV/dt; the caller advances V.
 // The fast FP32 form computes in RcpF (reciprocal divisions).
 template <class T, bool FAST>
-__device__ inline T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TTG<T> &g) {
+__device__ VG_INLINE T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TTG<T> &g) {
   using U = typename RateT<T, FAST>::type;
   using M = Mth<U, FAST>;
   U Su[TT_NSTATE];
@@ -397,38 +403,51 @@ __device__ inline T tt_step_rl(T S[TT_NSTATE], T V, T istim_term, T dt, const TT
 // 6x6 solve with partial pivoting of J d = B for the row-major array J
 // (J[j*6+i] = dg_j/dx_i), or, with transpose, of the system a column-major
 // LAPACK dgesv sees for that array (entry (r,c) = J[c*6+r]); B overwritten.
-template <class T> __device__ inline void tt_solve6(const T J[36], T B[6], bool transpose) {
+// Gaussian elimination with partial pivoting on a 6x6 system, fully unrolled:
+// the pivot row is swapped in by selects over the candidate rows, so the
+// matrix stays in registers (an index chosen at run time would put it in
+// scratch memory).  Same operations and pivot order as the plain loop.
+template <class T> __device__ VG_INLINE void tt_solve6(const T J[36], T B[6], bool transpose) {
   T a[6][6];
 #pragma unroll
   for (int r = 0; r < 6; r++)
 #pragma unroll
     for (int c = 0; c < 6; c++) a[r][c] = transpose ? J[c * 6 + r] : J[r * 6 + c];
+#pragma unroll
   for (int k = 0; k < 6; k++) {
     int p = k;
     T best = fabs(a[k][k]);
+#pragma unroll
     for (int r = k + 1; r < 6; r++)
       if (fabs(a[r][k]) > best) {
         best = fabs(a[r][k]);
         p = r;
       }
-    if (p != k) {
+#pragma unroll
+    for (int r = k + 1; r < 6; r++) {
+      const bool sw = r == p;
+#pragma unroll
       for (int c = 0; c < 6; c++) {
         const T t = a[k][c];
-        a[k][c] = a[p][c];
-        a[p][c] = t;
+        a[k][c] = sw ? a[r][c] : t;
+        a[r][c] = sw ? t : a[r][c];
       }
       const T t = B[k];
-      B[k] = B[p];
-      B[p] = t;
+      B[k] = sw ? B[r] : t;
+      B[r] = sw ? t : B[r];
     }
+#pragma unroll
     for (int r = k + 1; r < 6; r++) {
       const T l = a[r][k] / a[k][k];
+#pragma unroll
       for (int c = k + 1; c < 6; c++) a[r][c] -= l * a[k][c];
       B[r] -= l * B[k];
     }
   }
+#pragma unroll
   for (int k = 5; k >= 0; k--) {
     T s = B[k];
+#pragma unroll
     for (int c = k + 1; c < 6; c++) s -= a[k][c] * B[c];
     B[k] = s / a[k][k];
   }
@@ -441,7 +460,7 @@ template <class T> __device__ inline void tt_solve6(const T J[36], T B[6], bool 
 // (:172-181); by default the step solves J, which converges to the same
 // backward-Euler solution and stays physical through the upstroke.
 template <class T>
-__device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
+__device__ VG_INLINE T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
                                       const TTG<T> &g, int max_newton, bool jt, int *iters) {
   S[0] = V;
   T StateN[6], oldR[6], fxph[6], fxmh[6], J[36], B[6], R7[7];
@@ -464,22 +483,40 @@ __device__ inline T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
   T resid = T(100.);
   int it = 0;
   while (resid > T(1e-6) && it < max_newton) {
+    // Column i of the Jacobian.  The loop stays rolled (two current
+    // evaluations per pass); i is the same in every lane, and the state and
+    // column it picks are reached through unrolled tests of i, keeping S and
+    // J in registers.
     for (int i = 0; i < 6; i++) {
-      const int ki = TT_NR_STATE[i];
+      T xi = T(0);
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+        if (q == i) xi = S[TT_NR_STATE[q]];
       // 1e-3 x (Tusscher.cc:158); at x = 0 (0/0 there) a step of 1e-12.
-      const T eps = S[ki] != T(0) ? T(1e-3) * S[ki] : T(1e-12);
+      const T eps = xi != T(0) ? T(1e-3) * xi : T(1e-12);
       const T twoeps = T(2.) * eps;
-      S[ki] += eps;
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+        if (q == i) S[TT_NR_STATE[q]] += eps;
       tt_currents<T, false, false>(S, istim_term, g, R7);
 #pragma unroll
       for (int j = 0; j < 6; j++) fxph[j] = S[TT_NR_STATE[j]] - StateN[j] - dt * R7[1 + j];
-      S[ki] -= T(2) * eps;
+#pragma unroll
+      for (int q = 0; q < 6; q++)
+        if (q == i) S[TT_NR_STATE[q]] -= T(2) * eps;
       tt_currents<T, false, false>(S, istim_term, g, R7);
 #pragma unroll
       for (int j = 0; j < 6; j++) fxmh[j] = S[TT_NR_STATE[j]] - StateN[j] - dt * R7[1 + j];
-      S[ki] += eps;
 #pragma unroll
-      for (int j = 0; j < 6; j++) J[j * 6 + i] = (fxph[j] - fxmh[j]) / twoeps;
+      for (int q = 0; q < 6; q++)
+        if (q == i) S[TT_NR_STATE[q]] += eps;
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        const T d = (fxph[j] - fxmh[j]) / twoeps;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+          if (q == i) J[j * 6 + q] = d;
+      }
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) B[i] = S[TT_NR_STATE[i]] - StateN[i] - dt * oldR[i];
@@ -632,7 +669,7 @@ __device__ inline void pk_rates(const T S[PK_NSTATE], T stim, T R[PK_NSTATE]) {
 // |dV/dt| > 1.5 mV/ms; returns the last dV/dt evaluated.
 // The fast FP32 form computes the rates in RcpF (reciprocal divisions).
 template <class T, bool FAST>
-__device__ inline T pk_step(T S[PK_NSTATE], T V, T istim, T dt) {
+__device__ VG_INLINE T pk_step(T S[PK_NSTATE], T V, T istim, T dt) {
   using U = typename RateT<T, FAST>::type;
   S[0] = V;
   U Su[PK_NSTATE];
@@ -745,7 +782,7 @@ __device__ inline T mj_buffer(T c) {
 // Mahajan::ucla_rhsfun with the comp_* currents (Mahajan.cc:168-603): every
 // rate at gate step h, stimulus istim in uA/uF, tissue-type G_tof and G_Ks.
 template <class T, bool FAST, bool FAIL = false>
-__device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gks,
+__device__ VG_INLINE void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gks,
                                 T R[MJ_NSTATE]) {
   using M = Mth<T, FAST>;
   using K = MjK<FAIL>;
@@ -936,7 +973,7 @@ __device__ inline void mj_rates(const T S[MJ_NSTATE], T h, T istim, T gtof, T gk
 // returns the last dV/dt evaluated.
 // The fast FP32 form computes the rates in RcpF (reciprocal divisions).
 template <class T, bool FAST, bool FAIL = false>
-__device__ inline T mj_step(T S[MJ_NSTATE], double acc[MJ_NACC], T V, T istim, T dt, T gtof,
+__device__ VG_INLINE T mj_step(T S[MJ_NSTATE], double acc[MJ_NACC], T V, T istim, T dt, T gtof,
                             T gks, bool int_abs) {
   using U = typename RateT<T, FAST>::type;
   U Su[MJ_NSTATE], R[MJ_NSTATE];

@@ -697,6 +697,29 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_presolve(T *__restrict__ r, T *__
 // direction d (neighbours' d from LDS, double-buffered) and one barrier.
 // With [0.51, 1.49] (the LV tree) PK_ITERS = 6 iterations bound the error
 // reduction at 2 / T_6(theta / delta) ~ 1e-3.
+// The recurrence's coefficients depend on the interval only, so the host forms
+// them once per operator (pk_cheb_coef, in FP64 as the kernel would: IEEE
+// products and quotients round alike on both sides) and the kernel's
+// iterations carry no FP64 division, whose dependent chain per iteration had
+// cost more than the iteration's product and barrier.
+struct PkCheb {
+  double theta;                     // centre of the interval
+  double c0[PK_ITERS], c1[PK_ITERS];  // d <- c0[it] d + c1[it] r after iteration it
+};
+inline PkCheb pk_cheb_coef(double lmin, double lmax) {
+  PkCheb c{};
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  c.theta = theta;
+  double rho = 1.0 / sigma;
+  for (int it = 0; it < PK_ITERS; it++) {
+    const double rho_n = 1.0 / (2.0 * sigma - rho);
+    c.c0[it] = rho_n * rho;
+    c.c1[it] = 2.0 * rho_n / delta;
+    rho = rho_n;
+  }
+  return c;
+}
+
 template <class T>
 __global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__restrict__ x,
                                                     const T *__restrict__ blk_v,
@@ -704,12 +727,12 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__rest
                                                     int npk, const int64_t *__restrict__ cpl_row,
                                                     const T *__restrict__ cpl_v,
                                                     const int *__restrict__ cpl_j, int ncpl,
-                                                    double lmin, double lmax) {
+                                                    PkCheb cf) {
   extern __shared__ double pk_lds[];
   T *dl0 = reinterpret_cast<T *>(pk_lds), *dl1 = dl0 + npk;
   const int tid = threadIdx.x;
   constexpr int PK_ROWS = pk_rows<T>();
-  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  const double theta = cf.theta;
   T e[PK_ROWS], rr[PK_ROWS], d[PK_ROWS], ev[PK_ROWS][PK_MAXE];
   int ec[PK_ROWS][PK_MAXE];
 #pragma unroll
@@ -729,7 +752,9 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__rest
     }
   }
   __syncthreads();
-  double rho = 1.0 / sigma;
+  // Unrolled in FP32; rolled in FP64, whose unrolled form spills registers.
+  constexpr int PK_UNROLL = sizeof(T) == 8 ? 1 : PK_ITERS;
+#pragma unroll PK_UNROLL
   for (int it = 0; it < PK_ITERS; it++) {
     const T *dc = (it & 1) ? dl1 : dl0;
     T *dn = (it & 1) ? dl0 : dl1;
@@ -742,9 +767,7 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__rest
       rr[k] -= a;
     }
     if (it + 1 == PK_ITERS) break;
-    const double rho_n = 1.0 / (2.0 * sigma - rho);
-    const T c0 = (T)(rho_n * rho), c1 = (T)(2.0 * rho_n / delta);
-    rho = rho_n;
+    const T c0 = (T)cf.c0[it], c1 = (T)cf.c1[it];
 #pragma unroll
     for (int k = 0; k < PK_ROWS; k++) {
       const int i = tid + k * PK_BLK;
