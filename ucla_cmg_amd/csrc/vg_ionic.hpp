@@ -459,6 +459,10 @@ template <class T> __device__ VG_INLINE void tt_solve6(const T J[36], T B[6], bo
 // step with J^T as the reference's row-major/column-major mix-up does
 // (:172-181); by default the step solves J, which converges to the same
 // backward-Euler solution and stays physical through the upstroke.
+// In FP32 the absolute 1e-6 is below the states' rounding (K_i ~ 138 mM
+// has an ulp of 1.5e-5), so once Newton reaches that floor its steps only
+// wander and the loop would run to the cap: in FP32 it also ends at the
+// first pass whose step is no smaller than the one before.
 template <class T>
 __device__ VG_INLINE T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt,
                                       const TTG<T> &g, int max_newton, bool jt, int *iters) {
@@ -480,7 +484,7 @@ __device__ VG_INLINE T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt
   tt_currents<T, false, false>(S, istim_term, g, R7);
 #pragma unroll
   for (int i = 0; i < 6; i++) oldR[i] = R7[1 + i];
-  T resid = T(100.);
+  T resid = T(100.), prev = T(0);
   int it = 0;
   while (resid > T(1e-6) && it < max_newton) {
     // Column i of the Jacobian.  The loop stays rolled (two current
@@ -531,6 +535,10 @@ __device__ VG_INLINE T tt_step_reference(T S[TT_NSTATE], T V, T istim_term, T dt
     for (int i = 0; i < 6; i++) resid += B[i] * B[i];
     resid = sqrt(resid);
     it++;
+    if constexpr (sizeof(T) == 4) {
+      if (it > 1 && !(resid < prev)) break;
+      prev = resid;
+    }
   }
   *iters = it;
   return R7[0];
