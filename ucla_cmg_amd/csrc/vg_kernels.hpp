@@ -1226,12 +1226,16 @@ __device__ inline bf16x8 sg_frag(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t
   return __builtin_bit_cast(bf16x8, u32x4{d0, d1, d2, d3});
 }
 
-// acc[nt] = W^T (tiles of 32 units) times the KS operand fragments b.
+// acc[nt] = bias + W^T (tiles of 32 units) times the KS operand fragments b:
+// the accumulators start at the bias of the units lane half h holds (unit
+// 32 nt + 8 (j / 4) + 4 h + j % 4 for result j), so no add follows.
 template <int KS, int NT>
 __device__ inline void surr_mm(const __bf16 *W, int ldw, int r, int kq, const bf16x8 (&b)[8],
-                               f32x16 (&acc)[4]) {
+                               const float *bias, int h, f32x16 (&acc)[4]) {
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) acc[nt] = f32x16{};
+  for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[nt][j] = bias[nt * 32 + 8 * (j >> 2) + 4 * h + (j & 3)];
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
 #pragma unroll
@@ -1245,25 +1249,28 @@ __device__ inline void surr_mm(const __bf16 *W, int ldw, int r, int kq, const bf
   }
 }
 
-// Bias + SiLU of a hidden layer, handed over as the next layer's fragments.
-__device__ inline void surr_hand_over(const f32x16 (&acc)[4], const float *bias, int h,
-                                      bf16x8 (&b)[8]) {
+// SiLU of a hidden layer (bias already in acc), handed over as the next
+// layer's fragments.
+// For k step 2 nt + s, lane half 0 needs the units of g = 2 s (its own four,
+// then its partner's), half 1 those of g = 2 s + 1 (its partner's, then its
+// own).  v_permlane32_swap exchanges the upper 32 lanes of its first operand
+// with the lower 32 of its second: with P (g = 2 s) first and Q (g = 2 s + 1)
+// second, every lane then holds its fragment as {P', Q'}, with no select on
+// the lane half and no LDS-routed shuffle.
+__device__ inline void surr_hand_over(const f32x16 (&acc)[4], bf16x8 (&b)[8]) {
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
     float a[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++)
-      a[j] = sg_silu(acc[nt][j] + bias[nt * 32 + 8 * (j >> 2) + 4 * h + (j & 3)]);
+    for (int j = 0; j < 16; j++) a[j] = sg_silu(acc[nt][j]);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
-      // Own values: g = 2 s + h; the partner needs g = 2 s + 1 - h.
-      const int go = 4 * (2 * s), g1 = 4 * (2 * s + 1);
-      const uint32_t own0 = h ? sg_pack2(a[g1], a[g1 + 1]) : sg_pack2(a[go], a[go + 1]);
-      const uint32_t own1 = h ? sg_pack2(a[g1 + 2], a[g1 + 3]) : sg_pack2(a[go + 2], a[go + 3]);
-      const uint32_t snd0 = h ? sg_pack2(a[go], a[go + 1]) : sg_pack2(a[g1], a[g1 + 1]);
-      const uint32_t snd1 = h ? sg_pack2(a[go + 2], a[go + 3]) : sg_pack2(a[g1 + 2], a[g1 + 3]);
-      const uint32_t rcv0 = __shfl_xor(snd0, 32, 64), rcv1 = __shfl_xor(snd1, 32, 64);
-      b[2 * nt + s] = h ? sg_frag(rcv0, rcv1, own0, own1) : sg_frag(own0, own1, rcv0, rcv1);
+      const int gp = 8 * s, gq = 8 * s + 4;
+      const auto w0 = __builtin_amdgcn_permlane32_swap(sg_pack2(a[gp], a[gp + 1]),
+                                                       sg_pack2(a[gq], a[gq + 1]), false, false);
+      const auto w1 = __builtin_amdgcn_permlane32_swap(sg_pack2(a[gp + 2], a[gp + 3]),
+                                                       sg_pack2(a[gq + 2], a[gq + 3]), false, false);
+      b[2 * nt + s] = sg_frag(w0[0], w1[0], w0[1], w1[1]);
     }
   }
 }
@@ -1274,19 +1281,22 @@ constexpr int SG_OUT_SLOTS = 12;
 // States clamped to [0, 1] after the increment: the gates and R'.
 constexpr uint32_t SG_UNIT_STATES = (0x3Fu << 4) | (0x3Fu << 11) | (1u << 18);
 
-// Checks the real units of lane half h against their envelope (oc rows:
+// Checks the real units of lane half h (acc holds their bias) against their
+// envelope (oc rows:
 // t_lo, t_hi, mu_y, sd_y, c_y) and turns every slot into an increment
 // y[j] (padding units: c_y = 0, so 0).  The half is a runtime index, not a
 // template argument: both halves of a wave then run one instruction stream
 // instead of each half's copy under its own mask.
-__device__ inline bool surr_outputs(const f32x16 &acc, const float *bias, const float *oc, int h,
+__device__ inline bool surr_outputs(const f32x16 &acc, const float *oc, int h,
                                     float (&y)[SG_OUT_SLOTS]) {
+  // Bitwise, not short-circuit: one instruction stream, no branch per slot
+  // (u < 24 stays inside the 32-wide oc rows).
   bool ok = true;
 #pragma unroll
   for (int j = 0; j < SG_OUT_SLOTS; j++) {
     const int u = 8 * (j >> 2) + 4 * h + (j & 3);
-    const float t = acc[j] + bias[u];
-    ok = ok && (u >= SG_NOUT || (t >= oc[u] && t <= oc[32 + u]));
+    const float t = acc[j];
+    ok &= (u >= SG_NOUT) | ((t >= oc[u]) & (t <= oc[32 + u]));
     y[j] = sg_from_target(t, oc[64 + u], oc[96 + u], oc[128 + u]);
   }
   return ok;
@@ -1370,7 +1380,7 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
       ok = !(istim && istim[i] != T(0)) && !(f[0] > sd.up_v && f[SG_STATE_H] > sd.up_h) &&
            !(f[0] > sd.band_lo && f[0] < sd.band_hi);
 #pragma unroll
-      for (int k = 0; k < SG_NIN; k++) ok = ok && f[k] >= ic[k] && f[k] <= ic[32 + k];
+      for (int k = 0; k < SG_NIN; k++) ok &= (f[k] >= ic[k]) & (f[k] <= ic[32 + k]);
 #pragma unroll
       for (int k = 0; k < SG_NIN; k++) xn[k] = ok ? sg_norm_in(f[k], ic[64 + k], ic[96 + k]) : 0.0f;
     } else {
@@ -1389,17 +1399,21 @@ __global__ __launch_bounds__(SBLK) void k_ionic_surr(
                       sg_pack2(v[6], v[7]));
     }
     f32x16 acc[4];
-    surr_mm<SG_KIN / 16, 4>(wl + sg_wt_off(0), sg_wt_ld(0), r, kq, b, acc);
-    surr_hand_over(acc, bl, h, b);
+    surr_mm<SG_KIN / 16, 4>(wl + sg_wt_off(0), sg_wt_ld(0), r, kq, b, bl, h, acc);
+    surr_hand_over(acc, b);
 #pragma unroll 1
     for (int L = 1; L < SG_LAYERS - 1; L++) {
-      surr_mm<8, 4>(wl + sg_wt_off(L), sg_wt_ld(L), r, kq, b, acc);
-      surr_hand_over(acc, bl + L * SG_HID, h, b);
+      surr_mm<8, 4>(wl + sg_wt_off(L), sg_wt_ld(L), r, kq, b, bl + L * SG_HID, h, acc);
+      surr_hand_over(acc, b);
     }
-    surr_mm<8, 1>(wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, b, acc);
+    surr_mm<8, 1>(wl + sg_wt_off(SG_LAYERS - 1), sg_wt_ld(SG_LAYERS - 1), r, kq, b,
+                  bl + (SG_LAYERS - 1) * SG_HID, h, acc);
     float y[SG_OUT_SLOTS];
-    const bool ok_out = surr_outputs(acc[0], bl + (SG_LAYERS - 1) * SG_HID, oc, h, y);
-    const int both = __shfl_xor((int)ok_out, 32, 64) & (int)ok_out;
+    const bool ok_out = surr_outputs(acc[0], oc, h, y);
+    // The partner half's flag: the swap hands lanes 0-31 the upper half's
+    // value in its second result and lanes 32-63 the lower half's in its first.
+    const auto sw = __builtin_amdgcn_permlane32_swap((unsigned)ok_out, (unsigned)ok_out, false, false);
+    const int both = (int)(h ? sw[0] : sw[1]) & (int)ok_out;
     ok = ok && both;
     if (mine && ok) surr_store<T>(y, V, i, n_alloc, h, Vout, st);
     if (mine && h == 0) {
