@@ -1212,7 +1212,9 @@ __global__ __launch_bounds__(BLK) void k_ionic_pk(const T *__restrict__ Vin, con
 // (g = 2 s + h), four its partner l ^ 32 holds (same g), so a layer hands
 // over with one exchange of four BF16 per k step and no LDS traffic.
 
-constexpr int SBLK = 512;             // 8 waves, two per SIMD
+// 12 waves, three per SIMD: the weights' LDS image (~132 KB) admits one block
+// per CU, so the block's size sets the waves resident (166 VGPRs fit three).
+constexpr int SBLK = 768;
 constexpr int SWAVES = SBLK / 64;
 constexpr int SG_BIAS_TOTAL = (SG_LAYERS - 1) * SG_HID + SG_NOUTP;
 
@@ -1244,7 +1246,7 @@ __device__ inline void surr_mm(const __bf16 *W, int ldw, int r, int kq, const bf
       acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[ks], acc[nt], 0, 0, 0);
     }
     // Keep the weight fragments of one k step in flight at a time (the
-    // registers go to a second wave per SIMD instead).
+    // registers go to more waves per SIMD instead).
     __builtin_amdgcn_sched_barrier(0);
   }
 }
