@@ -1218,9 +1218,13 @@ constexpr int SBLK = 768;
 constexpr int SWAVES = SBLK / 64;
 constexpr int SG_BIAS_TOTAL = (SG_LAYERS - 1) * SG_HID + SG_NOUTP;
 
+// Two floats to BF16 (round to nearest even), a in the low half: one
+// two-operand v_cvt_pk_bf16_f32 (converting each float on its own costs a
+// conversion each plus a shift and an or).
 __device__ inline uint32_t sg_pack2(float a, float b) {
-  const __bf16 x = (__bf16)a, y = (__bf16)b;
-  return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
 __device__ inline bf16x8 sg_frag(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
