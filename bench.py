@@ -182,6 +182,17 @@ def cpu_baseline(mesh, scales, pidx, V, ragged, k_state, seconds_all=12.0, secon
     }
 
 
+# --value-streaming: how the CG reads the operator's values (-1: the library's
+# choice by operator size; 0 plain loads; 1 nontemporal loads).
+VALUE_STREAMING = -1
+
+
+def _streaming(solver):
+    if VALUE_STREAMING != -1:
+        solver.setValueStreaming(VALUE_STREAMING)
+    return solver
+
+
 def sweep_bench(vg, torch, steps, tune):
     """The batched sweep: 96 ten Tusscher slabs, 6.0k-236k nodes each
     (log-uniform, 6.62M in all), each with its own G_Na/G_Kr/G_CaL scaling
@@ -197,6 +208,7 @@ def sweep_bench(vg, torch, steps, tune):
                               integrator=vg.INTEG_RUSH_LARSEN, dt=0.02, D=(dl, dl, dl),
                               cg_tol=1e-7, param_scales=scales, param_index=inst.astype(np.uint16),
                               instance=inst, cg_x0=vg.X0_LAST)
+    _streaming(solver)
     stim = np.concatenate([meshgen.slab_corner_nodes(*m.meta["shape"], 3) + offs[k]
                            for k, m in enumerate(meshes)]).astype(np.int32)
     solver.enableActivation(-40.0)
@@ -230,6 +242,7 @@ def sweep_bench(vg, torch, steps, tune):
         "steps": steps,
         "instances": len(meshes),
         "nodes": n,
+        "value_streaming": solver.valueStreaming(),
         "instance_nodes_min_max": [int(min(m.n_nodes for m in meshes)),
                                    int(max(m.n_nodes for m in meshes))],
         "cg_iters_per_instance_solve": (st1["cg_iterations"] - st0["cg_iterations"]) /
@@ -391,6 +404,7 @@ def mahajan_bench(args, vg):
                          integrator=vg.INTEG_REFERENCE, D=(sigma, sigma, sigma), dt=dt,
                          dt_ionic=dt, cg_tol=1e-7, cg_x0=vg.X0_LAST, graph=args.mj_graph,
                          row_order=args.row_order, cg_blocks=args.cg_blocks)
+    _streaming(s)
     setup_s = time.perf_counter() - t0
     ring = meshgen.electrodes_around((3.0, 3.0, 0.5), 5.0, 8)
     el = np.vstack([ring, [[3.0, 3.0, 3.0], [3.0, 3.0, -2.0]]])
@@ -445,6 +459,7 @@ def mahajan_bench(args, vg):
         "cg_not_converged": st["cg_not_converged"], "graph": bool(args.mj_graph),
         "row_order": {1: "mesh", 2: "geometric"}.get(st["row_order"], st["row_order"]),
         "sell_wide_slices": st["sell_wide_slices"], "cg_blocks_used": s.cgBlocks(),
+        "value_streaming": s.valueStreaming(),
         "activated_frac": float((at >= 0).mean()),
         "phase_singularities": {"series_ms_count_charge": ps[::4],
                                 "max": max((p[1] for p in ps), default=0),
@@ -850,6 +865,8 @@ def main():
                     help="initial guess of each half-step solve (same stopping bound)")
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
                     help="replay each step as a captured hipGraph (no host syncs)")
+    ap.add_argument("--value-streaming", type=int, default=-1, choices=[-1, 0, 1],
+                    help="CG operator values: -1 library's choice by size, 0 plain, 1 nontemporal")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep --cg-blocks/--graph instead of autotuning them in the warm-up")
     ap.add_argument("--no-sweep", action="store_true",
@@ -903,6 +920,8 @@ def main():
     ap.add_argument("--purkinje", choices=["high", "low", "none"], default="high",
                     help="Purkinje tree with high/low PMJ resistance, or none")
     args = ap.parse_args()
+    global VALUE_STREAMING
+    VALUE_STREAMING = args.value_streaming
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -950,6 +969,7 @@ def main():
                                      "extrapolate": vg.X0_EXTRAPOLATE}[args.cg_x0],
                               graph=args.graph, row_order=args.row_order,
                               true_residual_every=TRUE_RESIDUAL_EVERY, **MODEL_KW)
+    _streaming(solver)
     solver.enableActivation(-40.0)
     solver.setupECG(electrodes10())
     solver.recordECG(50, int(BEAT_MS / 1.0) + (args.warmup + args.steps + 200) // 50 + 4)
@@ -1125,6 +1145,7 @@ def main():
             "cg_blocks": args.cg_blocks,
             "cg_x0": args.cg_x0,
             "cg_blocks_used": solver.cgBlocks(),
+            "value_streaming": solver.valueStreaming(),
             "graph": tuned["graph"] if tuned else bool(args.graph),
             "row_order": {1: "mesh", 2: "geometric"}.get(solver.stats()["row_order"]),
             "parallelism": "single GPU",

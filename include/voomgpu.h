@@ -266,6 +266,13 @@ int vg_set_true_residual_check(vg_handle *h, int every);
 int vg_set_cg_blocks(vg_handle *h, int cg_blocks);
 int vg_get_cg_blocks(vg_handle *h, int *cg_blocks);
 int vg_set_graph(vg_handle *h, int enable);
+/* How the CG's kernels read the operator's values: -1 the library's choice
+ * (nontemporal loads when one iteration's streams exceed the 256 MB Infinity
+ * Cache), 0 plain loads, 1 nontemporal loads.  Results are the same bit for
+ * bit either way; only the cache policy differs.  No reference counterpart:
+ * the reference has no device caches to steer. */
+int vg_set_value_streaming(vg_handle *h, int mode);
+int vg_get_value_streaming(vg_handle *h, int *on);
 
 int vg_get_voltage(vg_handle *h, double *V);
 int vg_set_voltage(vg_handle *h, const double *V);

@@ -134,6 +134,32 @@ def test_mixed_reference_integrator_tt_activation(gpu_device):
     assert passes <= 4.0
 
 
+def test_nontemporal_value_loads_give_the_same_bits(gpu_device):
+    """The CG's nontemporal value loads (the library's choice for operators
+    beyond the Infinity Cache) change only the cache policy: forced on for a
+    small slab, the run equals the default one bit for bit, in mixed precision
+    from the last increment and in FP64 from zero (both right-hand-side
+    forms)."""
+    nx, ny, nz = 17, 15, 4
+    mesh = meshgen.slab(nx, ny, nz, 0.025)
+    stim = meshgen.slab_corner_nodes(nx, ny, nz, 3)
+    for kw in (dict(precision=vg.PREC_MIXED, integrator=vg.INTEG_RUSH_LARSEN, cg_x0=vg.X0_LAST),
+               dict(precision=vg.PREC_FP64, integrator=vg.INTEG_REFERENCE, cg_x0=vg.X0_ZERO,
+                    cg_tol=1e-10)):
+        out = []
+        for mode in (-1, 1):
+            g = vg.CardiacSolver(mesh, cell_model=vg.MODEL_TT, dt=0.02, D=D_SLAB, **kw)
+            assert not g.valueStreaming()  # a small operator: plain loads
+            g.setValueStreaming(mode)
+            assert g.valueStreaming() == (mode == 1)
+            g.setStimulusCurrent(stim, 50.0 * 2000.0)
+            g.Solve(40)
+            out.append((g.getVoltage(), g.stats()["cg_iterations"]))
+            g.close()
+        assert out[0][1] == out[1][1]
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
 def test_ecg_matches_direct_quadrature(gpu_device):
     mesh = meshgen.slab(9, 7, 3, 0.05)
     gpu = vg.CardiacSolver(mesh, cell_model=vg.MODEL_LR, precision=vg.PREC_FP64, dt=0.01, D=D_SLAB)

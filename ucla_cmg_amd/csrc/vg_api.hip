@@ -201,6 +201,9 @@ struct vg_handle {
   bool pk_cheb = false;
   double pk_lmin = 0.0, pk_lmax = 0.0;
   PkCheb pk_coef{};  // Chebyshev coefficients of [pk_lmin, pk_lmax]
+  // Operator values read with nontemporal loads (see ld_vals): chosen when a
+  // CG iteration's streams exceed the Infinity Cache, or set by the caller.
+  bool nt_vals = false;
   DevBuf pk_blk_v, pk_blk_j, pk_cpl_row, pk_cpl_v, pk_cpl_j;
   DevBuf xhist;
   int solves_done[2] = {0, 0};
@@ -415,19 +418,24 @@ void launch_pk_presolve(vg_handle *h, int which) {
                        h->pk_cpl_v.as<T>(), h->pk_cpl_j.as<int>(), h->pk_ncpl);
 }
 
+// Nontemporal value loads pay when one CG iteration's streams -- the operator's
+// stored values, the row records read and written, x, diag -- exceed the
+// 256 MB Infinity Cache (the 4.6M-row slab: ~440 MB, 5.6 % faster); below it
+// they keep the values from staying resident there (the 1.03M-row LV: ~100 MB,
+// 19 % slower).
+bool nt_vals_auto(const vg_handle *h) {
+  const double bytes = (double)h->tsz * ((double)h->padded + 11.0 * (double)h->rows);
+  return bytes > 256.0 * 1024.0 * 1024.0;
+}
+
 template <class T>
 void launch_rhs(vg_handle *h, const T *V, int which) {
   ProfScope ps(h, VG_KC_RHS);
-  if (h->guess)
-    hipLaunchKernelGGL((k_rhs<T, true>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                       sell_of(h), h->Krhs.as<T>(),
-                       h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
-                       cgbufs_of(h));
-  else
-    hipLaunchKernelGGL((k_rhs<T, false>), dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
-                       sell_of(h), h->Krhs.as<T>(),
-                       h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(), h->diag.as<T>(),
-                       cgbufs_of(h));
+  auto kern = h->guess ? (h->nt_vals ? k_rhs<T, true, true> : k_rhs<T, true, false>)
+                       : (h->nt_vals ? k_rhs<T, false, true> : k_rhs<T, false, false>);
+  hipLaunchKernelGGL(kern, dim3(h->cg_blocks), dim3(BLK), 0, h->stream, sell_of(h),
+                     h->Krhs.as<T>(), h->Ahat.as<T>(), V, xbuf<T>(h, which), h->r.as<T>(),
+                     h->diag.as<T>(), cgbufs_of(h));
 }
 
 // The solve of half-step `which` turns the other half-step's history into
@@ -442,7 +450,8 @@ void launch_cg_start(vg_handle *h, int which) {
     hist = XHist<T>{xbuf<T>(h, o), base, base + h->n_alloc,
                     std::min(h->extrap_order, h->solves_done[o])};
   }
-  hipLaunchKernelGGL(k_cg_start<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
+  hipLaunchKernelGGL((h->nt_vals ? k_cg_start<T, true> : k_cg_start<T, false>),
+                     dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
                      h->r.as<T>(), h->q0.as<typename Vec4Of<T>::type>(), h->diag.as<T>(),
                      cgbufs_of(h), hist);
@@ -451,7 +460,8 @@ void launch_cg_start(vg_handle *h, int which) {
 // (Profiled per solve, around all its iteration launches: see cg_solve.)
 template <class T>
 void launch_cg_iter(vg_handle *h, int j, int which) {
-  hipLaunchKernelGGL(k_cg_iter<T>, dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
+  hipLaunchKernelGGL((h->nt_vals ? k_cg_iter<T, true> : k_cg_iter<T, false>),
+                     dim3(h->cg_blocks), dim3(BLK), 0, h->stream,
                      sell_of(h), h->Ahat.as<T>(),
                      h->q0.as<typename Vec4Of<T>::type>(), h->q1.as<typename Vec4Of<T>::type>(),
                      xbuf<T>(h, which), h->diag.as<T>(),
@@ -1383,6 +1393,7 @@ int vg_create(const vg_mesh *mesh, const vg_model *model, vg_handle **out) {
   h->n_wide = std::count_if(sell.wide.begin(), sell.wide.end(), [](int64_t w) { return w >= 0; });
   h->n_uniform = std::count_if(sell.ustart.begin(), sell.ustart.end(), [](int64_t u) { return u >= 0; });
   h->n_alloc = ((h->rows + BLK - 1) / BLK) * BLK;
+  h->nt_vals = nt_vals_auto(h.get());
   h->new2old = sell.new2old;
   h->old2new = sell.old2new;
   h->CL = csr.CL;
@@ -1710,6 +1721,27 @@ int vg_set_graph(vg_handle *h, int enable) {
   // Queued replays end before a host-driven solve resets the published record.
   if ((enable ? 1 : 0) != h->graph) HIPCHK(h, hipStreamSynchronize(h->stream));
   h->graph = enable ? 1 : 0;
+  return VG_OK;
+}
+
+int vg_set_value_streaming(vg_handle *h, int mode) {
+  if (!h || mode < -1 || mode > 1) return VG_ERR_ARG;
+  const bool nt = mode < 0 ? nt_vals_auto(h) : mode == 1;
+  if (nt == h->nt_vals) return VG_OK;
+  // Launches already queued keep their kernels; a captured step holds the
+  // old ones, so it is captured again.
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->nt_vals = nt;
+  if (h->gexec) {
+    HIPCHK(h, hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
+  return VG_OK;
+}
+
+int vg_get_value_streaming(vg_handle *h, int *on) {
+  if (!h || !on) return VG_ERR_ARG;
+  *on = h->nt_vals ? 1 : 0;
   return VG_OK;
 }
 

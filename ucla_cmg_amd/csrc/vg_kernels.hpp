@@ -262,7 +262,29 @@ struct Col4Of<int> {
 // lane of a slice whose values start at vs and columns at cs; colmap turns a
 // stored column into a row index.  Two groups (8 entries) are loaded before
 // their gathers, so each wave keeps two wide loads and 8 gathers in flight.
-template <class T, class C, class CM, class G>
+// A wide load of the operator's values.  NT: a nontemporal load, for
+// operators larger than the Infinity Cache, whose values are read once per
+// launch: the hint keeps them from displacing the row records the gathers
+// reuse (on the 4.6M-row slab 4.28 -> 4.04 ms/step; on a cache-resident
+// operator the same hint costs 19 %, so it is chosen per operator).
+template <bool NT, class V4>
+__device__ inline V4 ld_vals(const V4 *p) {
+  if constexpr (NT) {
+    typedef decltype(p->x) E;
+    typedef E ev4 __attribute__((ext_vector_type(4)));
+    const ev4 t = __builtin_nontemporal_load(reinterpret_cast<const ev4 *>(p));
+    return V4{t.x, t.y, t.z, t.w};
+  } else {
+    return *p;
+  }
+}
+template <bool NT, class E>
+__device__ inline E ld_val(const E *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <class T, bool NT = false, class C, class CM, class G>
 __device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restrict__ cs, int lane,
                              CM colmap, G gather) {
   using V4 = typename Vec4Of<T>::type;
@@ -277,7 +299,7 @@ __device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restri
     C4 c[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-      v[q] = vp[(g + q) * 64];
+      v[q] = ld_vals<NT>(vp + (g + q) * 64);
       c[q] = cp[(g + q) * 64];
     }
 #pragma unroll
@@ -289,7 +311,7 @@ __device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restri
     }
   }
   if (g < n4) {
-    const V4 v = vp[g * 64];
+    const V4 v = ld_vals<NT>(vp + g * 64);
     const C4 c = cp[g * 64];
     acc += v.x * gather(colmap(c.x));
     acc += v.y * gather(colmap(c.y));
@@ -305,7 +327,7 @@ __device__ inline T sell_dot(int wd, const T *__restrict__ vs, const C *__restri
   for (int q = 0; q < 3; q++)
     if (q < rem) {
       cc[q] = colmap(ct[q * 64]);
-      vv[q] = vt[q * 64];
+      vv[q] = ld_val<NT>(vt + q * 64);
     }
 #pragma unroll
   for (int q = 0; q < 3; q++)
@@ -329,7 +351,7 @@ __device__ inline int64_t wave_uniform(int64_t v) {
 // groups' values and gathers in flight at once (LV, CG iteration: 24.8 us
 // with two at a time, 22.8 us with three; a separate two-group tail, or the
 // chunk as a function template, measured 23.8 us).
-template <class T, class G>
+template <class T, bool NT = false, class G>
 __device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__restrict__ uo,
                                int lane, int row, G gather) {
   using V4 = typename Vec4Of<T>::type;
@@ -344,7 +366,7 @@ __device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__re
     int4 c[NG];
 #pragma unroll
     for (int q = 0; q < NG; q++) {
-      v[q] = vp[(g + q) * 64];
+      v[q] = ld_vals<NT>(vp + (g + q) * 64);
       c[q] = up[g + q];
     }
 #pragma unroll
@@ -356,7 +378,7 @@ __device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__re
     }
   }
   for (; g < n4; g++) {
-    const V4 v = vp[g * 64];
+    const V4 v = ld_vals<NT>(vp + g * 64);
     const int4 c = up[g];
     acc += v.x * gather(row + c.x);
     acc += v.y * gather(row + c.y);
@@ -371,7 +393,7 @@ __device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__re
     T vv[3];
 #pragma unroll
     for (int q = 0; q < 3; q++)
-      if (q < rem) vv[q] = vt[q * 64];
+      if (q < rem) vv[q] = ld_val<NT>(vt + q * 64);
 #pragma unroll
     for (int q = 0; q < 3; q++)
       if (q < rem) acc += vv[q] * gather(row + cc[q]);
@@ -380,25 +402,25 @@ __device__ inline T sell_dot_u(int wd, const T *__restrict__ vs, const int *__re
 }
 
 // One lane's row of a slice: sum_j val[j] * gather(col[j]).
-template <class T, class G>
+template <class T, bool NT = false, class G>
 __device__ inline T sell_row(const Sell &S, const T *__restrict__ val, int64_t slice, int lane,
                              G gather) {
   slice = wave_uniform(slice);
   const int64_t o = S.off[slice];
   const int wd = S.width[slice];
   const int64_t us = S.ustart[slice];
-  if (us >= 0) return sell_dot_u<T>(wd, val + o, S.uoff + us, lane, (int)(slice * 64 + lane), gather);
+  if (us >= 0) return sell_dot_u<T, NT>(wd, val + o, S.uoff + us, lane, (int)(slice * 64 + lane), gather);
   const int64_t wo = S.wide[slice];
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
-    return sell_dot<T>(wd, val + o, S.col16 + o, lane, [&](int c) { return row + c; }, gather);
+    return sell_dot<T, NT>(wd, val + o, S.col16 + o, lane, [&](int c) { return row + c; }, gather);
   }
-  return sell_dot<T>(wd, val + o, S.col32 + wo, lane, [](int c) { return c; }, gather);
+  return sell_dot<T, NT>(wd, val + o, S.col32 + wo, lane, [](int c) { return c; }, gather);
 }
 
 // Two operators on one pattern in one pass (columns loaded once):
 // acc_a = sum va[j] ga(col[j]), acc_b = sum vb[j] gb(col[j]).
-template <class T, class C, class CM, class GA, class GB>
+template <class T, bool NT = false, class C, class CM, class GA, class GB>
 __device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__restrict__ bs,
                                  const C *__restrict__ cs, int lane, CM colmap, GA ga, GB gb,
                                  T &acc_a, T &acc_b) {
@@ -410,7 +432,7 @@ __device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__re
   const C4 *__restrict__ cp = reinterpret_cast<const C4 *>(cs) + lane;
   T sa = T(0), sb = T(0);
   for (int g = 0; g < n4; g++) {
-    const V4 x = ap[g * 64], y = bp[g * 64];
+    const V4 x = ld_vals<NT>(ap + g * 64), y = ld_vals<NT>(bp + g * 64);
     const C4 c4 = cp[g * 64];
     const int c[4] = {colmap(c4.x), colmap(c4.y), colmap(c4.z), colmap(c4.w)};
     const T xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
@@ -428,8 +450,8 @@ __device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__re
   for (int q = 0; q < 3; q++)
     if (q < rem) {
       c[q] = colmap(cs[t + q * 64]);
-      x[q] = as[t + q * 64];
-      y[q] = bs[t + q * 64];
+      x[q] = ld_val<NT>(as + t + q * 64);
+      y[q] = ld_val<NT>(bs + t + q * 64);
     }
 #pragma unroll
   for (int q = 0; q < 3; q++)
@@ -442,7 +464,7 @@ __device__ inline void sell_dot2(int wd, const T *__restrict__ as, const T *__re
 }
 
 // sell_dot2 for a shared-offset slice (see sell_dot_u).
-template <class T, class GA, class GB>
+template <class T, bool NT = false, class GA, class GB>
 __device__ inline void sell_dot2_u(int wd, const T *__restrict__ as, const T *__restrict__ bs,
                                    const int *__restrict__ uo, int lane, int row, GA ga, GB gb,
                                    T &acc_a, T &acc_b) {
@@ -453,7 +475,7 @@ __device__ inline void sell_dot2_u(int wd, const T *__restrict__ as, const T *__
   const int4 *__restrict__ up = reinterpret_cast<const int4 *>(uo);
   T sa = T(0), sb = T(0);
   for (int g = 0; g < n4; g++) {
-    const V4 x = ap[g * 64], y = bp[g * 64];
+    const V4 x = ld_vals<NT>(ap + g * 64), y = ld_vals<NT>(bp + g * 64);
     const int4 c4 = up[g];
     const int c[4] = {row + c4.x, row + c4.y, row + c4.z, row + c4.w};
     const T xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
@@ -472,8 +494,8 @@ __device__ inline void sell_dot2_u(int wd, const T *__restrict__ as, const T *__
 #pragma unroll
     for (int q = 0; q < 3; q++)
       if (q < rem) {
-        x[q] = as[t + q * 64];
-        y[q] = bs[t + q * 64];
+        x[q] = ld_val<NT>(as + t + q * 64);
+        y[q] = ld_val<NT>(bs + t + q * 64);
       }
 #pragma unroll
     for (int q = 0; q < 3; q++)
@@ -486,7 +508,7 @@ __device__ inline void sell_dot2_u(int wd, const T *__restrict__ as, const T *__
   acc_b = sb;
 }
 
-template <class T, class GA, class GB>
+template <class T, bool NT = false, class GA, class GB>
 __device__ inline void sell_row2(const Sell &S, const T *__restrict__ va, const T *__restrict__ vb,
                                  int64_t slice, int lane, GA ga, GB gb, T &acc_a, T &acc_b) {
   slice = wave_uniform(slice);
@@ -494,17 +516,17 @@ __device__ inline void sell_row2(const Sell &S, const T *__restrict__ va, const 
   const int wd = S.width[slice];
   const int64_t us = S.ustart[slice];
   if (us >= 0) {
-    sell_dot2_u<T>(wd, va + o, vb + o, S.uoff + us, lane, (int)(slice * 64 + lane), ga, gb, acc_a,
+    sell_dot2_u<T, NT>(wd, va + o, vb + o, S.uoff + us, lane, (int)(slice * 64 + lane), ga, gb, acc_a,
                    acc_b);
     return;
   }
   const int64_t wo = S.wide[slice];
   if (wo < 0) {
     const int row = (int)(slice * 64 + lane);
-    sell_dot2<T>(wd, va + o, vb + o, S.col16 + o, lane, [&](int c) { return row + c; }, ga, gb,
+    sell_dot2<T, NT>(wd, va + o, vb + o, S.col16 + o, lane, [&](int c) { return row + c; }, ga, gb,
                  acc_a, acc_b);
   } else {
-    sell_dot2<T>(wd, va + o, vb + o, S.col32 + wo, lane, [](int c) { return c; }, ga, gb, acc_a,
+    sell_dot2<T, NT>(wd, va + o, vb + o, S.col32 + wo, lane, [](int c) { return c; }, ga, gb, acc_a,
                  acc_b);
   }
 }
@@ -808,7 +830,7 @@ __global__ __launch_bounds__(PK_BLK) void k_pk_cheb(T *__restrict__ r, T *__rest
 // noise of ~1e-7 |K_ii| 85 mV in every row, and in a tissue mostly at rest the
 // noise of 1M rows outweighs the wavefront's b, so a solve to 1e-7 ||b|| had
 // to resolve the noise.
-template <class T, bool GUESS>
+template <class T, bool GUESS, bool NT = false>
 __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
                                              const T *__restrict__ Ahat,
                                              const T *__restrict__ V, const T *__restrict__ x,
@@ -835,14 +857,14 @@ __global__ __launch_bounds__(BLK) void k_rhs(Sell S, const T *__restrict__ Krhs,
     if constexpr (GUESS) {
       T bk, ax;
       const T xi = x[row];
-      sell_row2<T>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c] - vi; },
+      sell_row2<T, NT>(S, Krhs, Ahat, sl, lane, [&](int c) { return V[c] - vi; },
                    [&](int c) { return x[c]; }, bk, ax);
       const T rr = bk - (xi + ax);  // A_hat's unit diagonal
       r[row] = rr;
       v[0] += d * (double)bk * (double)bk;
       v[1] += d * (double)rr * (double)rr;
     } else {
-      const T bk = sell_row<T>(S, Krhs, sl, lane, [&](int c) { return V[c] - vi; });
+      const T bk = sell_row<T, NT>(S, Krhs, sl, lane, [&](int c) { return V[c] - vi; });
       r[row] = bk;
       const double e = d * (double)bk * (double)bk;
       v[0] += e;
@@ -880,7 +902,7 @@ __device__ inline void rotate_history(const XHist<T> &h, int64_t row) {
 // is zero); w_0 = A_hat r_0, the records {r_0, w_0, 0, 0} of parity 0, and
 // partials of (r_0, r_0), (r_0, w_0), ||r_0||_D^2.  With hist.order > 0 it
 // also rotates the other half-step's increment history (row-local).
-template <class T>
+template <class T, bool NT = false>
 __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ val,
                                                   const T *__restrict__ r,
                                                   typename Vec4Of<T>::type *__restrict__ q,
@@ -912,7 +934,7 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
   for (int64_t sl = lo + (threadIdx.x >> 6); sl < hi; sl += step) {
     const int64_t row = sl * 64 + lane;
     const T ri = r[row];
-    const T acc = ri + sell_row<T>(S, val, sl, lane, [&](int c) { return r[c]; });
+    const T acc = ri + sell_row<T, NT>(S, val, sl, lane, [&](int c) { return r[c]; });
     q[row] = {ri, acc, T(0), T(0)};
     const double rr = (double)ri;
     v[0] += rr * rr;
@@ -937,18 +959,18 @@ __global__ __launch_bounds__(BLK) void k_cg_start(Sell S, const T *__restrict__ 
 // record, so a neighbour costs one wide gather; launch j reads the records
 // {r_j, w_j, s_{j-1}, p_{j-1}} of parity j and writes {r_{j+1}, w_{j+1}, s_j,
 // p_j} to the other array, so no other block's writes of this launch are read.
-template <class T, bool FIRST>
+template <class T, bool FIRST, bool NT>
 __device__ inline T cg_iter_row(const Sell &S, const T *__restrict__ val,
                                 const typename Vec4Of<T>::type *__restrict__ q0, int64_t slice,
                                 int lane, T alpha, T beta) {
-  return sell_row<T>(S, val, slice, lane, [&](int c) {
+  return sell_row<T, NT>(S, val, slice, lane, [&](int c) {
     const typename Vec4Of<T>::type g = q0[c];
     const T sc = FIRST ? g.y : g.y + beta * g.z;
     return g.x - alpha * sc;
   });
 }
 
-template <class T>
+template <class T, bool NT = false>
 __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ val,
                                                  typename Vec4Of<T>::type *qA,
                                                  typename Vec4Of<T>::type *qB,
@@ -1012,12 +1034,12 @@ __global__ __launch_bounds__(BLK) void k_cg_iter(Sell S, const T *__restrict__ v
       pi = ri;
       si = wi;
       xi = guess ? x[row] + a * pi : a * pi;
-      acc = cg_iter_row<T, true>(S, val, q0, sl, lane, a, bt);
+      acc = cg_iter_row<T, true, NT>(S, val, q0, sl, lane, a, bt);
     } else {
       pi = ri + bt * own.w;
       si = wi + bt * own.z;
       xi = x[row] + a * pi;
-      acc = cg_iter_row<T, false>(S, val, q0, sl, lane, a, bt);
+      acc = cg_iter_row<T, false, NT>(S, val, q0, sl, lane, a, bt);
     }
     const T rn = ri - a * si;
     acc += rn;  // A_hat's unit diagonal

@@ -52,6 +52,7 @@ EXPORTED = [
     "vg_set_stimulus_nodes", "vg_set_stimulus_sphere", "vg_set_stimulus_axis",
     "vg_set_stimulus_plane", "vg_clear_stimulus", "vg_step", "vg_synchronize",
     "vg_set_cg_blocks", "vg_get_cg_blocks", "vg_set_graph", "vg_set_true_residual_check",
+    "vg_set_value_streaming", "vg_get_value_streaming",
     "vg_get_voltage", "vg_set_voltage", "vg_get_states", "vg_set_states",
     "vg_get_states_ragged", "vg_set_states_ragged",
     "vg_get_lumped_capacitance", "vg_activation_enable", "vg_get_activation",
@@ -145,6 +146,8 @@ def load_library(path: str = None):
         "vg_set_cg_blocks": ([H, C.c_int], C.c_int),
         "vg_get_cg_blocks": ([H, C.POINTER(C.c_int)], C.c_int),
         "vg_set_graph": ([H, C.c_int], C.c_int),
+        "vg_set_value_streaming": ([H, C.c_int], C.c_int),
+        "vg_get_value_streaming": ([H, C.POINTER(C.c_int)], C.c_int),
         "vg_set_true_residual_check": ([H, C.c_int], C.c_int),
         "vg_get_voltage": ([H, C.c_void_p], C.c_int),
         "vg_set_voltage": ([H, C.c_void_p], C.c_int),
@@ -464,6 +467,16 @@ class CardiacSolver:
         """FP64 true residual after both solves of every ``every``-th step
         (0 = off); the largest goes to stats()['max_true_relres']."""
         self._chk(self.L.vg_set_true_residual_check(self.h, int(every)), "set_true_residual_check")
+
+    def setValueStreaming(self, mode: int = -1):
+        """How the CG reads the operator's values: -1 the library's choice by
+        operator size, 0 plain loads, 1 nontemporal loads (same results)."""
+        self._chk(self.L.vg_set_value_streaming(self.h, int(mode)), "set_value_streaming")
+
+    def valueStreaming(self) -> bool:
+        on = C.c_int()
+        self._chk(self.L.vg_get_value_streaming(self.h, C.byref(on)), "get_value_streaming")
+        return bool(on.value)
 
     def cgBlocks(self) -> int:
         g = C.c_int()
