@@ -1479,9 +1479,11 @@ __global__ __launch_bounds__(BLK) void k_apply(const T *__restrict__ Vb, const T
   for (int e = 0; e < MAX_ECG; e++) v[e] = 0.0;
   // One instance: whether x holds an increment is the same for every row, so
   // it is read once, and four rows per pass keep their loads in flight
-  // together (the grid is capped, so each thread takes many rows).
+  // together (the grid is capped, so each thread takes many rows).  Not with
+  // the ECG sums, whose accumulators the unrolled form slowed down.
   const bool x_one = x && !slice_inst && cg[0].x_valid != 0;
-#pragma unroll 4
+  constexpr int APPLY_UNROLL = ECG ? 1 : 4;
+#pragma unroll APPLY_UNROLL
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
     const bool use_x = slice_inst ? (x && cg[slice_inst[i >> 6]].x_valid != 0) : x_one;
     const T Vn = use_x ? Vb[i] + sinv[i] * x[i] : Vb[i];
